@@ -1,0 +1,218 @@
+"""ctypes binding of librtm_hip.so (the C ABI declared in include/rtmlib_modC.h).
+
+Every function here runs on the GPU through the HIP library. There is no CPU
+fallback: when the library is missing or no HIP device is present the calls
+raise.
+
+Array conventions follow the reference layout (W/gpu.h:8-14):
+  field      numpy (nz, nx), Fortran order (index iz + nz*ix)
+  history    numpy (nz, nx, nt), Fortran order
+  receivers  numpy (5, nrx, nt), C order: component f, trace ix, sample it
+             (each component block is the reference (nt, nx) layout it + nt*ix)
+  coef5      list of the five (nz, nx) coefficient arrays m1_x, m1_z, aux_m2,
+             aux_m3, aux_m2m3 (already dt/dx scaled)
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "librtm_hip.so")
+
+RTM_OPT_ADJ_ABC, RTM_OPT_SNAP_EVERY, RTM_OPT_SNAP_PREC, RTM_OPT_COMP_IMAGE, RTM_OPT_REC_X0, RTM_OPT_REC_NX = range(1, 7)
+
+_LIB = None
+
+
+class RtmError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"librtm_hip error {code}: {msg}")
+        self.code = code
+
+
+class RtmShot(C.Structure):
+    """Mirror of rtm_shot_t."""
+
+    _fields_ = [
+        ("nz", C.c_int), ("nx", C.c_int), ("nt", C.c_int),
+        ("dx", C.c_float), ("dt", C.c_float),
+        ("xs", C.c_int), ("zs", C.c_int),
+        ("zr", C.c_int), ("rx0", C.c_int), ("nrx", C.c_int),
+        ("abc", C.c_int), ("adj_abc", C.c_int), ("snap_every", C.c_int),
+        ("snap_prec", C.c_int), ("comp_image", C.c_int),
+    ]
+
+
+def load():
+    """Load the in-tree HIP library; raise if it has not been built."""
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is missing: run `python -m elastic_amd.build` first")
+        _LIB = C.CDLL(LIB_PATH)
+        _LIB.rtm_hip_last_error.restype = C.c_char_p
+    return _LIB
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+def _chk(code):
+    if code != 0:
+        raise RtmError(code, load().rtm_hip_last_error().decode())
+
+
+def _f32F(a):
+    return np.asfortranarray(a, dtype=np.float32)
+
+
+def pack_fields(arrs):
+    """List of (nz, nx) arrays -> contiguous (k, nx, nz) float32 block whose
+    k-th slab is the k-th field in the reference layout."""
+    return np.ascontiguousarray(np.stack([np.asarray(a, np.float32).T for a in arrs]))
+
+
+def unpack_fields(block):
+    return [np.asfortranarray(block[k].T) for k in range(block.shape[0])]
+
+
+def shot(nz, nx, nt, dx, dt, xs, zs, zr, rx0=0, nrx=None, abc=0, adj_abc=0, snap_every=1,
+         snap_prec=0, comp_image=0):
+    if nrx is None:
+        nrx = nx - rx0
+    return RtmShot(nz, nx, nt, dx, dt, xs, zs, zr, rx0, nrx, abc, adj_abc, snap_every, snap_prec, comp_image)
+
+
+def rtm_op3(hist, coefs, zrec=0):
+    """Drop-in adjoint rtm_op3_c_ semantics (status-returning form) on copies."""
+    lib = load()
+    h = [_f32F(a).copy(order="F") for a in hist]
+    c = [_f32F(a) for a in coefs]
+    nz, nx, nt = h[0].shape
+    _chk(lib.rtm_op3_hip(C.c_int(nt), C.c_int(nz), C.c_int(nx), C.c_int(zrec), *[_p(a) for a in h], *[_p(a) for a in c]))
+    return h
+
+
+def rtm_op3_c_void(hist, coefs, zrec=0):
+    """Call the void drop-in symbol rtm_op3_c_ itself (in place on copies)."""
+    lib = load()
+    h = [_f32F(a).copy(order="F") for a in hist]
+    c = [_f32F(a).copy(order="F") for a in coefs]
+    nz, nx, nt = h[0].shape
+    lib.rtm_op3_c_(C.c_int(nt), C.c_int(nz), C.c_int(nx), C.c_int(zrec), *[_p(a) for a in h], *[_p(a) for a in c])
+    return h
+
+
+def forward(s: RtmShot, coef5, wavelet, want_last=False):
+    lib = load()
+    cb = pack_fields(coef5)
+    w = np.ascontiguousarray(wavelet, np.float32)
+    rec = np.zeros((5, s.nrx, s.nt), np.float32)
+    last = np.zeros((10, s.nx, s.nz), np.float32) if want_last else None
+    _chk(lib.rtm_forward_hip(C.byref(s), _p(cb), _p(w), _p(rec), _p(last)))
+    if want_last:
+        return rec, unpack_fields(last)
+    return rec
+
+
+def migrate(s: RtmShot, coef5, wavelet, rec):
+    lib = load()
+    cb = pack_fields(coef5)
+    w = np.ascontiguousarray(wavelet, np.float32)
+    r = np.ascontiguousarray(rec, np.float32)
+    img = np.zeros((3, s.nx, s.nz), np.float32)
+    _chk(lib.rtm_migrate_hip(C.byref(s), _p(cb), _p(w), _p(r), _p(img)))
+    return tuple(unpack_fields(img))
+
+
+def born(s: RtmShot, coef5, wavelet, dm):
+    lib = load()
+    cb = pack_fields(coef5)
+    w = np.ascontiguousarray(wavelet, np.float32)
+    d = pack_fields(dm)
+    rec = np.zeros((5, s.nrx, s.nt), np.float32)
+    _chk(lib.rtm_born_hip(C.byref(s), _p(cb), _p(w), _p(d), _p(rec)))
+    return rec
+
+
+def model_op(rho, lam, mu, dx, dt):
+    """Staggered, dt/dx-scaled coefficients (host setup, FP32 reference order)."""
+    lib = load()
+    rho, lam, mu = _f32F(rho), _f32F(lam), _f32F(mu)
+    nz, nx = rho.shape
+    out = np.zeros((5, nx, nz), np.float32)
+    _chk(lib.rtm_model_op(C.c_int(nz), C.c_int(nx), C.c_float(dx), C.c_float(dt), _p(rho), _p(lam), _p(mu), _p(out)))
+    return unpack_fields(out)
+
+
+def ricker(nt, dt, fp, nsource=250):
+    lib = load()
+    out = np.zeros(nt, np.float32)
+    _chk(lib.rtm_ricker(C.c_int(nt), C.c_float(dt), C.c_float(fp), C.c_int(nsource), _p(out)))
+    return out
+
+
+def abc_k(field, abc):
+    lib = load()
+    f = _f32F(field).copy(order="F")
+    nz, nx = f.shape
+    _chk(lib.abc_k_hip(C.c_int(nz), C.c_int(nx), C.c_int(abc), _p(f)))
+    return f
+
+
+def cfl_check(vmax, dt, dx):
+    lib = load()
+    cr, lim = C.c_double(), C.c_double()
+    code = lib.rtm_cfl_check(C.c_float(vmax), C.c_float(dt), C.c_float(dx), C.byref(cr), C.byref(lim))
+    return code == 0, cr.value, lim.value
+
+
+def timings():
+    """[fwd ms, fwd launches, adj ms, adj launches, total ms] of the last call."""
+    lib = load()
+    out = np.zeros(5, np.float64)
+    lib.rtm_last_timings(_p(out))
+    return out
+
+
+class RtmOperator:
+    """Mirror of the Fortran module rtm_mod_adj (W/rtm-mod-adj.f90):
+    ``RtmOperator(...)`` is rtm_init (:43-117) and ``op(adj, ...)`` is rtm_op
+    (:120-129). xsource, zsource, zrec are Fortran 1-based indices."""
+
+    def __init__(self, nz, dz, nx, dx, dt, xsource, zsource, zrec, fp, nt, abc, free_surface, m1, m2, m3, lame=True):
+        lib = load()
+        self.nz, self.nx, self.nt = nz, nx, nt
+        self._ctx = C.c_void_p()
+        m1, m2, m3 = _f32F(m1), _f32F(m2), _f32F(m3)
+        _chk(lib.rtm_init_c_(C.byref(self._ctx), C.c_int(nz), C.c_float(dz), C.c_int(nx), C.c_float(dx), C.c_float(dt),
+                             C.c_int(xsource), C.c_int(zsource), C.c_int(zrec), C.c_float(fp), C.c_int(nt), C.c_int(abc),
+                             C.c_int(int(free_surface)), _p(m1), _p(m2), _p(m3), C.c_int(int(lame))))
+
+    def set_option(self, key, value):
+        _chk(load().rtm_set_option_c_(self._ctx, C.c_int(key), C.c_int(value)))
+
+    def op(self, adj, dm1, dm2, dm3, Vx, Vz, sigmaxx, sigmazz, sigmaxz):
+        """In place, like the Fortran rtm_op: adj=True writes dm1..dm3 (drho,
+        dlambda, dmu) from the (nt, nrx) data; adj=False writes the data."""
+        arrs = [dm1, dm2, dm3, Vx, Vz, sigmaxx, sigmazz, sigmaxz]
+        for a in arrs:
+            if a.dtype != np.float32 or not a.flags.f_contiguous:
+                raise ValueError("rtm_op arrays must be float32 Fortran-ordered (written in place)")
+        _chk(load().rtm_op_c_(self._ctx, C.c_int(int(adj)), *[_p(a) for a in arrs]))
+        return 0
+
+    def close(self):
+        if self._ctx:
+            load().rtm_final_c_(self._ctx)
+            self._ctx = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -1,0 +1,244 @@
+"""CPU oracle loader -- TEST INFRASTRUCTURE ONLY.
+
+Only ``tests/``, ``__graft_entry__.smoke()`` and ``bench.py``'s cpu_baseline leg
+import this package. It wraps ``liboracle.so`` (the C++ restatement in
+``rtm_oracle.cpp``) and, when present, ``_ref/librtm_ref.so`` (the reference's
+own scatter-form CPU adjoint, built from its source by ``Makefile``).
+
+Array conventions (reference layout, W/gpu.h:8-14):
+  field     numpy (nz, nx)      Fortran order   -> index iz + nz*ix
+  history   numpy (nz, nx, nt)  Fortran order   -> iz + nz*ix + nz*nx*it
+  receivers numpy (5, nrx, nt)  C order         -> component f, trace ix, sample it
+            (each component block is the reference's (nt, nx) layout it + nt*ix)
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+_REF = None
+
+
+class Params(C.Structure):
+    """Mirror of orc::Params in rtm_oracle.cpp."""
+
+    _fields_ = [
+        ("nz", C.c_int), ("nx", C.c_int), ("nt", C.c_int),
+        ("dx", C.c_double), ("dt", C.c_double),
+        ("xs", C.c_int), ("zs", C.c_int),
+        ("zr", C.c_int), ("rx0", C.c_int), ("nrx", C.c_int),
+        ("abc", C.c_int), ("adj_abc", C.c_int), ("snap_every", C.c_int),
+        ("nthreads", C.c_int),
+    ]
+
+
+def build():
+    """Compile liboracle.so (and _ref when the reference tree is mounted)."""
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        path = os.path.join(HERE, "liboracle.so")
+        if not os.path.exists(path):
+            build()
+        _LIB = C.CDLL(path)
+    return _LIB
+
+
+def ref_lib():
+    """The reference's own CPU adjoint (R/Src_cuda_722-1/rtmlib-modC.c) or None."""
+    global _REF
+    if _REF is None:
+        path = os.path.join(HERE, "_ref", "librtm_ref.so")
+        if not os.path.exists(path):
+            return None
+        _REF = C.CDLL(path)
+    return _REF
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+def _suffix(dtype):
+    return "f64" if np.dtype(dtype) == np.float64 else "f32"
+
+
+def _scal(dtype):
+    return C.c_double if np.dtype(dtype) == np.float64 else C.c_float
+
+
+def F(a, dtype):
+    return np.asfortranarray(a, dtype=dtype)
+
+
+def fd_coefs(dtype=np.float64):
+    out = np.zeros(5, dtype)
+    getattr(lib(), "orc_fd_coefs_" + _suffix(dtype))(_p(out))
+    return out
+
+
+def model_op(rho, lam, mu, dx, dt, dtype=np.float64):
+    """W/model-mod.f90:21-89 + rtmmod_init + aux_*_c -> (m1_x, m1_z, m2, m3, m2m3)."""
+    nz, nx = rho.shape
+    rho, lam, mu = F(rho, dtype), F(lam, dtype), F(mu, dtype)
+    outs = [np.zeros((nz, nx), dtype, order="F") for _ in range(5)]
+    s = _scal(dtype)
+    getattr(lib(), "orc_model_op_" + _suffix(dtype))(
+        C.c_int(nz), C.c_int(nx), s(dx), s(dt), _p(rho), _p(lam), _p(mu), *[_p(o) for o in outs])
+    return tuple(outs)
+
+
+def ricker(nt, dt, fp, nsource=250, dtype=np.float64):
+    out = np.zeros(nt, dtype)
+    s = _scal(dtype)
+    getattr(lib(), "orc_ricker_" + _suffix(dtype))(C.c_int(nt), s(dt), s(fp), C.c_int(nsource), _p(out))
+    return out
+
+
+def abc_filter(abc, dtype=np.float64):
+    out = np.zeros(abc + 5, dtype)
+    getattr(lib(), "orc_abc_filter_" + _suffix(dtype))(C.c_int(abc), _p(out))
+    return out
+
+
+def abc_K(field, abc, dtype=np.float64):
+    f = F(field, dtype).copy(order="F")
+    nz, nx = f.shape
+    getattr(lib(), "orc_abc_K_" + _suffix(dtype))(C.c_int(nz), C.c_int(nx), C.c_int(abc), _p(f))
+    return f
+
+
+OPS = ["Ax", "Bx", "Az", "Bz", "D", "E", "F", "G", "H", "J"]
+
+
+def fwd_op(op, coef, model, data, dtype=np.float64):
+    """data += op(model) (adj=.false., add=.true.). Returns a new array."""
+    k = OPS.index(op) if isinstance(op, str) else op
+    coef, model = F(coef, dtype), F(model, dtype)
+    data = F(data, dtype).copy(order="F")
+    nz, nx = data.shape
+    getattr(lib(), "orc_fwd_op_" + _suffix(dtype))(C.c_int(k), C.c_int(nz), C.c_int(nx), _p(coef), _p(model), _p(data))
+    return data
+
+
+def adj_op(op, coef, model, data, dtype=np.float64):
+    """model += op^T(data) in scatter form (adj=.true., add=.true.). Returns a new array."""
+    k = OPS.index(op) if isinstance(op, str) else op
+    coef, data = F(coef, dtype), F(data, dtype)
+    model = F(model, dtype).copy(order="F")
+    nz, nx = model.shape
+    getattr(lib(), "orc_adj_op_scatter_" + _suffix(dtype))(C.c_int(k), C.c_int(nz), C.c_int(nx), _p(coef), _p(model), _p(data))
+    return model
+
+
+def rtm_op3_c(hist, coefs, form="gather", dtype=np.float64, nthreads=1):
+    """In-place semantics of rtm_op3_c_ on copies. hist: 5 arrays (nz,nx,nt);
+    coefs: (m1_x, m1_z, aux_m2, aux_m3, aux_m2m3). Returns 5 new histories."""
+    h = [F(a, dtype).copy(order="F") for a in hist]
+    c = [F(a, dtype) for a in coefs]
+    nz, nx, nt = h[0].shape
+    if form == "gather":
+        getattr(lib(), "orc_rtm_op3_c_gather_" + _suffix(dtype))(
+            C.c_int(nt), C.c_int(nz), C.c_int(nx), *[_p(a) for a in h], *[_p(a) for a in c], C.c_int(nthreads))
+    else:
+        getattr(lib(), "orc_rtm_op3_c_scatter_" + _suffix(dtype))(
+            C.c_int(nt), C.c_int(nz), C.c_int(nx), *[_p(a) for a in h], *[_p(a) for a in c])
+    return h
+
+
+def ref_rtm_op3_c(hist, coefs, zrec=0):
+    """Run the reference's own rtm_op3_c_ (19-argument scatter form) on copies.
+    The five (nt,nx) arrays of that form are unused by it; zero arrays are passed."""
+    r = ref_lib()
+    if r is None:
+        raise RuntimeError("oracle/_ref/librtm_ref.so is not built")
+    h = [F(a, np.float32).copy(order="F") for a in hist]
+    c = [F(a, np.float32) for a in coefs]
+    nz, nx, nt = h[0].shape
+    if nt < 59:
+        raise ValueError("the reference adjoint prints progress with it%(nt/59): nt must be >= 59")
+    dummy = [np.zeros((nt, nx), np.float32, order="F") for _ in range(5)]
+    r.rtm_op3_c_(C.c_int(nt), C.c_int(nz), C.c_int(nx), C.c_int(zrec), *[_p(a) for a in h],
+                 *[_p(a) for a in dummy], *[_p(a) for a in c])
+    return h
+
+
+def rtm_op3_prestep(nt, zrec, data, coefs, dtype=np.float64):
+    """rtm_op3 pre-step and injection (W/rtmlib-mod.f90:91-146). data: 5 arrays (nt,nx)."""
+    c = [F(a, dtype) for a in coefs]
+    nz, nx = c[0].shape
+    h = [np.zeros((nz, nx, nt), dtype, order="F") for _ in range(5)]
+    d = [F(a, dtype) for a in data]
+    getattr(lib(), "orc_rtm_op3_prestep_" + _suffix(dtype))(
+        C.c_int(nt), C.c_int(nz), C.c_int(nx), C.c_int(zrec), *[_p(a) for a in h], *[_p(a) for a in d], *[_p(a) for a in c])
+    return h
+
+
+def make_params(nz, nx, nt, dx, dt, xs, zs, zr, rx0, nrx, abc, adj_abc=0, snap_every=1, nthreads=1):
+    return Params(nz, nx, nt, dx, dt, xs, zs, zr, rx0, nrx, abc, adj_abc, snap_every, nthreads)
+
+
+def forward(p: Params, wavelet, coefs, dtype=np.float64, want_hist=False, want_hist5=False):
+    """Forward source propagation (W/rtm-mod-adj.f90:178-260) with receivers.
+    Returns dict(rec=(5,nrx,nt), hist_vx, hist_vz, hist5)."""
+    c = [F(a, dtype) for a in coefs]
+    w = np.ascontiguousarray(wavelet, dtype)
+    rec = np.zeros((5, p.nrx, p.nt), dtype)
+    hvx = hvz = None
+    if want_hist:
+        hvx = np.zeros((p.nz, p.nx, p.nt), dtype, order="F")
+        hvz = np.zeros((p.nz, p.nx, p.nt), dtype, order="F")
+    h5 = [None] * 5
+    if want_hist5:
+        h5 = [np.zeros((p.nz, p.nx, p.nt), dtype, order="F") for _ in range(5)]
+    getattr(lib(), "orc_forward_" + _suffix(dtype))(
+        C.byref(p), _p(w), *[_p(a) for a in c], _p(rec), _p(hvx), _p(hvz), *[_p(a) for a in h5])
+    return dict(rec=rec, hist_vx=hvx, hist_vz=hvz, hist5=h5 if want_hist5 else None)
+
+
+def image_S(vx, vz, vx1, vz1, dx, dt, dtype=np.float64):
+    """S1..S6 at one level (W/image-mod-rtm.f90:26-67). vx1/vz1 may be None (last level)."""
+    vx, vz = F(vx, dtype), F(vz, dtype)
+    vx1 = None if vx1 is None else F(vx1, dtype)
+    vz1 = None if vz1 is None else F(vz1, dtype)
+    nz, nx = vx.shape
+    S = [np.zeros((nz, nx), dtype, order="F") for _ in range(6)]
+    s = _scal(dtype)
+    getattr(lib(), "orc_image_S_" + _suffix(dtype))(
+        C.c_int(nz), C.c_int(nx), s(dx), s(dt), _p(vx), _p(vz), _p(vx1), _p(vz1), *[_p(a) for a in S])
+    return S
+
+
+def rtm(p: Params, wavelet, coefs, rec, dtype=np.float64):
+    """Full single-shot RTM -> (drho, dlambda, dmu), each (nz, nx) Fortran order."""
+    c = [F(a, dtype) for a in coefs]
+    w = np.ascontiguousarray(wavelet, dtype)
+    r = np.ascontiguousarray(rec, dtype)
+    out = [np.zeros((p.nz, p.nx), dtype, order="F") for _ in range(3)]
+    getattr(lib(), "orc_rtm_" + _suffix(dtype))(C.byref(p), _p(w), *[_p(a) for a in c], _p(r), *[_p(a) for a in out])
+    return tuple(out)
+
+
+def born(p: Params, wavelet, coefs, dm, dtype=np.float64):
+    """Born modelling (W/rtm-mod-adj.f90:273-375) -> receivers (5, nrx, nt)."""
+    c = [F(a, dtype) for a in coefs]
+    w = np.ascontiguousarray(wavelet, dtype)
+    d = [F(a, dtype) for a in dm]
+    rec = np.zeros((5, p.nrx, p.nt), dtype)
+    getattr(lib(), "orc_born_" + _suffix(dtype))(C.byref(p), _p(w), *[_p(a) for a in c], *[_p(a) for a in d], _p(rec))
+    return rec
+
+
+def rel_l2(a, b):
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    den = np.linalg.norm(b.ravel())
+    return float(np.linalg.norm((a - b).ravel()) / (den if den > 0 else 1.0))
