@@ -1,5 +1,5 @@
-// elastic2d.hpp -- kernel argument blocks and launchers of the 2D elastic
-// velocity-stress time step (forward, adjoint, imaging) for gfx950.
+// elastic2d.hpp -- device descriptors and launchers of the 2D elastic
+// velocity-stress RTM step kernels (gfx950).
 #pragma once
 
 #include "common.hpp"
@@ -9,68 +9,99 @@ namespace erm {
 // Field order everywhere: 0 Vx, 1 Vz, 2 sigma_xx, 3 sigma_zz, 4 sigma_xz.
 // Coefficient order: 0 m1_x, 1 m1_z, 2 aux_m2, 3 aux_m3, 4 aux_m2m3 (each
 // already scaled by dt/dx and zeroed outside its operator's index range).
-struct StepArgs {
-  int nz, nx, ldz;     // grid and column stride of every field below
-  float* out[5];       // level being computed
-  const float* x2[5];  // level it-2 (forward) / it+2 (adjoint); may alias out
-  const float* y1[5];  // level it-1 (forward) / it+1 (adjoint)
-  const float* xin[5]; // adjoint only: level-it content added first (drop-in form) or null
-  const float* coef[5];
-  const float* kz;     // sponge row factors (nz) or null = no sponge
-  const float* kx;     // sponge column factors (nx)
-  // explosive source (forward): amp added to sxx, szz at (zs, xs)
-  int zs, xs;
-  float src_amp;
-  int has_src;
-  const float* bsrc[5];   // Born scattering sources of this level (forward) or null
-  // receivers on row zr, columns rx0 .. rx0+nrx-1
-  int zr, rx0, nrx;
-  float* rec_out[5];      // forward: row sample of this level, contiguous over ix
-  const float* rec_in[5]; // adjoint: injected row of this level
-  // source-wavefield history written by the forward step (full grid, fp32)
-  float* hvx;
-  float* hvz;
-  // imaging (adjoint): source Vx, Vz at this level and the next one
-  const float* svx;
-  const float* svz;
-  const float* svx1;  // null at the last level (S1 = S2 = 0 there)
-  const float* svz1;
-  float* img[3];      // drho, dlambda, dmu
-  float* imgc[3];     // Neumaier compensation terms (comp mode)
-  float inv_dx, inv_dt, img_w;
-  int do_image;
-  int comp;
-  int xcd_remap;
+
+// Snapshot formats of the source wavefield kept for the imaging condition.
+enum SnapFmt : int {
+  SNAP_HIST = 0,  // fp32 Vx, Vz of every level (the plain form)
+  SNAP_F32 = 1,   // every j-th level: Vx, Vz, dVx, dVz in fp32
+  SNAP_BF16 = 2,  // same four arrays in bf16 with a max-abs scale per strip
+  SNAP_F16 = 3    // same four arrays in fp16 with a max-abs scale per strip
 };
 
-// Tile geometry of the step kernels: 64 z-cells (one wave64 along the fast
-// axis) by 4*TXR x-cells, 256 threads, radius-5 halo staged in LDS.
+// One shot (one computational grid) as the kernels see it. Lives in device
+// memory; a kernel reads the descriptor of its tile's shot with scalar loads.
+struct ShotDev {
+  int nz, nx, ldz, nt;
+  long long n2;  // ldz * nx: elements per field
+  // Level storage: field f of level L is at
+  //   lev + ((L + lev_off) % lev_mod) * lev_stride + f * fstride
+  // (ring: lev_mod 2, lev_stride 5*n2, fstride n2; drop-in history: 0-based
+  // levels, lev_mod nt, lev_stride n2, fstride nt*n2).
+  float* lev;
+  long long lev_stride, fstride;
+  int lev_mod, lev_off;
+  const float* coef;  // 5 * n2, masked
+  const float* kz;    // sponge factors (nz), null = none
+  const float* kx;    // (nx)
+  // explosive source: src_amp[L-1] = dt*source(L-1) is added at (zs, xs) to
+  // sxx and szz when level L is computed
+  int zs, xs;
+  const float* src_amp;
+  // receivers: row zr, columns rx0..rx0+nrx-1; rec is [5][nt][nrx], level L
+  // at row L-1 (recorded by the forward, injected by the adjoint)
+  int zr, rx0, nrx;
+  float* rec;
+  // source-wavefield snapshots
+  int snap_fmt, snap_every, snap_sw, snap_nbz, snap_nstrip;
+  void* snap;          // storage (see SnapFmt)
+  float* snap_scale;   // [k][4][nstrip] for SNAP_BF16 / SNAP_F16
+  // imaging
+  float* img;   // 3 * n2: drho, dlambda, dmu
+  float* imgc;  // 3 * n2 compensation terms
+  float inv_dx, inv_dt, img_w;
+  // Born modelling
+  float* bsrc;       // 5 * n2 scattering sources of the current level
+  const float* dm;   // 3 * n2 model perturbation
+};
+
+struct TileRef {
+  int shot, z0, x0, tile;
+};
+
+// Forward flags
+enum : int {
+  F_REC = 1,     // record row zr into rec (store)
+  F_RECSUB = 2,  // record by subtracting (observed = true - smooth)
+  F_K = 4,       // apply the sponge
+  F_SRC = 8,     // inject the explosive source
+  F_BORN = 16,   // add the Born sources bsrc
+  F_SNAP = 32    // write the source-wavefield snapshot belonging to this step
+};
+// Adjoint flags
+enum : int {
+  A_XIN = 1,   // drop-in form: add the level's own content first
+  A_INJ = 2,   // inject the receiver row of this level
+  A_K = 4,     // exact-adjoint sponge
+  A_IMG = 8,   // accumulate the imaging condition at this level
+  A_NEXT = 16, // level L+1 exists (S1, S2 nonzero)
+  A_COMP = 32  // compensated image accumulation
+};
+
+// Tile geometry: 64 z-cells (one wave64 along the fast axis) by 4*TXR
+// x-cells, 256 threads, radius-5 halo staged in LDS.
 constexpr int TILE_Z = 64;
 constexpr int TILE_WY = 4;
+inline int tile_x(int txr) { return TILE_WY * txr; }
 
-int launch_fwd_step(const StepArgs& a, int txr, hipStream_t st);
-int launch_adj_step(const StepArgs& a, int txr, hipStream_t st);
+// Step launches over a tile table covering one or many shots.
+int launch_fwd(const ShotDev* shots, const TileRef* tiles, int ntiles, int L, int flags,
+               int txr, int snap_fmt, int xcd, hipStream_t st);
+int launch_adj(const ShotDev* shots, const TileRef* tiles, int ntiles, int L, int flags,
+               int txr, int snap_fmt, int xcd, hipStream_t st);
+// Snapshot of the last level (no later step writes it): dV = 0.
+int launch_snap_last(const ShotDev* shots, const TileRef* tiles, int ntiles, int L, int txr,
+                     int snap_fmt, hipStream_t st);
+// Born scattering sources of level L of every shot (image_op2 forward branch).
+int launch_born_sources(const ShotDev* shots, const TileRef* tiles, int ntiles, int L,
+                        hipStream_t st);
 
-// Born scattering sources of one level (image_op2 forward branch,
-// W/image-mod-rtm.f90:163-179) from the background Vx, Vz at levels it, it+1.
-struct BornArgs {
-  int nz, nx, ldz;
-  const float* svx; const float* svz; const float* svx1; const float* svz1;
-  const float* dm[3];
-  float* src[5];
-  float inv_dx, inv_dt;
-};
-int launch_born_sources(const BornArgs& a, hipStream_t st);
-
-// out[i] = value over n floats (fill kernel; used instead of hipMemsetAsync so
-// the launch sequence stays capturable in a HIP graph).
 int launch_fill(float* out, size_t n, float value, hipStream_t st);
-
-// Sponge K applied in place to one field (abc_k_hip).
 int launch_apply_k(float* f, int nz, int nx, int ldz, const float* kz, const float* kx,
                    hipStream_t st);
-
-// Final image reduction: img = (double)s + (double)c, rounded to float.
+// img = (double)img + (double)imgc, rounded once.
 int launch_image_finalize(float* s, const float* c, size_t n, hipStream_t st);
+// stack[z0+iz, x0+ix] += img[iz0+iz, ix0+ix] (fp64 stack, 3 images).
+int launch_stack(double* stack, int snz, int snx, const float* img, int ldz, long long n2,
+                 int iz0, int ix0, int nz, int nx, int z0, int x0, hipStream_t st);
 
 }  // namespace erm

@@ -1,22 +1,24 @@
 // elastic2d_kernels.hip -- gfx950 kernels of the 2D elastic velocity-stress
 // RTM time step (10th-order staggered grid, radius 5).
 //
-// One launch advances all five fields (Vx, Vz, sxx, szz, sxz) by one time
-// level. A workgroup owns a 64 (z) x TX (x) tile: one wave64 spans the fast z
-// axis so every global access of a wave is one contiguous 256-byte column
-// segment, four waves split the x extent, and the radius-5 halo of the
-// level being differentiated is staged once in LDS. The stencils are not a
-// dense contraction, so they run on the VALU; the kernel is bounded by HBM
-// bandwidth (80 algorithmic bytes per cell-update).
+// One launch advances all five fields (Vx, Vz, sxx, szz, sxz) of every shot
+// in a tile table by one time level. A workgroup owns a 64 (z) x 4*TXR (x)
+// tile of one shot: one wave64 spans the fast z axis so each global access of
+// a wave is one contiguous 256-byte column segment, four waves split the x
+// extent, and the radius-5 halo of the level being differentiated is staged
+// once in LDS. The stencils are not a dense contraction, so they run on the
+// VALU; the step is bounded by HBM bandwidth (80 algorithmic bytes per
+// cell-update: 5 fields read at level L-1, 5 at L-2, 5 written, 5
+// coefficients).
 //
 // The reference operators (W/rtmlib-mod.f90:200-633) all reduce to two
 // staggered differences along either axis:
 //   S+ f(i) = sum_k c_k (f[i+6-k] - f[i-5+k])   (D, E, F, G, Az, Bx)
 //   S- f(i) = sum_k c_k (f[i+5-k] - f[i-6+k])   (Ax, Bz, H, J)
-// and their transposes satisfy (m S+)^T = -S-(m .), (m S-)^T = -S+(m .), so
-// the gather-form adjoint (W/rtmlib-modC.c:120-253) is the same stencil
-// applied to coefficient-weighted products, which the adjoint kernel forms
-// while staging the halo.
+// and (m S+)^T = -S-(m .), (m S-)^T = -S+(m .), so the gather-form adjoint
+// (W/rtmlib-modC.c:120-253) is the same stencil applied to
+// coefficient-weighted products, which the adjoint kernel forms while it
+// stages the halo.
 
 #include "elastic2d.hpp"
 
@@ -54,7 +56,7 @@ __device__ __forceinline__ float sm(const float* f, int p) {
 }
 
 // Workgroups are dispatched round-robin over the 8 XCDs; renumber them so
-// each XCD receives a contiguous range of tiles (column-major tile order),
+// each XCD receives a contiguous range of tiles (shot-major, column-major),
 // which keeps neighbouring tiles -- and their shared halos -- in one L2.
 __device__ __forceinline__ int xcd_tile(int L, int N) {
   const int q = N >> 3, r = N & 7;
@@ -62,7 +64,6 @@ __device__ __forceinline__ int xcd_tile(int L, int N) {
   return (x < r) ? x * (q + 1) + pos : r * (q + 1) + (x - r) * q + pos;
 }
 
-// Neumaier compensated accumulation of t into (s, c).
 __device__ __forceinline__ void neumaier(float& s, float& c, float t) {
   const float u = s + t;
   if (fabsf(s) >= fabsf(t))
@@ -72,57 +73,106 @@ __device__ __forceinline__ void neumaier(float& s, float& c, float t) {
   s = u;
 }
 
+__device__ __forceinline__ float wave_max(float v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o, 64));
+  return v;
+}
+
+// bf16 / fp16 conversions (round to nearest even).
+__device__ __forceinline__ unsigned short q_bf16(float x) {
+  unsigned u = __float_as_uint(x);
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return (unsigned short)(u >> 16);
+}
+__device__ __forceinline__ float dq_bf16(unsigned short h) { return __uint_as_float(((unsigned)h) << 16); }
+__device__ __forceinline__ unsigned short q_f16(float x) {
+  const _Float16 h = (_Float16)x;
+  return __builtin_bit_cast(unsigned short, h);
+}
+__device__ __forceinline__ float dq_f16(unsigned short b) { return (float)__builtin_bit_cast(_Float16, b); }
+
+template <int SNAP>
+struct SnapQ {
+  static constexpr float Q = SNAP == SNAP_F16 ? 32768.f : 1.f;
+  __device__ static unsigned short enc(float x) { return SNAP == SNAP_F16 ? q_f16(x) : q_bf16(x); }
+  __device__ static float dec(unsigned short h) { return SNAP == SNAP_F16 ? dq_f16(h) : dq_bf16(h); }
+};
+
+struct LevelPtr {
+  float* p;
+};
+__device__ __forceinline__ float* lev_ptr(float* lev, long long stride, int mod, int off, int L) {
+  return lev + (long long)((L + off) % mod) * stride;
+}
+
 // ---------------------------------------------------------------------------
-// Forward step, W/rtm-mod-adj.f90:225-252 (and the Born propagation :323-348):
-//   Vx  = K(Vx(it-2)  + m1_x (S-x sxx + S+z sxz))            Ax, Bx
-//   Vz  = K(Vz(it-2)  + m1_z (S+x sxz + S-z szz))            Az, Bz
-//   sxx = K(sxx(it-2) + src + m2m3 S+x Vx + m2 S+z Vz)       D, E
-//   szz = K(szz(it-2) + src + m2 S+x Vx + m2m3 S+z Vz)       F, G
-//   sxz = K(sxz(it-2) + m3 (S-z Vx + S-x Vz))                H, J
-// with all differences of level it-1. Coefficients are zero outside each
+// Forward step, W/rtm-mod-adj.f90:225-252 (Born propagation :323-348):
+//   Vx  = K(Vx(L-2)  + m1_x (S-x sxx + S+z sxz))            Ax, Bx
+//   Vz  = K(Vz(L-2)  + m1_z (S+x sxz + S-z szz))            Az, Bz
+//   sxx = K(sxx(L-2) + src + m2m3 S+x Vx + m2 S+z Vz)       D, E
+//   szz = K(szz(L-2) + src + m2 S+x Vx + m2m3 S+z Vz)       F, G
+//   sxz = K(sxz(L-2) + m3 (S-z Vx + S-x Vz))                H, J
+// all differences of level L-1. Coefficients are zero outside each
 // operator's range, which reproduces the reference loop bounds exactly.
-template <int TXR>
-__global__ __launch_bounds__(256) void k_fwd_step(const StepArgs a, int nbz, int ntiles) {
+// With F_SNAP the step also stores the snapshot of the imaging level L-1
+// (Vx, Vz of L-1 from the staged tile and dV = V(L) - V(L-1)), or with
+// SNAP_HIST the Vx, Vz of level L.
+template <int TXR, int SNAP>
+__global__ __launch_bounds__(256) void k_fwd(const ShotDev* __restrict__ shots,
+                                             const TileRef* __restrict__ tiles, int ntiles, int L,
+                                             int flags, int xcd) {
   using T = Tile<TXR>;
   __shared__ float sh[5][T::N];
-  const int t = a.xcd_remap ? xcd_tile(blockIdx.x, ntiles) : (int)blockIdx.x;
-  const int z0 = (t % nbz) * T::TZ, x0 = (t / nbz) * T::TX;
+  const TileRef tr = tiles[xcd ? xcd_tile(blockIdx.x, ntiles) : (int)blockIdx.x];
+  const ShotDev& S = shots[tr.shot];
+  const int nz = S.nz, nx = S.nx, ldz = S.ldz;
+  const long long fs = S.fstride, n2 = S.n2;
+  float* const out = lev_ptr(S.lev, S.lev_stride, S.lev_mod, S.lev_off, L);
+  const float* const x2 = lev_ptr(S.lev, S.lev_stride, S.lev_mod, S.lev_off, L - 2);
+  const float* const y1 = lev_ptr(S.lev, S.lev_stride, S.lev_mod, S.lev_off, L - 1);
+  const float* const coef = S.coef;
+  const int z0 = tr.z0, x0 = tr.x0;
   const int tz = threadIdx.x, ty = threadIdx.y;
 
   for (int lx = ty; lx < T::LX; lx += T::WY) {
     const int gx = x0 - T::H + lx;
-    const bool xin = gx >= 0 && gx < a.nx;
+    const bool xin = gx >= 0 && gx < nx;
     for (int lz = tz; lz < T::LZ; lz += T::TZ) {
       const int gz = z0 - T::H + lz;
-      const bool in = xin && gz >= 0 && gz < a.nz;
-      const size_t gi = (size_t)gz + (size_t)a.ldz * gx;
+      const bool in = xin && gz >= 0 && gz < nz;
+      const long long gi = (long long)gz + (long long)ldz * gx;
       const int p = lx * T::LZ + lz;
 #pragma unroll
-      for (int f = 0; f < 5; ++f) sh[f][p] = in ? a.y1[f][gi] : 0.f;
+      for (int f = 0; f < 5; ++f) sh[f][p] = in ? y1[f * fs + gi] : 0.f;
     }
   }
   __syncthreads();
 
   const int gz = z0 + tz;
-  if (gz >= a.nz) return;
-  const float kz = a.kz ? a.kz[gz] : 1.f;
+  const bool zok = gz < nz;
+  const float kz = ((flags & F_K) && zok) ? S.kz[gz] : 1.f;
+  constexpr int SX = T::LZ;
+  float sv[4][TXR];
 #pragma unroll
   for (int j = 0; j < TXR; ++j) {
     const int lxo = ty * TXR + j;
     const int gx = x0 + lxo;
-    if (gx >= a.nx) break;
+    const bool ok = zok && gx < nx;
     const int p = (lxo + T::H) * T::LZ + (tz + T::H);
-    const size_t gi = (size_t)gz + (size_t)a.ldz * gx;
-    const float m1x = a.coef[0][gi], m1z = a.coef[1][gi], m2 = a.coef[2][gi],
-                m3 = a.coef[3][gi], m2m3 = a.coef[4][gi];
+#pragma unroll
+    for (int a = 0; a < 4; ++a) sv[a][j] = 0.f;
+    if (!ok) continue;
+    const long long gi = (long long)gz + (long long)ldz * gx;
+    const float m1x = coef[gi], m1z = coef[n2 + gi], m2 = coef[2 * n2 + gi],
+                m3 = coef[3 * n2 + gi], m2m3 = coef[4 * n2 + gi];
     float v[5];
 #pragma unroll
-    for (int f = 0; f < 5; ++f) v[f] = a.x2[f][gi];
-    if (a.bsrc[0]) {
+    for (int f = 0; f < 5; ++f) v[f] = x2[f * fs + gi];
+    if (flags & F_BORN) {
 #pragma unroll
-      for (int f = 0; f < 5; ++f) v[f] = a.bsrc[f][gi] + v[f];
+      for (int f = 0; f < 5; ++f) v[f] = S.bsrc[f * n2 + gi] + v[f];
     }
-    constexpr int SX = T::LZ;
     const float dxx_xm = sm<SX>(sh[2], p);  // Ax
     const float dxz_zp = sp<1>(sh[4], p);   // Bx
     const float dxz_xp = sp<SX>(sh[4], p);  // Az
@@ -135,9 +185,10 @@ __global__ __launch_bounds__(256) void k_fwd_step(const StepArgs a, int nbz, int
     v[0] = v[0] + m1x * dxz_zp;
     v[1] = v[1] + m1z * dxz_xp;
     v[1] = v[1] + m1z * dzz_zm;
-    if (a.has_src && gz == a.zs && gx == a.xs) {
-      v[2] = v[2] + a.src_amp;
-      v[3] = v[3] + a.src_amp;
+    if ((flags & F_SRC) && gz == S.zs && gx == S.xs) {
+      const float amp = S.src_amp[L - 1];
+      v[2] = v[2] + amp;
+      v[3] = v[3] + amp;
     }
     v[2] = v[2] + m2m3 * dvx_xp;
     v[2] = v[2] + m2 * dvz_zp;
@@ -145,58 +196,190 @@ __global__ __launch_bounds__(256) void k_fwd_step(const StepArgs a, int nbz, int
     v[3] = v[3] + m2m3 * dvz_zp;
     v[4] = v[4] + m3 * dvx_zm;
     v[4] = v[4] + m3 * dvz_xm;
-    if (a.kz) {
-      const float kx = a.kx[gx];
+    if (flags & F_K) {
+      const float kx = S.kx[gx];
 #pragma unroll
       for (int f = 0; f < 5; ++f) v[f] = (v[f] * kz) * kx;
     }
 #pragma unroll
-    for (int f = 0; f < 5; ++f) a.out[f][gi] = v[f];
-    if (a.hvx) {
-      a.hvx[gi] = v[0];
-      a.hvz[gi] = v[1];
-    }
-    if (a.rec_out[0] && gz == a.zr && gx >= a.rx0 && gx < a.rx0 + a.nrx) {
+    for (int f = 0; f < 5; ++f) out[f * fs + gi] = v[f];
+    if ((flags & (F_REC | F_RECSUB)) && gz == S.zr && gx >= S.rx0 && gx < S.rx0 + S.nrx) {
+      float* r = S.rec + (long long)(L - 1) * S.nrx + (gx - S.rx0);
+      const long long rs = (long long)S.nt * S.nrx;
 #pragma unroll
-      for (int f = 0; f < 5; ++f) a.rec_out[f][gx - a.rx0] = v[f];
+      for (int f = 0; f < 5; ++f) {
+        if (flags & F_RECSUB)
+          r[f * rs] = r[f * rs] - v[f];
+        else
+          r[f * rs] = v[f];
+      }
+    }
+    if (SNAP == SNAP_HIST && (flags & F_SNAP)) {
+      float* h = static_cast<float*>(S.snap) + (long long)(L - 1) * 2 * n2;
+      h[gi] = v[0];
+      h[n2 + gi] = v[1];
+    }
+    if (SNAP != SNAP_HIST) {
+      sv[0][j] = sh[0][p];
+      sv[1][j] = sh[1][p];
+      sv[2][j] = v[0] - sh[0][p];
+      sv[3][j] = v[1] - sh[1][p];
     }
   }
+
+  if (SNAP != SNAP_HIST && (flags & F_SNAP)) {
+    // snapshot k of level L-1 = j*k (flag set by the host only then)
+    const long long k0 = (L - 1) / S.snap_every - 1;
+    if (SNAP == SNAP_F32) {
+      float* sp4 = static_cast<float*>(S.snap) + k0 * 4 * n2;
+#pragma unroll
+      for (int j = 0; j < TXR; ++j) {
+        const int gx = x0 + ty * TXR + j;
+        if (!zok || gx >= nx) continue;
+        const long long gi = (long long)gz + (long long)ldz * gx;
+#pragma unroll
+        for (int a = 0; a < 4; ++a) sp4[a * n2 + gi] = sv[a][j];
+      }
+    } else {
+      unsigned short* q4 = static_cast<unsigned short*>(S.snap) + k0 * 4 * n2;
+      const int strip = (z0 >> 6) + S.snap_nbz * ((x0 >> S.snap_sw) + ty);
+#pragma unroll
+      for (int a = 0; a < 4; ++a) {
+        float m = 0.f;
+#pragma unroll
+        for (int j = 0; j < TXR; ++j) m = fmaxf(m, fabsf(sv[a][j]));
+        m = wave_max(m);
+        const float inv = m > 0.f ? SnapQ<SNAP>::Q / m : 0.f;
+        if (tz == 0) S.snap_scale[(k0 * 4 + a) * S.snap_nstrip + strip] = m / SnapQ<SNAP>::Q;
+#pragma unroll
+        for (int j = 0; j < TXR; ++j) {
+          const int gx = x0 + ty * TXR + j;
+          if (!zok || gx >= nx) continue;
+          const long long gi = (long long)gz + (long long)ldz * gx;
+          q4[a * n2 + gi] = SnapQ<SNAP>::enc(sv[a][j] * inv);
+        }
+      }
+    }
+  }
+}
+
+// Snapshot of the imaging level L that no later forward step writes (the
+// last level): Vx, Vz from the ring, dV = 0.
+template <int TXR, int SNAP>
+__global__ __launch_bounds__(256) void k_snap_last(const ShotDev* __restrict__ shots,
+                                                   const TileRef* __restrict__ tiles, int L) {
+  const TileRef tr = tiles[blockIdx.x];
+  const ShotDev& S = shots[tr.shot];
+  const int nz = S.nz, nx = S.nx, ldz = S.ldz;
+  const long long fs = S.fstride, n2 = S.n2;
+  const float* const cur = lev_ptr(S.lev, S.lev_stride, S.lev_mod, S.lev_off, L);
+  const int z0 = tr.z0, x0 = tr.x0, tz = threadIdx.x, ty = threadIdx.y;
+  const int gz = z0 + tz;
+  const bool zok = gz < nz;
+  const long long k0 = L / S.snap_every - 1;
+  float sv[2][TXR];
+#pragma unroll
+  for (int j = 0; j < TXR; ++j) {
+    const int gx = x0 + ty * TXR + j;
+    const bool ok = zok && gx < nx;
+    const long long gi = (long long)gz + (long long)ldz * gx;
+    sv[0][j] = ok ? cur[gi] : 0.f;
+    sv[1][j] = ok ? cur[fs + gi] : 0.f;
+  }
+  if (SNAP == SNAP_F32) {
+    float* sp4 = static_cast<float*>(S.snap) + k0 * 4 * n2;
+#pragma unroll
+    for (int j = 0; j < TXR; ++j) {
+      const int gx = x0 + ty * TXR + j;
+      if (!zok || gx >= nx) continue;
+      const long long gi = (long long)gz + (long long)ldz * gx;
+      sp4[gi] = sv[0][j];
+      sp4[n2 + gi] = sv[1][j];
+      sp4[2 * n2 + gi] = 0.f;
+      sp4[3 * n2 + gi] = 0.f;
+    }
+  } else {
+    unsigned short* q4 = static_cast<unsigned short*>(S.snap) + k0 * 4 * n2;
+    const int strip = (z0 >> 6) + S.snap_nbz * ((x0 >> S.snap_sw) + ty);
+#pragma unroll
+    for (int a = 0; a < 2; ++a) {
+      float m = 0.f;
+#pragma unroll
+      for (int j = 0; j < TXR; ++j) m = fmaxf(m, fabsf(sv[a][j]));
+      m = wave_max(m);
+      const float inv = m > 0.f ? SnapQ<SNAP>::Q / m : 0.f;
+      if (tz == 0) {
+        S.snap_scale[(k0 * 4 + a) * S.snap_nstrip + strip] = m / SnapQ<SNAP>::Q;
+        S.snap_scale[(k0 * 4 + a + 2) * S.snap_nstrip + strip] = 0.f;
+      }
+#pragma unroll
+      for (int j = 0; j < TXR; ++j) {
+        const int gx = x0 + ty * TXR + j;
+        if (!zok || gx >= nx) continue;
+        const long long gi = (long long)gz + (long long)ldz * gx;
+        q4[a * n2 + gi] = SnapQ<SNAP>::enc(sv[a][j] * inv);
+        q4[(a + 2) * n2 + gi] = 0;
+      }
+    }
+  }
+}
+
+// Source Vx (a=0) or Vz (a=1) of imaging level L at (gz, gx), decoded.
+template <int SNAP>
+__device__ __forceinline__ float src_field(const ShotDev& S, int a, int L, long long gi, int gz,
+                                           int gx) {
+  const long long n2 = S.n2;
+  if (SNAP == SNAP_HIST) return static_cast<const float*>(S.snap)[((long long)(L - 1) * 2 + a) * n2 + gi];
+  const long long k0 = L / S.snap_every - 1;
+  if (SNAP == SNAP_F32) return static_cast<const float*>(S.snap)[(k0 * 4 + a) * n2 + gi];
+  const int strip = (gz >> 6) + S.snap_nbz * (gx >> S.snap_sw);
+  const float s = S.snap_scale[(k0 * 4 + a) * S.snap_nstrip + strip];
+  return SnapQ<SNAP>::dec(static_cast<const unsigned short*>(S.snap)[(k0 * 4 + a) * n2 + gi]) * s;
 }
 
 // ---------------------------------------------------------------------------
 // Adjoint step (gather form of rtm_op3_c_, W/rtmlib-modC.c:120-253, with the
 // receiver injection of W/rtmlib-mod.f90:99-146):
-//   Vx  = K'(inj + Vx(it+2)  - S-x(m2m3 sxx + m2 szz) - S+z(m3 sxz))
-//   Vz  = K'(inj + Vz(it+2)  - S-z(m2 sxx + m2m3 szz) - S+x(m3 sxz))
-//   sxx = K'(inj + sxx(it+2) - S+x(m1_x Vx))
-//   szz = K'(inj + szz(it+2) - S+z(m1_z Vz))
-//   sxz = K'(inj + sxz(it+2) - S-z(m1_x Vx) - S-x(m1_z Vz))
-// products of level it+1. K' is the sponge (exact adjoint) or identity.
-// With do_image, the Lame imaging condition of this level
-// (W/image-mod-rtm.f90:154-162, S1..S6 of :26-67) is accumulated in the same
-// pass from the source Vx, Vz history.
-template <int TXR>
-__global__ __launch_bounds__(256) void k_adj_step(const StepArgs a, int nbz, int ntiles) {
+//   Vx  = K'(b + - S-x(m2m3 sxx + m2 szz) - S+z(m3 sxz))
+//   Vz  = K'(b + - S-z(m2 sxx + m2m3 szz) - S+x(m3 sxz))
+//   sxx = K'(b - S+x(m1_x Vx))
+//   szz = K'(b - S+z(m1_z Vz))
+//   sxz = K'(b - S-z(m1_x Vx) - S-x(m1_z Vz))
+// products of level L+1, b = (injected row or the level's own content) +
+// level L+2. K' is the sponge (exact adjoint) or identity (the drop-in).
+// With A_IMG the Lame imaging condition of level L (W/image-mod-rtm.f90:
+// 154-162 with S1..S6 of :26-67) is accumulated in the same pass.
+template <int TXR, int SNAP>
+__global__ __launch_bounds__(256) void k_adj(const ShotDev* __restrict__ shots,
+                                             const TileRef* __restrict__ tiles, int ntiles, int L,
+                                             int flags, int xcd) {
   using T = Tile<TXR>;
   __shared__ float sh[5][T::N];
-  const int t = a.xcd_remap ? xcd_tile(blockIdx.x, ntiles) : (int)blockIdx.x;
-  const int z0 = (t % nbz) * T::TZ, x0 = (t / nbz) * T::TX;
+  const TileRef tr = tiles[xcd ? xcd_tile(blockIdx.x, ntiles) : (int)blockIdx.x];
+  const ShotDev& S = shots[tr.shot];
+  const int nz = S.nz, nx = S.nx, ldz = S.ldz;
+  const long long fs = S.fstride, n2 = S.n2;
+  float* const out = lev_ptr(S.lev, S.lev_stride, S.lev_mod, S.lev_off, L);
+  const float* const x2 = lev_ptr(S.lev, S.lev_stride, S.lev_mod, S.lev_off, L + 2);
+  const float* const y1 = lev_ptr(S.lev, S.lev_stride, S.lev_mod, S.lev_off, L + 1);
+  const float* const coef = S.coef;
+  const int z0 = tr.z0, x0 = tr.x0;
   const int tz = threadIdx.x, ty = threadIdx.y;
 
   for (int lx = ty; lx < T::LX; lx += T::WY) {
     const int gx = x0 - T::H + lx;
-    const bool xin = gx >= 0 && gx < a.nx;
+    const bool xin = gx >= 0 && gx < nx;
     for (int lz = tz; lz < T::LZ; lz += T::TZ) {
       const int gz = z0 - T::H + lz;
-      const bool in = xin && gz >= 0 && gz < a.nz;
+      const bool in = xin && gz >= 0 && gz < nz;
       const int p = lx * T::LZ + lz;
       float p0 = 0.f, p1 = 0.f, p2 = 0.f, p3 = 0.f, p4 = 0.f;
       if (in) {
-        const size_t gi = (size_t)gz + (size_t)a.ldz * gx;
-        const float vx = a.y1[0][gi], vz = a.y1[1][gi], xx = a.y1[2][gi], zz = a.y1[3][gi],
-                    xz = a.y1[4][gi];
-        const float m1x = a.coef[0][gi], m1z = a.coef[1][gi], m2 = a.coef[2][gi],
-                    m3 = a.coef[3][gi], m2m3 = a.coef[4][gi];
+        const long long gi = (long long)gz + (long long)ldz * gx;
+        const float vx = y1[gi], vz = y1[fs + gi], xx = y1[2 * fs + gi], zz = y1[3 * fs + gi],
+                    xz = y1[4 * fs + gi];
+        const float m1x = coef[gi], m1z = coef[n2 + gi], m2 = coef[2 * n2 + gi],
+                    m3 = coef[3 * n2 + gi], m2m3 = coef[4 * n2 + gi];
         p0 = m2m3 * xx + m2 * zz;
         p1 = m2 * xx + m2m3 * zz;
         p2 = m3 * xz;
@@ -213,31 +396,33 @@ __global__ __launch_bounds__(256) void k_adj_step(const StepArgs a, int nbz, int
   __syncthreads();
 
   const int gz = z0 + tz;
-  const bool zok = gz < a.nz;
-  const float kz = (a.kz && zok) ? a.kz[gz] : 1.f;
-  const bool row = a.rec_in[0] && gz == a.zr;
+  const bool zok = gz < nz;
+  const float kz = ((flags & A_K) && zok) ? S.kz[gz] : 1.f;
+  const bool row = (flags & A_INJ) && gz == S.zr;
   float o[TXR][5];
   constexpr int SX = T::LZ;
 #pragma unroll
   for (int j = 0; j < TXR; ++j) {
     const int lxo = ty * TXR + j;
     const int gx = x0 + lxo;
-    const bool ok = zok && gx < a.nx;
+    const bool ok = zok && gx < nx;
     const int p = (lxo + T::H) * T::LZ + (tz + T::H);
 #pragma unroll
     for (int f = 0; f < 5; ++f) o[j][f] = 0.f;
     if (!ok) continue;
-    const size_t gi = (size_t)gz + (size_t)a.ldz * gx;
+    const long long gi = (long long)gz + (long long)ldz * gx;
     float b[5];
-    if (a.xin[0]) {
+    if (flags & A_XIN) {
 #pragma unroll
-      for (int f = 0; f < 5; ++f) b[f] = a.xin[f][gi] + a.x2[f][gi];
-    } else if (row && gx >= a.rx0 && gx < a.rx0 + a.nrx) {
+      for (int f = 0; f < 5; ++f) b[f] = out[f * fs + gi] + x2[f * fs + gi];
+    } else if (row && gx >= S.rx0 && gx < S.rx0 + S.nrx) {
+      const float* r = S.rec + (long long)(L - 1) * S.nrx + (gx - S.rx0);
+      const long long rs = (long long)S.nt * S.nrx;
 #pragma unroll
-      for (int f = 0; f < 5; ++f) b[f] = a.rec_in[f][gx - a.rx0] + a.x2[f][gi];
+      for (int f = 0; f < 5; ++f) b[f] = r[f * rs] + x2[f * fs + gi];
     } else {
 #pragma unroll
-      for (int f = 0; f < 5; ++f) b[f] = a.x2[f][gi];
+      for (int f = 0; f < 5; ++f) b[f] = x2[f * fs + gi];
     }
     float v0 = b[0] - sm<SX>(sh[0], p);
     v0 -= sp<1>(sh[2], p);
@@ -248,134 +433,152 @@ __global__ __launch_bounds__(256) void k_adj_step(const StepArgs a, int nbz, int
     float v4 = b[4] - sm<1>(sh[3], p);
     v4 -= sm<SX>(sh[4], p);
     float v[5] = {v0, v1, v2, v3, v4};
-    if (a.kz) {
-      const float kx = a.kx[gx];
+    if (flags & A_K) {
+      const float kx = S.kx[gx];
 #pragma unroll
       for (int f = 0; f < 5; ++f) v[f] = (v[f] * kz) * kx;
     }
 #pragma unroll
     for (int f = 0; f < 5; ++f) {
-      a.out[f][gi] = v[f];
+      out[f * fs + gi] = v[f];
       o[j][f] = v[f];
     }
   }
 
-  if (!a.do_image) return;  // uniform over the grid: no barrier is skipped unevenly
+  if (!(flags & A_IMG)) return;  // uniform over the launch
   __syncthreads();
   for (int lx = ty; lx < T::LX; lx += T::WY) {
     const int gx = x0 - T::H + lx;
-    const bool xin = gx >= 0 && gx < a.nx;
+    const bool xin = gx >= 0 && gx < nx;
     for (int lz = tz; lz < T::LZ; lz += T::TZ) {
       const int gz2 = z0 - T::H + lz;
-      const bool in = xin && gz2 >= 0 && gz2 < a.nz;
-      const size_t gi = (size_t)gz2 + (size_t)a.ldz * gx;
+      const bool in = xin && gz2 >= 0 && gz2 < nz;
+      const long long gi = (long long)gz2 + (long long)ldz * gx;
       const int p = lx * T::LZ + lz;
-      sh[0][p] = in ? a.svx[gi] : 0.f;
-      sh[1][p] = in ? a.svz[gi] : 0.f;
+      sh[0][p] = in ? src_field<SNAP>(S, 0, L, gi, gz2, gx) : 0.f;
+      sh[1][p] = in ? src_field<SNAP>(S, 1, L, gi, gz2, gx) : 0.f;
     }
   }
   __syncthreads();
-  const bool zin4 = gz >= 4 && gz <= a.nz - 6, zin5 = gz >= 5 && gz <= a.nz - 6;
+  const bool next = flags & A_NEXT;
+  const bool zin4 = gz >= 4 && gz <= nz - 6, zin5 = gz >= 5 && gz <= nz - 6;
+  float* const img = S.img;
+  float* const imgc = S.imgc;
+  const float w = S.img_w, inv_dx = S.inv_dx, inv_dt = S.inv_dt;
 #pragma unroll
   for (int j = 0; j < TXR; ++j) {
     const int lxo = ty * TXR + j;
     const int gx = x0 + lxo;
-    if (!zok || gx >= a.nx) continue;
+    if (!zok || gx >= nx) continue;
     const int p = (lxo + T::H) * T::LZ + (tz + T::H);
-    const size_t gi = (size_t)gz + (size_t)a.ldz * gx;
+    const long long gi = (long long)gz + (long long)ldz * gx;
     // S1, S2: forward time difference (T, W/image-mod-rtm.f90:288-297)
     float s1 = 0.f, s2 = 0.f;
-    if (a.svx1) {
-      s1 = a.inv_dt * (a.svx1[gi] - sh[0][p]);
-      s2 = a.inv_dt * (a.svz1[gi] - sh[1][p]);
+    if (next) {
+      if (SNAP == SNAP_HIST) {
+        const float* h1 = static_cast<const float*>(S.snap) + (long long)L * 2 * n2;
+        s1 = inv_dt * (h1[gi] - sh[0][p]);
+        s2 = inv_dt * (h1[n2 + gi] - sh[1][p]);
+      } else {
+        s1 = inv_dt * src_field<SNAP>(S, 2, L, gi, gz, gx);
+        s2 = inv_dt * src_field<SNAP>(S, 3, L, gi, gz, gx);
+      }
     }
     // S4 = vxdx, S5 = vzdz (range 4..n-6), S6 = vxdz + vzdx (range 5..n-6)
     float s4 = 0.f, s5 = 0.f, s6 = 0.f;
-    if (zin4 && gx >= 4 && gx <= a.nx - 6) {
-      s4 = sp<SX>(sh[0], p) * a.inv_dx;
-      s5 = sp<1>(sh[1], p) * a.inv_dx;
+    if (zin4 && gx >= 4 && gx <= nx - 6) {
+      s4 = sp<SX>(sh[0], p) * inv_dx;
+      s5 = sp<1>(sh[1], p) * inv_dx;
     }
-    if (zin5 && gx >= 5 && gx <= a.nx - 6) {
-      s6 = sm<1>(sh[0], p) * a.inv_dx;
-      s6 += sm<SX>(sh[1], p) * a.inv_dx;
+    if (zin5 && gx >= 5 && gx <= nx - 6) {
+      s6 = sm<1>(sh[0], p) * inv_dx;
+      s6 += sm<SX>(sh[1], p) * inv_dx;
     }
     const float s3 = s4 + s5;
     const float* v = o[j];
-    const float w = a.img_w;
     const float t1 = w * (v[0] * s1 + v[1] * s2);
     const float t2 = -w * (v[2] * s3 + v[3] * s3);
     const float t3 = -w * (v[2] * 2.f * s4 + v[3] * 2.f * s5 + v[4] * s6);
-    if (a.comp) {
-      float s = a.img[0][gi], c = a.imgc[0][gi];
+    if (flags & A_COMP) {
+      float s = img[gi], c = imgc[gi];
       neumaier(s, c, t1);
-      a.img[0][gi] = s; a.imgc[0][gi] = c;
-      s = a.img[1][gi]; c = a.imgc[1][gi];
+      img[gi] = s;
+      imgc[gi] = c;
+      s = img[n2 + gi];
+      c = imgc[n2 + gi];
       neumaier(s, c, t2);
-      a.img[1][gi] = s; a.imgc[1][gi] = c;
-      s = a.img[2][gi]; c = a.imgc[2][gi];
+      img[n2 + gi] = s;
+      imgc[n2 + gi] = c;
+      s = img[2 * n2 + gi];
+      c = imgc[2 * n2 + gi];
       neumaier(s, c, t3);
-      a.img[2][gi] = s; a.imgc[2][gi] = c;
+      img[2 * n2 + gi] = s;
+      imgc[2 * n2 + gi] = c;
     } else {
-      a.img[0][gi] += t1;
-      a.img[1][gi] += t2;
-      a.img[2][gi] += t3;
+      img[gi] += t1;
+      img[n2 + gi] += t2;
+      img[2 * n2 + gi] += t3;
     }
   }
 }
 
 // ---------------------------------------------------------------------------
-// Born scattering sources of one level, image_op2 forward branch
-// (W/image-mod-rtm.f90:171-179) with S1..S6 of the background level.
-__global__ __launch_bounds__(256) void k_born_sources(const BornArgs a) {
-  constexpr int TXR = 4;
-  using T = Tile<TXR>;
+// Born scattering sources of level L, image_op2 forward branch
+// (W/image-mod-rtm.f90:171-179) from the background S1..S6 (fp32 history).
+__global__ __launch_bounds__(256) void k_born_sources(const ShotDev* __restrict__ shots,
+                                                      const TileRef* __restrict__ tiles, int L) {
+  using T = Tile<4>;
   __shared__ float sh[2][T::N];
-  const int nbz = (a.nz + T::TZ - 1) / T::TZ;
-  const int t = blockIdx.x;
-  const int z0 = (t % nbz) * T::TZ, x0 = (t / nbz) * T::TX;
-  const int tz = threadIdx.x, ty = threadIdx.y;
+  const TileRef tr = tiles[blockIdx.x];
+  const ShotDev& S = shots[tr.shot];
+  const int nz = S.nz, nx = S.nx, ldz = S.ldz;
+  const long long n2 = S.n2;
+  const float* h = static_cast<const float*>(S.snap) + (long long)(L - 1) * 2 * n2;
+  const float* h1 = L < S.nt ? h + 2 * n2 : nullptr;
+  const int z0 = tr.z0, x0 = tr.x0, tz = threadIdx.x, ty = threadIdx.y;
   for (int lx = ty; lx < T::LX; lx += T::WY) {
     const int gx = x0 - T::H + lx;
-    const bool xin = gx >= 0 && gx < a.nx;
+    const bool xin = gx >= 0 && gx < nx;
     for (int lz = tz; lz < T::LZ; lz += T::TZ) {
       const int gz = z0 - T::H + lz;
-      const bool in = xin && gz >= 0 && gz < a.nz;
-      const size_t gi = (size_t)gz + (size_t)a.ldz * gx;
+      const bool in = xin && gz >= 0 && gz < nz;
+      const long long gi = (long long)gz + (long long)ldz * gx;
       const int p = lx * T::LZ + lz;
-      sh[0][p] = in ? a.svx[gi] : 0.f;
-      sh[1][p] = in ? a.svz[gi] : 0.f;
+      sh[0][p] = in ? h[gi] : 0.f;
+      sh[1][p] = in ? h[n2 + gi] : 0.f;
     }
   }
   __syncthreads();
   const int gz = z0 + tz;
-  if (gz >= a.nz) return;
+  if (gz >= nz) return;
   constexpr int SX = T::LZ;
-  for (int j = 0; j < TXR; ++j) {
-    const int lxo = ty * TXR + j;
+  for (int j = 0; j < 4; ++j) {
+    const int lxo = ty * 4 + j;
     const int gx = x0 + lxo;
-    if (gx >= a.nx) break;
+    if (gx >= nx) break;
     const int p = (lxo + T::H) * T::LZ + (tz + T::H);
-    const size_t gi = (size_t)gz + (size_t)a.ldz * gx;
+    const long long gi = (long long)gz + (long long)ldz * gx;
     float s1 = 0.f, s2 = 0.f, s4 = 0.f, s5 = 0.f, s6 = 0.f;
-    if (a.svx1) {
-      s1 = a.inv_dt * (a.svx1[gi] - sh[0][p]);
-      s2 = a.inv_dt * (a.svz1[gi] - sh[1][p]);
+    if (h1) {
+      s1 = S.inv_dt * (h1[gi] - sh[0][p]);
+      s2 = S.inv_dt * (h1[n2 + gi] - sh[1][p]);
     }
-    if (gz >= 4 && gz <= a.nz - 6 && gx >= 4 && gx <= a.nx - 6) {
-      s4 = sp<SX>(sh[0], p) * a.inv_dx;
-      s5 = sp<1>(sh[1], p) * a.inv_dx;
+    if (gz >= 4 && gz <= nz - 6 && gx >= 4 && gx <= nx - 6) {
+      s4 = sp<SX>(sh[0], p) * S.inv_dx;
+      s5 = sp<1>(sh[1], p) * S.inv_dx;
     }
-    if (gz >= 5 && gz <= a.nz - 6 && gx >= 5 && gx <= a.nx - 6) {
-      s6 = sm<1>(sh[0], p) * a.inv_dx;
-      s6 += sm<SX>(sh[1], p) * a.inv_dx;
+    if (gz >= 5 && gz <= nz - 6 && gx >= 5 && gx <= nx - 6) {
+      s6 = sm<1>(sh[0], p) * S.inv_dx;
+      s6 += sm<SX>(sh[1], p) * S.inv_dx;
     }
     const float s3 = s4 + s5;
-    const float d1 = a.dm[0][gi], d2 = a.dm[1][gi], d3 = a.dm[2][gi];
-    a.src[0][gi] = d1 * s1;
-    a.src[1][gi] = d1 * s2;
-    a.src[2][gi] = -(d2 * s3) - (d3 * 2.f * s4);
-    a.src[3][gi] = -(d2 * s3) - (d3 * 2.f * s5);
-    a.src[4][gi] = -(d3 * s6);
+    const float d1 = S.dm[gi], d2 = S.dm[n2 + gi], d3 = S.dm[2 * n2 + gi];
+    float* src = S.bsrc;
+    src[gi] = d1 * s1;
+    src[n2 + gi] = d1 * s2;
+    src[2 * n2 + gi] = -(d2 * s3) - (d3 * 2.f * s4);
+    src[3 * n2 + gi] = -(d2 * s3) - (d3 * 2.f * s5);
+    src[4 * n2 + gi] = -(d3 * s6);
   }
 }
 
@@ -399,52 +602,102 @@ __global__ void k_image_finalize(float* s, const float* c, size_t n) {
     s[i] = (float)((double)s[i] + (double)c[i]);
 }
 
+__global__ void k_stack(double* stack, int snz, int snx, const float* img, int ldz, long long n2,
+                        int iz0, int ix0, int nz, int nx, int z0, int x0) {
+  const int iz = blockIdx.x * blockDim.x + threadIdx.x;
+  const int ix = blockIdx.y;
+  if (iz >= nz || ix >= nx) return;
+  const long long src = (long long)(iz0 + iz) + (long long)ldz * (ix0 + ix);
+  const long long dst = (long long)(z0 + iz) + (long long)snz * (x0 + ix);
+  const long long sn = (long long)snz * snx;
+#pragma unroll
+  for (int c = 0; c < 3; ++c) stack[c * sn + dst] += (double)img[c * n2 + src];
+}
+
 // ---------------------------------------------------------------------------
 template <int TXR>
-static int launch_fwd_t(const StepArgs& a, hipStream_t st) {
-  using T = Tile<TXR>;
-  const int nbz = (a.nz + T::TZ - 1) / T::TZ;
-  const int nbx = (a.nx + T::TX - 1) / T::TX;
-  const int nt = nbz * nbx;
-  hipLaunchKernelGGL(k_fwd_step<TXR>, dim3(nt), dim3(T::TZ, T::WY), 0, st, a, nbz, nt);
-  HIPCHK(hipGetLastError());
-  return OK;
+static void fwd_dispatch(int snap, dim3 g, hipStream_t st, const ShotDev* s, const TileRef* t,
+                         int n, int L, int fl, int xcd) {
+  const dim3 b(TILE_Z, TILE_WY);
+  switch (snap) {
+    case SNAP_HIST: hipLaunchKernelGGL((k_fwd<TXR, SNAP_HIST>), g, b, 0, st, s, t, n, L, fl, xcd); break;
+    case SNAP_F32: hipLaunchKernelGGL((k_fwd<TXR, SNAP_F32>), g, b, 0, st, s, t, n, L, fl, xcd); break;
+    case SNAP_BF16: hipLaunchKernelGGL((k_fwd<TXR, SNAP_BF16>), g, b, 0, st, s, t, n, L, fl, xcd); break;
+    default: hipLaunchKernelGGL((k_fwd<TXR, SNAP_F16>), g, b, 0, st, s, t, n, L, fl, xcd); break;
+  }
 }
 template <int TXR>
-static int launch_adj_t(const StepArgs& a, hipStream_t st) {
-  using T = Tile<TXR>;
-  const int nbz = (a.nz + T::TZ - 1) / T::TZ;
-  const int nbx = (a.nx + T::TX - 1) / T::TX;
-  const int nt = nbz * nbx;
-  hipLaunchKernelGGL(k_adj_step<TXR>, dim3(nt), dim3(T::TZ, T::WY), 0, st, a, nbz, nt);
+static void adj_dispatch(int snap, dim3 g, hipStream_t st, const ShotDev* s, const TileRef* t,
+                         int n, int L, int fl, int xcd) {
+  const dim3 b(TILE_Z, TILE_WY);
+  switch (snap) {
+    case SNAP_HIST: hipLaunchKernelGGL((k_adj<TXR, SNAP_HIST>), g, b, 0, st, s, t, n, L, fl, xcd); break;
+    case SNAP_F32: hipLaunchKernelGGL((k_adj<TXR, SNAP_F32>), g, b, 0, st, s, t, n, L, fl, xcd); break;
+    case SNAP_BF16: hipLaunchKernelGGL((k_adj<TXR, SNAP_BF16>), g, b, 0, st, s, t, n, L, fl, xcd); break;
+    default: hipLaunchKernelGGL((k_adj<TXR, SNAP_F16>), g, b, 0, st, s, t, n, L, fl, xcd); break;
+  }
+}
+template <int TXR>
+static void snap_last_dispatch(int snap, dim3 g, hipStream_t st, const ShotDev* s,
+                               const TileRef* t, int L) {
+  const dim3 b(TILE_Z, TILE_WY);
+  switch (snap) {
+    case SNAP_F32: hipLaunchKernelGGL((k_snap_last<TXR, SNAP_F32>), g, b, 0, st, s, t, L); break;
+    case SNAP_BF16: hipLaunchKernelGGL((k_snap_last<TXR, SNAP_BF16>), g, b, 0, st, s, t, L); break;
+    case SNAP_F16: hipLaunchKernelGGL((k_snap_last<TXR, SNAP_F16>), g, b, 0, st, s, t, L); break;
+    default: break;
+  }
+}
+
+int launch_fwd(const ShotDev* shots, const TileRef* tiles, int ntiles, int L, int flags, int txr,
+               int snap, int xcd, hipStream_t st) {
+  if (ntiles <= 0) return OK;
+  const dim3 g(ntiles);
+  switch (txr) {
+    case 1: fwd_dispatch<1>(snap, g, st, shots, tiles, ntiles, L, flags, xcd); break;
+    case 2: fwd_dispatch<2>(snap, g, st, shots, tiles, ntiles, L, flags, xcd); break;
+    case 4: fwd_dispatch<4>(snap, g, st, shots, tiles, ntiles, L, flags, xcd); break;
+    case 8: fwd_dispatch<8>(snap, g, st, shots, tiles, ntiles, L, flags, xcd); break;
+    default: set_error("unsupported tile width %d", txr); return E_ARG;
+  }
   HIPCHK(hipGetLastError());
   return OK;
 }
 
-int launch_fwd_step(const StepArgs& a, int txr, hipStream_t st) {
+int launch_adj(const ShotDev* shots, const TileRef* tiles, int ntiles, int L, int flags, int txr,
+               int snap, int xcd, hipStream_t st) {
+  if (ntiles <= 0) return OK;
+  const dim3 g(ntiles);
   switch (txr) {
-    case 1: return launch_fwd_t<1>(a, st);
-    case 2: return launch_fwd_t<2>(a, st);
-    case 4: return launch_fwd_t<4>(a, st);
-    case 8: return launch_fwd_t<8>(a, st);
+    case 1: adj_dispatch<1>(snap, g, st, shots, tiles, ntiles, L, flags, xcd); break;
+    case 2: adj_dispatch<2>(snap, g, st, shots, tiles, ntiles, L, flags, xcd); break;
+    case 4: adj_dispatch<4>(snap, g, st, shots, tiles, ntiles, L, flags, xcd); break;
+    case 8: adj_dispatch<8>(snap, g, st, shots, tiles, ntiles, L, flags, xcd); break;
     default: set_error("unsupported tile width %d", txr); return E_ARG;
   }
-}
-int launch_adj_step(const StepArgs& a, int txr, hipStream_t st) {
-  switch (txr) {
-    case 1: return launch_adj_t<1>(a, st);
-    case 2: return launch_adj_t<2>(a, st);
-    case 4: return launch_adj_t<4>(a, st);
-    case 8: return launch_adj_t<8>(a, st);
-    default: set_error("unsupported tile width %d", txr); return E_ARG;
-  }
+  HIPCHK(hipGetLastError());
+  return OK;
 }
 
-int launch_born_sources(const BornArgs& a, hipStream_t st) {
-  using T = Tile<4>;
-  const int nbz = (a.nz + T::TZ - 1) / T::TZ;
-  const int nbx = (a.nx + T::TX - 1) / T::TX;
-  hipLaunchKernelGGL(k_born_sources, dim3(nbz * nbx), dim3(T::TZ, T::WY), 0, st, a);
+int launch_snap_last(const ShotDev* shots, const TileRef* tiles, int ntiles, int L, int txr,
+                     int snap, hipStream_t st) {
+  if (ntiles <= 0 || snap == SNAP_HIST) return OK;
+  const dim3 g(ntiles);
+  switch (txr) {
+    case 1: snap_last_dispatch<1>(snap, g, st, shots, tiles, L); break;
+    case 2: snap_last_dispatch<2>(snap, g, st, shots, tiles, L); break;
+    case 4: snap_last_dispatch<4>(snap, g, st, shots, tiles, L); break;
+    case 8: snap_last_dispatch<8>(snap, g, st, shots, tiles, L); break;
+    default: set_error("unsupported tile width %d", txr); return E_ARG;
+  }
+  HIPCHK(hipGetLastError());
+  return OK;
+}
+
+int launch_born_sources(const ShotDev* shots, const TileRef* tiles, int ntiles, int L,
+                        hipStream_t st) {
+  if (ntiles <= 0) return OK;
+  hipLaunchKernelGGL(k_born_sources, dim3(ntiles), dim3(TILE_Z, TILE_WY), 0, st, shots, tiles, L);
   HIPCHK(hipGetLastError());
   return OK;
 }
@@ -470,6 +723,15 @@ int launch_image_finalize(float* s, const float* c, size_t n, hipStream_t st) {
   size_t blocks = (n + 255) / 256;
   if (blocks > 65536) blocks = 65536;
   hipLaunchKernelGGL(k_image_finalize, dim3((unsigned)blocks), dim3(256), 0, st, s, c, n);
+  HIPCHK(hipGetLastError());
+  return OK;
+}
+
+int launch_stack(double* stack, int snz, int snx, const float* img, int ldz, long long n2,
+                 int iz0, int ix0, int nz, int nx, int z0, int x0, hipStream_t st) {
+  if (nz <= 0 || nx <= 0) return OK;
+  hipLaunchKernelGGL(k_stack, dim3((nz + 255) / 256, nx), dim3(256), 0, st, stack, snz, snx, img,
+                     ldz, n2, iz0, ix0, nz, nx, z0, x0);
   HIPCHK(hipGetLastError());
   return OK;
 }

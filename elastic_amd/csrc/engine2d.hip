@@ -1,0 +1,565 @@
+// engine2d.hip -- multi-shot 2D elastic RTM engine (host side).
+#include "engine2d.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdlib>
+#include <cstring>
+
+namespace erm {
+
+static int env_int(const char* name, int dflt) {
+  const char* v = getenv(name);
+  return v ? atoi(v) : dflt;
+}
+
+int ensure_device() {
+  static int state = 0;  // 0 unknown, 1 ok, -1 none
+  if (state == 0) {
+    int n = 0;
+    hipError_t e = hipGetDeviceCount(&n);
+    state = (e == hipSuccess && n > 0) ? 1 : -1;
+    if (state < 0) set_error("no HIP device available (%s)", hipGetErrorString(e));
+  }
+  return state > 0 ? OK : E_HIP;
+}
+
+// Tile width (x-cells per thread) of the step kernels and XCD-aware tile
+// numbering; RTM_TXR / RTM_XCD_REMAP override them (autotuner, experiments).
+int default_txr() { return env_int("RTM_TXR", 4); }
+int default_xcd() { return env_int("RTM_XCD_REMAP", 1); }
+
+// Sponge taper, W/abc-mod.f90:35-50, and per-row/column factors of K
+// (:59-78): rows first then columns, so corners are damped twice.
+static std::vector<float> abc_filter(int abc) {
+  std::vector<float> f(abc + 5);
+  const float alfa = 0.09f;
+  for (int i = 1; i <= abc; ++i) {
+    const float aux = (1.0f * (float)i) / (1.0f * (float)abc);
+    f[i - 1] = std::exp(-alfa * aux * aux);
+  }
+  for (int i = abc + 1; i <= abc + 5; ++i) f[i - 1] = f[abc - 1];
+  return f;
+}
+
+void sponge_factors(int n, int abc, std::vector<float>& k) {
+  k.assign(n, 1.0f);
+  if (abc <= 0) return;
+  const int abc2 = abc + 5;
+  const std::vector<float> f = abc_filter(abc);
+  for (int i1 = 1; i1 <= abc2; ++i1) {
+    const int top = i1 - 1;
+    if (top < n) k[top] = k[top] * f[abc2 - i1];
+    const int bot = n - abc2 + i1 - 1;
+    if (bot >= 0 && bot < n) k[bot] = k[bot] * f[i1 - 1];
+  }
+}
+
+// Each coefficient array is zero outside the 0-based index range of the
+// operators that use it (W/rtmlib-mod.f90, 1-based: Ax/Bx ix 6..nx-5, iz
+// 5..nz-5; Az/Bz ix 5..nx-5, iz 6..nz-5; D/E/F/G 5..n-5; H/J 6..n-5).
+static const int kMaskIx0[5] = {5, 4, 4, 5, 4};
+static const int kMaskIz0[5] = {4, 5, 4, 5, 4};
+
+int fill_masked_coefs(const float* coef5, int nz, int nx, int ldz, float* dev) {
+  const size_t n2 = (size_t)ldz * nx;
+  std::vector<float> h(5 * n2, 0.0f);
+  for (int c = 0; c < 5; ++c) {
+    const float* src = coef5 + (size_t)c * nz * nx;
+    float* dst = h.data() + c * n2;
+    for (int ix = kMaskIx0[c]; ix <= nx - 6; ++ix)
+      for (int iz = kMaskIz0[c]; iz <= nz - 6; ++iz) dst[iz + (size_t)ldz * ix] = src[iz + (size_t)nz * ix];
+  }
+  HIPCHK(hipMemcpy(dev, h.data(), h.size() * sizeof(float), hipMemcpyHostToDevice));
+  return OK;
+}
+
+static int nstrips(int nz, int nx, int txr) {
+  const int nbz = (nz + TILE_Z - 1) / TILE_Z;
+  const int tx = tile_x(txr);
+  const int nbx = (nx + tx - 1) / tx;
+  return nbz * nbx * TILE_WY;
+}
+
+Engine::~Engine() {
+  destroy_graphs();
+  for (auto& e : ev_)
+    if (e) (void)hipEventDestroy(e);
+  if (st_) (void)hipStreamDestroy(st_);
+}
+
+void Engine::destroy_graphs() {
+  for (auto& g : groups_) {
+    if (g.gfwd) (void)hipGraphExecDestroy(g.gfwd);
+    if (g.gadj) (void)hipGraphExecDestroy(g.gadj);
+    if (g.gsyn) (void)hipGraphExecDestroy(g.gsyn);
+    g.gfwd = g.gadj = g.gsyn = nullptr;
+  }
+}
+
+int Engine::setup(const EngineCfg& cfg, const std::vector<ShotSpec>& shots,
+                  const std::vector<const float*>& coef5, const float* wavelet) {
+  STCHK(ensure_device());
+  cfg_ = cfg;
+  specs_ = shots;
+  if (shots.empty() || coef5.size() != shots.size() || !wavelet) {
+    set_error("engine setup: no shots, coefficient list mismatch or null wavelet");
+    return E_ARG;
+  }
+  if (cfg_.nt < 1 || cfg_.snap_every < 1 || !(cfg_.dx > 0) || !(cfg_.dt > 0)) {
+    set_error("engine setup: nt, snap_every must be >= 1 and dx, dt > 0");
+    return E_ARG;
+  }
+  if (cfg_.txr != 1 && cfg_.txr != 2 && cfg_.txr != 4 && cfg_.txr != 8) {
+    set_error("tile width %d not in {1,2,4,8}", cfg_.txr);
+    return E_ARG;
+  }
+  if (cfg_.snap_fmt == SNAP_HIST && cfg_.snap_every != 1) cfg_.snap_fmt = SNAP_F32;
+  HIPCHK(hipStreamCreateWithFlags(&st_, hipStreamNonBlocking));
+  for (auto& e : ev_) HIPCHK(hipEventCreate(&e));
+  res_.resize(shots.size());
+  for (size_t s = 0; s < shots.size(); ++s) {
+    const ShotSpec& p = shots[s];
+    if (p.nz < 12 || p.nx < 12) {
+      set_error("shot %zu: grid %d x %d too small (the 10th-order operators need >= 12)", s, p.nz, p.nx);
+      return E_ARG;
+    }
+    if (p.xs < 0 || p.xs >= p.nx || p.zs < 0 || p.zs >= p.nz) {
+      set_error("shot %zu: source cell (%d,%d) outside the %d x %d grid", s, p.zs, p.xs, p.nz, p.nx);
+      return E_ARG;
+    }
+    if (p.zr < 0 || p.zr >= p.nz || p.rx0 < 0 || p.nrx < 0 || p.rx0 + p.nrx > p.nx) {
+      set_error("shot %zu: receiver row %d / columns [%d,%d) outside the grid", s, p.zr, p.rx0, p.rx0 + p.nrx);
+      return E_ARG;
+    }
+    if (cfg_.abc > 0 && (2 * (cfg_.abc + 5) > p.nz || 2 * (cfg_.abc + 5) > p.nx)) {
+      set_error("shot %zu: sponge of %d cells per edge does not fit %d x %d", s, cfg_.abc + 5, p.nz, p.nx);
+      return E_ARG;
+    }
+    if (!coef5[s]) { set_error("shot %zu: null coefficients", s); return E_ARG; }
+    Res& r = res_[s];
+    r.ldz = device_ldz(p.nz);
+    r.n2 = (long long)r.ldz * p.nx;
+    STCHK(r.coef.alloc(5 * r.n2 * sizeof(float), "coefficients"));
+    STCHK(fill_masked_coefs(coef5[s], p.nz, p.nx, r.ldz, r.coef.as<float>()));
+    if (cfg_.abc > 0) {
+      std::vector<float> hz, hx;
+      sponge_factors(p.nz, cfg_.abc, hz);
+      sponge_factors(p.nx, cfg_.abc, hx);
+      STCHK(r.kfac.alloc((size_t)(p.nz + p.nx) * sizeof(float), "sponge factors"));
+      HIPCHK(hipMemcpy(r.kfac.p, hz.data(), p.nz * sizeof(float), hipMemcpyHostToDevice));
+      HIPCHK(hipMemcpy(r.kfac.as<float>() + p.nz, hx.data(), p.nx * sizeof(float), hipMemcpyHostToDevice));
+    }
+    STCHK(r.rec.alloc((size_t)5 * cfg_.nt * std::max(p.nrx, 1) * sizeof(float), "receiver data"));
+    STCHK(launch_fill(r.rec.as<float>(), (size_t)5 * cfg_.nt * std::max(p.nrx, 1), 0.f, st_));
+    STCHK(r.img.alloc(3 * r.n2 * sizeof(float), "images"));
+    STCHK(launch_fill(r.img.as<float>(), 3 * r.n2, 0.f, st_));
+    if (cfg_.comp) STCHK(r.imgc.alloc(3 * r.n2 * sizeof(float), "image compensation"));
+    n2max_ = std::max(n2max_, r.n2);
+    r.nstrip = nstrips(p.nz, p.nx, cfg_.txr);
+  }
+  // dt*source(L-1) for level L (explosive_source, W/source-mod.f90:47-53)
+  std::vector<float> amp(cfg_.nt, 0.f);
+  for (int L = 2; L <= cfg_.nt; ++L) amp[L - 1] = cfg_.dt * wavelet[L - 2];
+  STCHK(src_amp_.alloc(cfg_.nt * sizeof(float), "source amplitudes"));
+  HIPCHK(hipMemcpy(src_amp_.p, amp.data(), cfg_.nt * sizeof(float), hipMemcpyHostToDevice));
+  if (cfg_.stack_nz > 0 && cfg_.stack_nx > 0) {
+    STCHK(stack_.alloc((size_t)3 * cfg_.stack_nz * cfg_.stack_nx * sizeof(double), "stacked image"));
+    HIPCHK(hipMemsetAsync(stack_.p, 0, stack_.bytes, st_));
+  }
+  HIPCHK(hipStreamSynchronize(st_));
+  STCHK(alloc_working());
+  return build_groups();
+}
+
+// Working set of one shot: the two-level ring (10 fields) followed by its
+// source-wavefield snapshots (and their per-strip scales).
+int Engine::alloc_working() {
+  const long long K = cfg_.nt / cfg_.snap_every;
+  size_t largest = 0, total = 0;
+  for (auto& r : res_) {
+    switch (cfg_.snap_fmt) {
+      case SNAP_HIST: r.snap_bytes = (size_t)2 * cfg_.nt * r.n2 * sizeof(float); r.scale_bytes = 0; break;
+      case SNAP_F32: r.snap_bytes = (size_t)4 * K * r.n2 * sizeof(float); r.scale_bytes = 0; break;
+      default:
+        r.snap_bytes = (size_t)4 * K * r.n2 * sizeof(unsigned short);
+        r.scale_bytes = (size_t)4 * K * r.nstrip * sizeof(float);
+    }
+    r.snap_bytes = (r.snap_bytes + 255) / 256 * 256;
+    r.scale_bytes = (r.scale_bytes + 255) / 256 * 256;
+    r.work_bytes = (size_t)10 * r.n2 * sizeof(float) + r.snap_bytes + r.scale_bytes;
+    largest = std::max(largest, r.work_bytes);
+    total += r.work_bytes;
+  }
+  size_t freeb = 0, totb = 0;
+  HIPCHK(hipMemGetInfo(&freeb, &totb));
+  size_t avail = (size_t)(cfg_.mem_fraction * (double)freeb);
+  if (largest > avail) {
+    set_error("one shot needs %zu bytes of working memory; %zu bytes usable", largest, avail);
+    return E_NOMEM;
+  }
+  arena_bytes_ = std::min(total, avail);
+  return work_.alloc(arena_bytes_, "group working set");
+}
+
+// Shots are packed in order into groups whose working sets fit the arena
+// (and at most max_group shots); one tile table spans all shots of a group.
+int Engine::build_groups() {
+  groups_.clear();
+  const int ns = (int)specs_.size();
+  std::vector<TileRef> tiles;
+  const int tx = tile_x(cfg_.txr);
+  int s = 0;
+  G_ = 0;
+  while (s < ns) {
+    Group g;
+    g.desc0 = s;
+    g.tile0 = (int)tiles.size();
+    size_t used = 0;
+    while (s < ns && used + res_[s].work_bytes <= arena_bytes_ &&
+           (cfg_.max_group <= 0 || (int)g.shots.size() < cfg_.max_group)) {
+      res_[s].work_off = used;
+      used += res_[s].work_bytes;
+      const ShotSpec& p = specs_[s];
+      const int slot = (int)g.shots.size();
+      const int nbz = (p.nz + TILE_Z - 1) / TILE_Z, nbx = (p.nx + tx - 1) / tx;
+      for (int bx = 0; bx < nbx; ++bx)
+        for (int bz = 0; bz < nbz; ++bz) tiles.push_back({slot, bz * TILE_Z, bx * tx, bz + nbz * bx});
+      g.shots.push_back(s);
+      ++s;
+    }
+    g.ntiles = (int)tiles.size() - g.tile0;
+    G_ = std::max(G_, (int)g.shots.size());
+    groups_.push_back(g);
+  }
+  // descriptor variants: 0 migration model, 1 true model, 2 Born; descriptor
+  // of shot s at [variant * nshots + s]
+  h_desc_.assign((size_t)3 * ns, ShotDev{});
+  STCHK(d_desc_.alloc(h_desc_.size() * sizeof(ShotDev), "shot descriptors"));
+  STCHK(d_tiles_.alloc(tiles.size() * sizeof(TileRef), "tile table"));
+  HIPCHK(hipMemcpy(d_tiles_.p, tiles.data(), tiles.size() * sizeof(TileRef), hipMemcpyHostToDevice));
+  for (int g = 0; g < (int)groups_.size(); ++g) STCHK(fill_desc(g, false, false));
+  return OK;
+}
+
+// Descriptors of group g (variant 0: migration coefficients; 1: true
+// coefficients; 2: Born) pointing at the shots' resident buffers and at the
+// group's working slots.
+int Engine::fill_desc(int g, bool true_model, bool born_mode) {
+  const int variant = born_mode ? 2 : (true_model ? 1 : 0);
+  const Group& gr = groups_[g];
+  const size_t ns = specs_.size();
+  for (size_t k = 0; k < gr.shots.size(); ++k) {
+    const int s = gr.shots[k];
+    const ShotSpec& p = specs_[s];
+    Res& r = res_[s];
+    ShotDev d{};
+    d.nz = p.nz; d.nx = p.nx; d.ldz = r.ldz; d.nt = cfg_.nt; d.n2 = r.n2;
+    d.lev = ring_of(s);
+    d.lev_stride = 5 * r.n2; d.fstride = r.n2; d.lev_mod = 2; d.lev_off = 2;
+    if (true_model && !r.coef_true.p) { set_error("no true model set"); return E_ARG; }
+    d.coef = true_model ? r.coef_true.as<float>() : r.coef.as<float>();
+    d.kz = r.kfac.p ? r.kfac.as<float>() : nullptr;
+    d.kx = r.kfac.p ? r.kfac.as<float>() + p.nz : nullptr;
+    d.zs = p.zs; d.xs = p.xs; d.src_amp = src_amp_.as<float>();
+    d.zr = p.zr; d.rx0 = p.rx0; d.nrx = p.nrx; d.rec = r.rec.as<float>();
+    d.snap_fmt = born_mode ? SNAP_HIST : cfg_.snap_fmt;
+    d.snap_every = born_mode ? 1 : cfg_.snap_every;
+    d.snap_sw = cfg_.txr == 1 ? 0 : cfg_.txr == 2 ? 1 : cfg_.txr == 4 ? 2 : 3;
+    d.snap_nbz = (p.nz + TILE_Z - 1) / TILE_Z;
+    d.snap_nstrip = r.nstrip;
+    d.snap = snap_of(s);
+    d.snap_scale = r.scale_bytes ? reinterpret_cast<float*>(snap_of(s) + r.snap_bytes) : nullptr;
+    d.img = r.img.as<float>();
+    d.imgc = r.imgc.p ? r.imgc.as<float>() : nullptr;
+    d.inv_dx = 1.0f / cfg_.dx;
+    d.inv_dt = 1.0f / cfg_.dt;
+    d.img_w = (float)cfg_.snap_every;
+    if (born_mode) {
+      d.bsrc = born_buf_.as<float>() + k * 8 * n2max_;
+      d.dm = born_buf_.as<float>() + k * 8 * n2max_ + 5 * n2max_;
+    }
+    h_desc_[variant * ns + s] = d;
+  }
+  const size_t off = variant * ns + gr.desc0;
+  HIPCHK(hipMemcpy(d_desc_.as<ShotDev>() + off, h_desc_.data() + off, gr.shots.size() * sizeof(ShotDev),
+                   hipMemcpyHostToDevice));
+  return OK;
+}
+
+// mode 0: imaging forward (migration model, snapshots); 1: record with the
+// true model (store); 2: record with the migration model (subtract);
+// 3: record with the migration model (store); 4: Born background (fp32
+// history of every level); 5: Born scattered field (record, store).
+int Engine::enqueue_forward(int g, int mode, hipStream_t st) {
+  const Group& gr = groups_[g];
+  const int variant = (mode == 1) ? 1 : (mode >= 4 ? 2 : 0);
+  const ShotDev* desc = d_desc_.as<ShotDev>() + (size_t)variant * specs_.size() + gr.desc0;
+  const TileRef* tiles = d_tiles_.as<TileRef>() + gr.tile0;
+  const int snap = (mode == 0) ? cfg_.snap_fmt : SNAP_HIST;
+  for (int s : gr.shots) {
+    STCHK(launch_fill(ring_of(s), 10 * res_[s].n2, 0.f, st));
+    if ((mode == 0 && cfg_.snap_fmt == SNAP_HIST) || mode == 4)
+      STCHK(launch_fill(reinterpret_cast<float*>(snap_of(s)), 2 * res_[s].n2, 0.f, st));  // level 1
+  }
+  const bool born = mode == 5;
+  const int first = born ? 1 : 2;
+  for (int L = first; L <= cfg_.nt; ++L) {
+    int fl = 0;
+    if (born || L >= 3) fl |= F_K;  // level 2 of the source run is the bare injection
+    if (!born) fl |= F_SRC;
+    if (born) {
+      fl |= F_BORN | F_REC;
+      STCHK(launch_born_sources(desc, tiles, gr.ntiles, L, st));
+    }
+    if (mode == 1 || mode == 3) fl |= F_REC;
+    if (mode == 2) fl |= F_RECSUB;
+    if (mode == 4) fl |= F_SNAP;
+    if (mode == 0) {
+      if (cfg_.snap_fmt == SNAP_HIST) fl |= F_SNAP;
+      else if ((L - 1) % cfg_.snap_every == 0) fl |= F_SNAP;
+    }
+    STCHK(launch_fwd(desc, tiles, gr.ntiles, L, fl, cfg_.txr, snap, cfg_.xcd, st));
+  }
+  if (mode == 0 && cfg_.snap_fmt != SNAP_HIST && cfg_.nt % cfg_.snap_every == 0)
+    STCHK(launch_snap_last(desc, tiles, gr.ntiles, cfg_.nt, cfg_.txr, cfg_.snap_fmt, st));
+  return OK;
+}
+
+int Engine::enqueue_adjoint(int g, hipStream_t st) {
+  const Group& gr = groups_[g];
+  const ShotDev* desc = d_desc_.as<ShotDev>() + gr.desc0;
+  const TileRef* tiles = d_tiles_.as<TileRef>() + gr.tile0;
+  for (size_t k = 0; k < gr.shots.size(); ++k) {
+    Res& r = res_[gr.shots[k]];
+    STCHK(launch_fill(ring_of(gr.shots[k]), 10 * r.n2, 0.f, st));
+    STCHK(launch_fill(r.img.as<float>(), 3 * r.n2, 0.f, st));
+    if (cfg_.comp) STCHK(launch_fill(r.imgc.as<float>(), 3 * r.n2, 0.f, st));
+  }
+  for (int L = cfg_.nt; L >= 1; --L) {
+    int fl = A_INJ;
+    if (cfg_.adj_abc && cfg_.abc > 0) fl |= A_K;
+    if (L % cfg_.snap_every == 0) fl |= A_IMG;
+    if (L < cfg_.nt) fl |= A_NEXT;
+    if (cfg_.comp) fl |= A_COMP;
+    STCHK(launch_adj(desc, tiles, gr.ntiles, L, fl, cfg_.txr, cfg_.snap_fmt, cfg_.xcd, st));
+  }
+  if (cfg_.comp)
+    for (size_t k = 0; k < gr.shots.size(); ++k) {
+      Res& r = res_[gr.shots[k]];
+      STCHK(launch_image_finalize(r.img.as<float>(), r.imgc.as<float>(), 3 * r.n2, st));
+    }
+  return OK;
+}
+
+// which: 0 imaging forward, 1 adjoint+imaging, 2 data synthesis (both runs)
+int Engine::run_phase(int g, int which, hipStream_t st) {
+  Group& gr = groups_[g];
+  hipGraphExec_t* slot = which == 0 ? &gr.gfwd : which == 1 ? &gr.gadj : &gr.gsyn;
+  auto enqueue = [&]() -> int {
+    if (which == 0) return enqueue_forward(g, 0, st);
+    if (which == 1) return enqueue_adjoint(g, st);
+    for (size_t k = 0; k < gr.shots.size(); ++k) {
+      Res& r = res_[gr.shots[k]];
+      STCHK(launch_fill(r.rec.as<float>(), (size_t)5 * cfg_.nt * std::max(specs_[gr.shots[k]].nrx, 1), 0.f, st));
+    }
+    STCHK(enqueue_forward(g, 1, st));
+    return enqueue_forward(g, 2, st);
+  };
+  if (!cfg_.use_graph) return enqueue();
+  if (!*slot) {
+    hipGraph_t graph = nullptr;
+    HIPCHK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+    const int rc = enqueue();
+    hipError_t e = hipStreamEndCapture(st, &graph);
+    if (rc != OK) {
+      if (graph) (void)hipGraphDestroy(graph);
+      return rc;
+    }
+    if (e != hipSuccess) {
+      set_error("graph capture failed: %s", hipGetErrorString(e));
+      return E_HIP;
+    }
+    e = hipGraphInstantiate(slot, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (e != hipSuccess) {
+      *slot = nullptr;
+      set_error("graph instantiation failed: %s", hipGetErrorString(e));
+      return E_HIP;
+    }
+  }
+  HIPCHK(hipGraphLaunch(*slot, st));
+  return OK;
+}
+
+static double cells_of(const ShotSpec& p) { return (double)p.nz * (double)p.nx; }
+
+int Engine::set_true_model(const std::vector<const float*>& coef5_true) {
+  if (coef5_true.size() != specs_.size()) { set_error("true-model list size mismatch"); return E_ARG; }
+  for (size_t s = 0; s < specs_.size(); ++s) {
+    Res& r = res_[s];
+    STCHK(r.coef_true.alloc(5 * r.n2 * sizeof(float), "true-model coefficients"));
+    STCHK(fill_masked_coefs(coef5_true[s], specs_[s].nz, specs_[s].nx, r.ldz, r.coef_true.as<float>()));
+  }
+  for (int g = 0; g < (int)groups_.size(); ++g) STCHK(fill_desc(g, true, false));
+  return OK;
+}
+
+int Engine::set_data(int shot, const float* h) {
+  if (shot < 0 || shot >= (int)specs_.size() || !h) { set_error("bad shot index or null data"); return E_ARG; }
+  const int nt = cfg_.nt, nr = specs_[shot].nrx;
+  std::vector<float> t((size_t)5 * nt * nr);
+  for (int f = 0; f < 5; ++f)
+    for (int ix = 0; ix < nr; ++ix)
+      for (int it = 0; it < nt; ++it) t[((size_t)f * nt + it) * nr + ix] = h[(size_t)f * nt * nr + it + (size_t)nt * ix];
+  if (!t.empty()) HIPCHK(hipMemcpy(res_[shot].rec.p, t.data(), t.size() * sizeof(float), hipMemcpyHostToDevice));
+  return OK;
+}
+
+int Engine::get_data(int shot, float* h) {
+  if (shot < 0 || shot >= (int)specs_.size() || !h) { set_error("bad shot index or null data"); return E_ARG; }
+  HIPCHK(hipStreamSynchronize(st_));
+  const int nt = cfg_.nt, nr = specs_[shot].nrx;
+  std::vector<float> t((size_t)5 * nt * nr);
+  if (!t.empty()) HIPCHK(hipMemcpy(t.data(), res_[shot].rec.p, t.size() * sizeof(float), hipMemcpyDeviceToHost));
+  for (int f = 0; f < 5; ++f)
+    for (int ix = 0; ix < nr; ++ix)
+      for (int it = 0; it < nt; ++it) h[(size_t)f * nt * nr + it + (size_t)nt * ix] = t[((size_t)f * nt + it) * nr + ix];
+  return OK;
+}
+
+int Engine::synthesize() {
+  for (auto& r : res_)
+    if (!r.coef_true.p) { set_error("synthesize: set the true model first"); return E_ARG; }
+  for (int g = 0; g < (int)groups_.size(); ++g) STCHK(run_phase(g, 2, st_));
+  HIPCHK(hipStreamSynchronize(st_));
+  return OK;
+}
+
+int Engine::model_data() {
+  tm_ = Timing{};
+  HIPCHK(hipEventRecord(ev_[0], st_));
+  for (int g = 0; g < (int)groups_.size(); ++g) {
+    for (int s : groups_[g].shots)
+      STCHK(launch_fill(res_[s].rec.as<float>(), (size_t)5 * cfg_.nt * std::max(specs_[s].nrx, 1), 0.f, st_));
+    STCHK(enqueue_forward(g, 3, st_));
+    for (int s : groups_[g].shots) tm_.fwd_cells += cells_of(specs_[s]) * (cfg_.nt - 1);
+    tm_.fwd_launches += cfg_.nt - 1;
+    last_group_ = g;
+  }
+  HIPCHK(hipEventRecord(ev_[1], st_));
+  HIPCHK(hipStreamSynchronize(st_));
+  float ms = 0;
+  HIPCHK(hipEventElapsedTime(&ms, ev_[0], ev_[1]));
+  tm_.fwd_ms = tm_.total_ms = ms;
+  return OK;
+}
+
+int Engine::migrate() {
+  tm_ = Timing{};
+  if (stack_.p) HIPCHK(hipMemsetAsync(stack_.p, 0, stack_.bytes, st_));
+  for (int g = 0; g < (int)groups_.size(); ++g) {
+    HIPCHK(hipEventRecord(ev_[0], st_));
+    STCHK(run_phase(g, 0, st_));
+    HIPCHK(hipEventRecord(ev_[1], st_));
+    STCHK(run_phase(g, 1, st_));
+    HIPCHK(hipEventRecord(ev_[2], st_));
+    for (int s : groups_[g].shots) {
+      const ShotSpec& p = specs_[s];
+      if (stack_.p && p.img_nz > 0 && p.img_nx > 0)
+        STCHK(launch_stack(stack_.as<double>(), cfg_.stack_nz, cfg_.stack_nx, res_[s].img.as<float>(), res_[s].ldz,
+                           res_[s].n2, p.img_z0, p.img_x0, p.img_nz, p.img_nx, p.stack_z0, p.stack_x0, st_));
+    }
+    HIPCHK(hipEventRecord(ev_[3], st_));
+    HIPCHK(hipStreamSynchronize(st_));
+    float a = 0, b = 0, c = 0;
+    HIPCHK(hipEventElapsedTime(&a, ev_[0], ev_[1]));
+    HIPCHK(hipEventElapsedTime(&b, ev_[1], ev_[2]));
+    HIPCHK(hipEventElapsedTime(&c, ev_[0], ev_[3]));
+    tm_.fwd_ms += a;
+    tm_.adj_ms += b;
+    tm_.total_ms += c;
+    tm_.fwd_launches += cfg_.nt - 1;
+    tm_.adj_launches += cfg_.nt;
+    for (int s : groups_[g].shots) {
+      tm_.fwd_cells += cells_of(specs_[s]) * (cfg_.nt - 1);
+      tm_.adj_cells += cells_of(specs_[s]) * cfg_.nt;
+    }
+    last_group_ = g;
+  }
+  return OK;
+}
+
+int Engine::born(const std::vector<const float*>& dm) {
+  if (groups_.size() != 1) { set_error("Born modelling runs single-group engines only"); return E_ARG; }
+  if (dm.size() != specs_.size()) { set_error("dm list size mismatch"); return E_ARG; }
+  if (cfg_.snap_fmt != SNAP_HIST) { set_error("Born modelling needs the fp32 history format"); return E_ARG; }
+  STCHK(born_buf_.alloc((size_t)G_ * 8 * n2max_ * sizeof(float), "Born sources"));
+  STCHK(launch_fill(born_buf_.as<float>(), (size_t)G_ * 8 * n2max_, 0.f, st_));
+  HIPCHK(hipStreamSynchronize(st_));
+  for (size_t k = 0; k < specs_.size(); ++k) {
+    const ShotSpec& p = specs_[k];
+    float* d = born_buf_.as<float>() + k * 8 * n2max_ + 5 * n2max_;
+    for (int c = 0; c < 3; ++c)  // component stride = the shot's own n2 (as the kernels index it)
+      HIPCHK(hipMemcpy2D(d + c * res_[k].n2, res_[k].ldz * sizeof(float), dm[k] + (size_t)c * p.nz * p.nx,
+                         p.nz * sizeof(float), p.nz * sizeof(float), p.nx, hipMemcpyHostToDevice));
+    STCHK(launch_fill(res_[k].rec.as<float>(), (size_t)5 * cfg_.nt * std::max(p.nrx, 1), 0.f, st_));
+  }
+  STCHK(fill_desc(0, false, true));
+  tm_ = Timing{};
+  STCHK(enqueue_forward(0, 4, st_));
+  HIPCHK(hipEventRecord(ev_[0], st_));
+  STCHK(enqueue_forward(0, 5, st_));
+  HIPCHK(hipEventRecord(ev_[1], st_));
+  HIPCHK(hipStreamSynchronize(st_));
+  float ms = 0;
+  HIPCHK(hipEventElapsedTime(&ms, ev_[0], ev_[1]));
+  tm_.fwd_ms = tm_.total_ms = ms;
+  tm_.fwd_launches = cfg_.nt;
+  last_group_ = 0;
+  return OK;
+}
+
+int Engine::get_image(int shot, float* img3) {
+  if (shot < 0 || shot >= (int)specs_.size() || !img3) { set_error("bad shot index or null output"); return E_ARG; }
+  HIPCHK(hipStreamSynchronize(st_));
+  const ShotSpec& p = specs_[shot];
+  const Res& r = res_[shot];
+  for (int c = 0; c < 3; ++c)
+    HIPCHK(hipMemcpy2D(img3 + (size_t)c * p.nz * p.nx, p.nz * sizeof(float), r.img.as<float>() + c * r.n2,
+                       r.ldz * sizeof(float), p.nz * sizeof(float), p.nx, hipMemcpyDeviceToHost));
+  return OK;
+}
+
+int Engine::get_stack(float* img3) {
+  if (!stack_.p || !img3) { set_error("no stacked image configured"); return E_ARG; }
+  HIPCHK(hipStreamSynchronize(st_));
+  const size_t n = (size_t)3 * cfg_.stack_nz * cfg_.stack_nx;
+  std::vector<double> h(n);
+  HIPCHK(hipMemcpy(h.data(), stack_.p, n * sizeof(double), hipMemcpyDeviceToHost));
+  for (size_t i = 0; i < n; ++i) img3[i] = (float)h[i];
+  return OK;
+}
+
+int Engine::get_last_levels(int shot, float* last10) {
+  if (last_group_ < 0) { set_error("nothing has run"); return E_ARG; }
+  const Group& gr = groups_[last_group_];
+  auto it = std::find(gr.shots.begin(), gr.shots.end(), shot);
+  if (it == gr.shots.end() || !last10) { set_error("shot %d is not in the last group run", shot); return E_ARG; }
+  HIPCHK(hipStreamSynchronize(st_));
+  const float* base = ring_of(shot);
+  const ShotSpec& p = specs_[shot];
+  const Res& r = res_[shot];
+  const size_t nn = (size_t)p.nz * p.nx;
+  const int nt = cfg_.nt;
+  for (int lv = 0; lv < 2; ++lv) {
+    const int L = nt - 1 + lv;  // levels nt-1, nt (slot L % 2)
+    for (int f = 0; f < 5; ++f)
+      HIPCHK(hipMemcpy2D(last10 + (size_t)(5 * lv + f) * nn, p.nz * sizeof(float),
+                         base + (size_t)(L % 2) * 5 * r.n2 + f * r.n2, r.ldz * sizeof(float), p.nz * sizeof(float),
+                         p.nx, hipMemcpyDeviceToHost));
+  }
+  return OK;
+}
+
+}  // namespace erm

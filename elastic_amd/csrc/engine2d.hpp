@@ -1,0 +1,115 @@
+// engine2d.hpp -- multi-shot 2D elastic RTM engine (host side).
+//
+// All inputs (coefficients, observed data, the source wavelet) are made
+// resident in HBM at setup. Shots are processed in groups sized to the free
+// device memory; a group's shots (uneven grids allowed) share one tile table,
+// so each time level is ONE kernel launch over all of them. The forward and
+// adjoint loops of a group can be captured once in HIP graphs and replayed.
+#pragma once
+
+#include <vector>
+
+#include "elastic2d.hpp"
+
+namespace erm {
+
+struct ShotSpec {
+  int nz, nx;                     // computational grid
+  int xs, zs, zr, rx0, nrx;       // source cell, receiver row and columns
+  int img_z0, img_x0, img_nz, img_nx;  // image region added into the stack
+  int stack_z0, stack_x0;         // its position in the stacked image
+};
+
+struct EngineCfg {
+  int nt = 0;
+  float dx = 0, dt = 0;
+  int abc = 0;         // sponge width (<=0: none)
+  int adj_abc = 0;     // 0: adjoint without sponge, 1: exact adjoint sponge
+  int snap_every = 1;  // imaging decimation
+  int snap_fmt = SNAP_HIST;
+  int comp = 0;        // compensated image sums
+  int txr = 4;         // tile width (x cells per thread)
+  int xcd = 1;         // XCD-aware tile numbering
+  int use_graph = 0;   // capture the time loops in HIP graphs
+  int max_group = 0;   // cap on shots per group (0 = memory bound only)
+  int stack_nz = 0, stack_nx = 0;  // stacked image (0 = none)
+  double mem_fraction = 0.85;      // share of free memory the working set may use
+};
+
+struct Timing {
+  double fwd_ms = 0, adj_ms = 0, total_ms = 0;
+  long long fwd_launches = 0, adj_launches = 0;
+  double fwd_cells = 0, adj_cells = 0;  // cell-updates (computational grid)
+};
+
+class Engine {
+ public:
+  ~Engine();
+  int setup(const EngineCfg& cfg, const std::vector<ShotSpec>& shots,
+            const std::vector<const float*>& coef5, const float* wavelet);
+  // Second coefficient set per shot (the "true" model) for data synthesis.
+  int set_true_model(const std::vector<const float*>& coef5_true);
+  int set_data(int shot, const float* rec_host);  // 5*(nt,nrx)
+  int get_data(int shot, float* rec_host);
+  // observed = forward(true) - forward(migration) for every shot
+  int synthesize();
+  // forward + adjoint + imaging of every shot, images stacked
+  int migrate();
+  // forward of every shot with the migration model, recording receivers
+  int model_data();
+  // Born modelling of one-group runs (single shot API); dm 3*(nz,nx) per shot
+  int born(const std::vector<const float*>& dm);
+  int get_image(int shot, float* img3);          // 3*(nz,nx)
+  int get_stack(float* img3);                    // 3*(stack_nz,stack_nx)
+  int get_last_levels(int shot, float* last10);  // levels nt-1, nt of the last run
+  const Timing& timing() const { return tm_; }
+  int group_size() const { return G_; }
+  int ngroups() const { return (int)groups_.size(); }
+
+ private:
+  struct Res {
+    int ldz = 0;
+    long long n2 = 0;
+    int nstrip = 0;
+    size_t snap_bytes = 0, scale_bytes = 0, work_bytes = 0;  // working set of one shot
+    size_t work_off = 0;  // offset of its working set in the arena (within its group)
+    DevBuf coef, coef_true, kfac, rec, img, imgc;
+  };
+  float* ring_of(int s) const { return reinterpret_cast<float*>(work_.as<char>() + res_[s].work_off); }
+  char* snap_of(int s) const { return work_.as<char>() + res_[s].work_off + (size_t)10 * res_[s].n2 * sizeof(float); }
+  struct Group {
+    std::vector<int> shots;
+    int tile0 = 0, ntiles = 0;  // slice of the tile table
+    int desc0 = 0;              // first descriptor
+    hipGraphExec_t gfwd = nullptr, gadj = nullptr, gsyn = nullptr;
+  };
+  int alloc_working();
+  int build_groups();
+  int fill_desc(int g, bool true_model, bool born_mode);
+  int enqueue_forward(int g, int mode, hipStream_t st);
+  int enqueue_adjoint(int g, hipStream_t st);
+  int run_phase(int g, int which, hipStream_t st);
+  void destroy_graphs();
+
+  EngineCfg cfg_;
+  std::vector<ShotSpec> specs_;
+  std::vector<Res> res_;
+  std::vector<Group> groups_;
+  int G_ = 0;
+  long long n2max_ = 0;
+  size_t arena_bytes_ = 0;
+  DevBuf src_amp_, work_, d_desc_, d_tiles_, stack_, born_buf_;
+  std::vector<ShotDev> h_desc_;
+  hipStream_t st_ = nullptr;
+  hipEvent_t ev_[4] = {};
+  int last_group_ = -1;
+  Timing tm_;
+};
+
+int fill_masked_coefs(const float* coef5, int nz, int nx, int ldz, float* dev);
+void sponge_factors(int n, int abc, std::vector<float>& k);
+int ensure_device();
+int default_txr();
+int default_xcd();
+
+}  // namespace erm

@@ -216,3 +216,108 @@ class RtmOperator:
             self.close()
         except Exception:
             pass
+
+
+# --------------------------------------------------------------------------- batch
+class RtmBatchShot(C.Structure):
+    """Mirror of rtm_batch_shot_t."""
+
+    _fields_ = [(n, C.c_int) for n in ("nz", "nx", "xs", "zs", "zr", "rx0", "nrx", "img_z0", "img_x0", "img_nz",
+                                       "img_nx", "stack_z0", "stack_x0")]
+
+
+class RtmBatchCfg(C.Structure):
+    """Mirror of rtm_batch_cfg_t."""
+
+    _fields_ = [
+        ("nt", C.c_int), ("dx", C.c_float), ("dt", C.c_float), ("abc", C.c_int), ("adj_abc", C.c_int),
+        ("snap_every", C.c_int), ("snap_prec", C.c_int), ("comp_image", C.c_int), ("txr", C.c_int),
+        ("use_graph", C.c_int), ("max_group", C.c_int), ("stack_nz", C.c_int), ("stack_nx", C.c_int),
+        ("mem_fraction", C.c_double),
+    ]
+
+
+SNAP_PREC = {"fp32": 0, "bf16": 1, "fp16": 2}
+
+
+class Batch:
+    """Multi-shot RTM on device-resident inputs (rtm_batch_* C API)."""
+
+    def __init__(self, nt, dx, dt, shots, coefs, wavelet, abc=0, adj_abc=0, snap_every=1, snap_prec="fp32",
+                 comp_image=0, txr=0, use_graph=0, max_group=0, stack_shape=(0, 0), mem_fraction=0.0):
+        lib = load()
+        self.nt = nt
+        self.shots = [dict(s) for s in shots]
+        n = len(shots)
+        arr = (RtmBatchShot * n)()
+        for i, s in enumerate(shots):
+            for k, _ in RtmBatchShot._fields_:
+                setattr(arr[i], k, int(s.get(k, 0)))
+        cfg = RtmBatchCfg(nt, dx, dt, abc, adj_abc, snap_every, SNAP_PREC[snap_prec] if isinstance(snap_prec, str) else snap_prec,
+                          comp_image, txr, use_graph, max_group, stack_shape[0], stack_shape[1], mem_fraction)
+        self._keep = [pack_fields(c) for c in coefs]
+        ptrs = (C.c_void_p * n)(*[k.ctypes.data for k in self._keep])
+        w = np.ascontiguousarray(wavelet, np.float32)
+        self._h = C.c_void_p()
+        _chk(lib.rtm_batch_create(C.byref(self._h), C.byref(cfg), C.c_int(n), arr, ptrs, _p(w)))
+        self._keep = None  # the library copied the coefficients
+
+    def set_true_model(self, coefs):
+        keep = [pack_fields(c) for c in coefs]
+        ptrs = (C.c_void_p * len(keep))(*[k.ctypes.data for k in keep])
+        _chk(load().rtm_batch_set_true_model(self._h, ptrs))
+
+    def synthesize(self):
+        _chk(load().rtm_batch_synthesize(self._h))
+
+    def set_data(self, i, rec):
+        r = np.ascontiguousarray(rec, np.float32)
+        _chk(load().rtm_batch_set_data(self._h, C.c_int(i), _p(r)))
+
+    def get_data(self, i):
+        s = self.shots[i]
+        rec = np.zeros((5, s["nrx"], self.nt), np.float32)
+        _chk(load().rtm_batch_get_data(self._h, C.c_int(i), _p(rec)))
+        return rec
+
+    def migrate(self):
+        _chk(load().rtm_batch_migrate(self._h))
+
+    def image(self, i):
+        s = self.shots[i]
+        img = np.zeros((3, s["nx"], s["nz"]), np.float32)
+        _chk(load().rtm_batch_get_image(self._h, C.c_int(i), _p(img)))
+        return tuple(unpack_fields(img))
+
+    def stack(self, shape):
+        img = np.zeros((3, shape[1], shape[0]), np.float32)
+        _chk(load().rtm_batch_get_stack(self._h, _p(img)))
+        return tuple(unpack_fields(img))
+
+    def info(self):
+        out = np.zeros(8, np.float64)
+        _chk(load().rtm_batch_info(self._h, _p(out)))
+        keys = ["fwd_ms", "fwd_launches", "adj_ms", "adj_launches", "total_ms", "fwd_cells", "adj_cells", "group_size"]
+        return dict(zip(keys, out.tolist()))
+
+    def close(self):
+        if self._h:
+            load().rtm_batch_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def bandwidth_probe(nbytes=1 << 30, reps=20):
+    """Achieved read+write GB/s of a streaming device copy and ms per copy."""
+    g, ms = C.c_double(), C.c_double()
+    _chk(load().rtm_bandwidth_probe(C.c_size_t(nbytes), C.c_int(reps), C.byref(g), C.byref(ms)))
+    return g.value, ms.value
+
+
+def synchronize():
+    _chk(load().rtm_hip_finalize())

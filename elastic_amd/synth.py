@@ -134,3 +134,30 @@ def reference_dt_nt(dx, ttime):
     f = np.float32
     dt = f(0.4) * f(dx) / (np.sqrt(f(2.0)) * f(2000.0))
     return float(dt), int(f(ttime) / dt)
+
+
+def window_shot(vp_true, vp_mig, x0, w, xs_model, abc, dx, dt, zsrc_m=20.0, zrec_m=20.0):
+    """One shot of a batched survey: the model columns [x0, x0+w) padded by
+    abc+5 cells. Returns (spec dict for the batch API, true coef5, migration
+    coef5) with coefficients from the library's model_op."""
+    from . import lib
+
+    pad = abc + 5
+    nzp = vp_true.shape[0]
+    coefs = []
+    for vp in (vp_true, vp_mig):
+        m = elastic_from_vp(pad_edge(vp[:, x0:x0 + w], pad))
+        coefs.append(lib.model_op(m.rho, m.lam, m.mu, dx, dt))
+    spec = dict(nz=nzp + 2 * pad, nx=w + 2 * pad, xs=pad + (xs_model - x0), zs=pad + int(round(zsrc_m / dx)),
+                zr=pad + int(round(zrec_m / dx)), rx0=pad, nrx=w, img_z0=pad, img_x0=pad, img_nz=nzp, img_nx=w,
+                stack_z0=0, stack_x0=x0)
+    return spec, coefs[0], coefs[1]
+
+
+def survey(nz, nx, nshots, seed, abc, dx, dt, wmin=600, wmax=1200, sigma=10.0):
+    """Seeded batched survey (config-1 construction): layered model, smoothed
+    migration model, shot windows. Returns (vp_true, vp_mig, list of
+    (xs, x0, w))."""
+    vp = layered_vp(nz, nx, seed)
+    vps = smooth(vp, sigma)
+    return vp, vps, batch_apertures(nshots, nx, seed + 1, min(wmin, nx), min(wmax, nx))

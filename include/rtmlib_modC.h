@@ -148,6 +148,63 @@ int abc_k_hip(int nz, int nx, int abc, float *field);
  * 10th-order staggered limit 1/(sqrt(2)*sum|c_k|), else RTM_E_CFL. */
 int rtm_cfl_check(float vmax, float dt, float dx, double *courant, double *limit);
 
+/* ------------------------------------------------------------------------
+ * Batched multi-shot RTM (no counterpart in the reference, which runs one
+ * shot per process). Every shot has its own computational grid (uneven
+ * aperture windows allowed); all inputs are made resident in HBM by
+ * rtm_batch_create, shots are processed in groups sized to the free device
+ * memory, and each time level of a group is one kernel launch over all of its
+ * shots (optionally replayed from HIP graphs). Images of each shot are kept
+ * and the region [img_z0, img_z0+img_nz) x [img_x0, img_x0+img_nx) of each is
+ * added (in FP64) into a stacked image at (stack_z0, stack_x0).
+ * ---------------------------------------------------------------------- */
+typedef struct rtm_batch rtm_batch;
+
+typedef struct {
+  int nz, nx;                 /* computational grid of this shot */
+  int xs, zs, zr, rx0, nrx;   /* source cell, receiver row and columns */
+  int img_z0, img_x0, img_nz, img_nx;  /* region stacked into the image */
+  int stack_z0, stack_x0;
+} rtm_batch_shot_t;
+
+typedef struct {
+  int nt;
+  float dx, dt;
+  int abc, adj_abc;
+  int snap_every;   /* imaging decimation j */
+  int snap_prec;    /* 0 fp32, 1 bf16, 2 fp16 */
+  int comp_image;   /* compensated image sums */
+  int txr;          /* tile width 1/2/4/8 (0 = default) */
+  int use_graph;    /* capture time loops in HIP graphs */
+  int max_group;    /* cap on shots per group (0 = memory bound) */
+  int stack_nz, stack_nx;  /* stacked image size (0 = none) */
+  double mem_fraction;     /* share of free device memory for working sets (0 = 0.85) */
+} rtm_batch_cfg_t;
+
+int rtm_batch_create(rtm_batch **b, const rtm_batch_cfg_t *cfg, int nshots,
+                     const rtm_batch_shot_t *shots, const float *const *coef5,
+                     const float *wavelet);
+/* Second coefficient set per shot, used by rtm_batch_synthesize. */
+int rtm_batch_set_true_model(rtm_batch *b, const float *const *coef5_true);
+/* Observed data of every shot on the GPU: forward(true) - forward(migration). */
+int rtm_batch_synthesize(rtm_batch *b);
+int rtm_batch_set_data(rtm_batch *b, int shot, const float *rec);  /* 5*(nt,nrx) */
+int rtm_batch_get_data(rtm_batch *b, int shot, float *rec);
+/* RTM of every shot (device-resident inputs; the timed work). */
+int rtm_batch_migrate(rtm_batch *b);
+int rtm_batch_get_image(rtm_batch *b, int shot, float *img3);  /* 3*(nz,nx) */
+int rtm_batch_get_stack(rtm_batch *b, float *img3);            /* 3*(stack_nz,stack_nx) */
+/* [0] fwd ms [1] fwd launches [2] adj ms [3] adj launches [4] total ms
+ * [5] fwd cell-updates [6] adj cell-updates [7] shots per group */
+int rtm_batch_info(rtm_batch *b, double *out8);
+int rtm_batch_destroy(rtm_batch *b);
+
+/* Streaming device-to-device copy of nbytes, timed over reps launches with
+ * HIP events: achieved read+write GB/s and ms per copy. Used to measure the
+ * attainable HBM bandwidth and to calibrate the FETCH_SIZE/WRITE_SIZE
+ * counters against a kernel of known traffic. */
+int rtm_bandwidth_probe(size_t nbytes, int reps, double *gbps, double *ms_per_copy);
+
 /* Library lifecycle: optional explicit device selection and release of the
  * cached device state of the calling process. */
 int rtm_hip_init(int device);
