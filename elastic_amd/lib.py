@@ -321,3 +321,63 @@ def bandwidth_probe(nbytes=1 << 30, reps=20):
 
 def synchronize():
     _chk(load().rtm_hip_finalize())
+
+
+# --------------------------------------------------------------------------- 3D acoustic
+class Ac3dCfg(C.Structure):
+    _fields_ = [(n, C.c_int) for n in ("n1", "n2", "n3", "nt", "sz", "sx", "sy", "zr", "abc", "snap_every",
+                                       "snap_prec")]
+
+
+class Acoustic3D:
+    """3D acoustic subdomain propagator + imaging (include/acoustic3d.h).
+    Arrays: c (n1, n2, n3) Fortran order (z fastest); data (nt, n3, n2)."""
+
+    def __init__(self, c, amp, src, zr, abc=40, snap_every=5, snap_prec="bf16"):
+        lib = load()
+        c = np.asfortranarray(c, np.float32)
+        self.shape = c.shape
+        self.nt = len(amp)
+        cfg = Ac3dCfg(*c.shape, self.nt, *src, zr, abc, snap_every, 1 if snap_prec == "bf16" else 0)
+        a = np.ascontiguousarray(amp, np.float32)
+        self._h = C.c_void_p()
+        _chk(lib.ac3d_create(C.byref(self._h), C.byref(cfg), _p(c), _p(a)))
+
+    def model_data(self, c_true=None):
+        ct = None if c_true is None else np.asfortranarray(c_true, np.float32)
+        _chk(load().ac3d_model_data(self._h, _p(ct)))
+
+    def migrate(self):
+        _chk(load().ac3d_migrate(self._h))
+
+    def data(self):
+        n1, n2, n3 = self.shape
+        out = np.zeros((self.nt, n3, n2), np.float32)
+        _chk(load().ac3d_get_data(self._h, _p(out)))
+        return out
+
+    def set_data(self, rec):
+        r = np.ascontiguousarray(rec, np.float32)
+        _chk(load().ac3d_set_data(self._h, _p(r)))
+
+    def image(self):
+        n1, n2, n3 = self.shape
+        out = np.zeros((n3, n2, n1), np.float32)
+        _chk(load().ac3d_get_image(self._h, _p(out)))
+        return np.asfortranarray(out.transpose(2, 1, 0))
+
+    def timings(self):
+        out = np.zeros(6, np.float64)
+        _chk(load().ac3d_timings(self._h, _p(out)))
+        return out
+
+    def close(self):
+        if self._h:
+            load().ac3d_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -242,3 +242,59 @@ def rel_l2(a, b):
     b = np.asarray(b, np.float64)
     den = np.linalg.norm(b.ravel())
     return float(np.linalg.norm((a - b).ravel()) / (den if den > 0 else 1.0))
+
+
+# --------------------------------------------------------------------------- 3D acoustic
+def ac3_weights():
+    w = np.zeros(6)
+    lib().ac3_weights_f64(_p(w))
+    return w
+
+
+def ac3_laplacian(p):
+    p = np.asfortranarray(p, np.float64)
+    out = np.zeros_like(p, order="F")
+    n1, n2, n3 = p.shape
+    lib().ac3_laplacian_f64(C.c_int(n1), C.c_int(n2), C.c_int(n3), _p(p), _p(out))
+    return out
+
+
+def ac3_taper(n, abc):
+    """Per-axis sponge factors (W/abc-mod.f90 taper, 1 inside)."""
+    k = np.ones(n)
+    if abc <= 0:
+        return k
+    f = abc_filter(abc)
+    abc2 = abc + 5
+    for i1 in range(1, abc2 + 1):
+        k[i1 - 1] *= f[abc2 - i1]
+        k[n - abc2 + i1 - 1] *= f[i1 - 1]
+    return k
+
+
+def ac3_run(c, amp, src, zr, abc=0, mode=0, rec=None, j=5, snap=None, want_snap=False, want_img=False,
+            dtype=np.float64, nthreads=1):
+    """3D acoustic pass (acoustic3d_oracle.cpp). mode 0: source pass returning
+    (rec (nt, n3, n2), snapshots (nsnap, n1, n2, n3) F-ordered per snapshot);
+    mode 1: receiver pass with injected rec and imaging against snap."""
+    c = np.asfortranarray(c, dtype)
+    n1, n2, n3 = c.shape
+    nt = len(amp)
+    a = np.ascontiguousarray(amp, dtype)
+    kz, kx, ky = (np.ascontiguousarray(ac3_taper(n, abc), dtype) for n in (n1, n2, n3))
+    nsnap = nt // j
+    if mode == 0:
+        r = np.zeros((nt, n3, n2), dtype)
+        s = np.zeros((max(nsnap, 1), n3, n2, n1), dtype) if want_snap else None
+        img = None
+    else:
+        r = np.ascontiguousarray(rec, dtype)
+        s = np.ascontiguousarray(snap, dtype)
+        img = np.zeros((n3, n2, n1), dtype)
+    getattr(lib(), "ac3_run_" + _suffix(dtype))(
+        C.c_int(n1), C.c_int(n2), C.c_int(n3), C.c_int(nt), _p(c), _p(kz) if abc > 0 else None,
+        _p(kx) if abc > 0 else None, _p(ky) if abc > 0 else None, C.c_int(mode), C.c_int(src[0]), C.c_int(src[1]),
+        C.c_int(src[2]), _p(a), C.c_int(zr), _p(r), C.c_int(j), _p(s), _p(img), C.c_int(nthreads))
+    if mode == 0:
+        return r, s
+    return img
