@@ -162,7 +162,7 @@ __global__ __launch_bounds__(256) void k_ac3_step(const Ac3Args a) {
 // Snapshot of one level: bf16 with the level's max-abs scale, or fp32 copy.
 __global__ void k_ac3_snap(const float* p, void* out, size_t n, const unsigned* smax, int bf16) {
   const float m = __uint_as_float(*smax);
-  const float inv = m > 0.f ? 1.f / m : 0.f;
+  const float inv = m > 1e-30f ? 1.f / m : 0.f;  // subnormal maxima would overflow 1/m
   for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
        i += (size_t)gridDim.x * blockDim.x) {
     if (bf16) {

@@ -92,6 +92,10 @@ __device__ __forceinline__ unsigned short q_f16(float x) {
 }
 __device__ __forceinline__ float dq_f16(unsigned short b) { return (float)__builtin_bit_cast(_Float16, b); }
 
+// Strips whose max-abs is below this are stored as zero: Q / m would
+// overflow for subnormal maxima (the precursor ahead of the wavefront).
+constexpr float kSnapFloor = 1e-30f;
+
 template <int SNAP>
 struct SnapQ {
   static constexpr float Q = SNAP == SNAP_F16 ? 32768.f : 1.f;
@@ -249,7 +253,7 @@ __global__ __launch_bounds__(256) void k_fwd(const ShotDev* __restrict__ shots,
 #pragma unroll
         for (int j = 0; j < TXR; ++j) m = fmaxf(m, fabsf(sv[a][j]));
         m = wave_max(m);
-        const float inv = m > 0.f ? SnapQ<SNAP>::Q / m : 0.f;
+        const float inv = m > kSnapFloor ? SnapQ<SNAP>::Q / m : 0.f;
         if (tz == 0) S.snap_scale[(k0 * 4 + a) * S.snap_nstrip + strip] = m / SnapQ<SNAP>::Q;
 #pragma unroll
         for (int j = 0; j < TXR; ++j) {
@@ -307,7 +311,7 @@ __global__ __launch_bounds__(256) void k_snap_last(const ShotDev* __restrict__ s
 #pragma unroll
       for (int j = 0; j < TXR; ++j) m = fmaxf(m, fabsf(sv[a][j]));
       m = wave_max(m);
-      const float inv = m > 0.f ? SnapQ<SNAP>::Q / m : 0.f;
+      const float inv = m > kSnapFloor ? SnapQ<SNAP>::Q / m : 0.f;
       if (tz == 0) {
         S.snap_scale[(k0 * 4 + a) * S.snap_nstrip + strip] = m / SnapQ<SNAP>::Q;
         S.snap_scale[(k0 * 4 + a + 2) * S.snap_nstrip + strip] = 0.f;
