@@ -65,7 +65,8 @@ enum : int {
   F_K = 4,       // apply the sponge
   F_SRC = 8,     // inject the explosive source
   F_BORN = 16,   // add the Born sources bsrc
-  F_SNAP = 32    // write the source-wavefield snapshot belonging to this step
+  F_SNAP = 32,   // write the source-wavefield snapshot belonging to this step
+  F_NTS = 64     // non-temporal stores of the new level
 };
 // Adjoint flags
 enum : int {
@@ -74,7 +75,8 @@ enum : int {
   A_K = 4,     // exact-adjoint sponge
   A_IMG = 8,   // accumulate the imaging condition at this level
   A_NEXT = 16, // level L+1 exists (S1, S2 nonzero)
-  A_COMP = 32  // compensated image accumulation
+  A_COMP = 32, // compensated image accumulation
+  A_NTS = 64   // non-temporal stores of the new level
 };
 
 // Tile geometry: 64 z-cells (one wave64 along the fast axis) by 4*TXR

@@ -208,11 +208,58 @@ __global__ void k_nonfinite(const float* p, size_t n, int* flag) {
 
 // Streaming copy used to measure the achievable HBM bandwidth and to
 // calibrate the FETCH_SIZE / WRITE_SIZE counters (known bytes per dispatch).
-__global__ __launch_bounds__(256) void k_copy_probe(const float4* __restrict__ src,
-                                                    float4* __restrict__ dst, size_t n4) {
-  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n4;
-       i += (size_t)gridDim.x * blockDim.x)
-    dst[i] = src[i];
+typedef float v4f __attribute__((ext_vector_type(4)));
+
+// mode 0: copy; 1: copy with non-temporal loads/stores; 2: read-only
+// (reduction into one value per thread); 3: write-only (fill).
+__global__ __launch_bounds__(256) void k_copy_probe(const v4f* __restrict__ src,
+                                                    v4f* __restrict__ dst, size_t n4, int mode) {
+  // four independent 16-byte accesses per thread in flight
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  v4f acc = {0.f, 0.f, 0.f, 0.f};
+  for (; i + 3 * stride < n4; i += 4 * stride) {
+    if (mode == 3) {
+      const v4f z = {1.f, 2.f, 3.f, 4.f};
+      dst[i] = z;
+      dst[i + stride] = z;
+      dst[i + 2 * stride] = z;
+      dst[i + 3 * stride] = z;
+      continue;
+    }
+    v4f a, b, c, d;
+    if (mode == 1) {
+      a = __builtin_nontemporal_load(src + i);
+      b = __builtin_nontemporal_load(src + i + stride);
+      c = __builtin_nontemporal_load(src + i + 2 * stride);
+      d = __builtin_nontemporal_load(src + i + 3 * stride);
+    } else {
+      a = src[i];
+      b = src[i + stride];
+      c = src[i + 2 * stride];
+      d = src[i + 3 * stride];
+    }
+    if (mode == 2) {
+      acc += (a + b) + (c + d);
+    } else if (mode == 1) {
+      __builtin_nontemporal_store(a, dst + i);
+      __builtin_nontemporal_store(b, dst + i + stride);
+      __builtin_nontemporal_store(c, dst + i + 2 * stride);
+      __builtin_nontemporal_store(d, dst + i + 3 * stride);
+    } else {
+      dst[i] = a;
+      dst[i + stride] = b;
+      dst[i + 2 * stride] = c;
+      dst[i + 3 * stride] = d;
+    }
+  }
+  for (; i < n4; i += stride) {
+    if (mode == 2)
+      acc += src[i];
+    else
+      dst[i] = src[i];
+  }
+  if (mode == 2 && acc.x == 123.456f) dst[0] = acc;  // keeps the reads live
 }
 
 static int host_finite(const float* p, size_t n, const char* what) {
@@ -368,7 +415,7 @@ RTM_EXPORT int rtm_cfl_check(float vmax, float dt, float dx, double* courant, do
 }
 
 
-RTM_EXPORT int rtm_bandwidth_probe(size_t nbytes, int reps, double* gbps, double* ms_per_copy) {
+RTM_EXPORT int rtm_bandwidth_probe(size_t nbytes, int reps, int mode, double* gbps, double* ms_per_copy) {
   if (nbytes < 1024 || reps < 1) { set_error("bad probe size"); return E_ARG; }
   STCHK(ensure_device());
   const size_t n4 = nbytes / 16;
@@ -379,11 +426,12 @@ RTM_EXPORT int rtm_bandwidth_probe(size_t nbytes, int reps, double* gbps, double
   hipEvent_t e0, e1;
   HIPCHK(hipEventCreate(&e0));
   HIPCHK(hipEventCreate(&e1));
+  if (mode < 0 || mode > 3) { set_error("probe mode must be 0..3"); return E_ARG; }
   const unsigned blocks = 256 * 32;  // 32 workgroups per CU, grid-stride
-  hipLaunchKernelGGL(k_copy_probe, dim3(blocks), dim3(256), 0, nullptr, a.as<float4>(), b.as<float4>(), n4);
+  hipLaunchKernelGGL(k_copy_probe, dim3(blocks), dim3(256), 0, nullptr, a.as<v4f>(), b.as<v4f>(), n4, mode);
   HIPCHK(hipEventRecord(e0, nullptr));
   for (int r = 0; r < reps; ++r)
-    hipLaunchKernelGGL(k_copy_probe, dim3(blocks), dim3(256), 0, nullptr, a.as<float4>(), b.as<float4>(), n4);
+    hipLaunchKernelGGL(k_copy_probe, dim3(blocks), dim3(256), 0, nullptr, a.as<v4f>(), b.as<v4f>(), n4, mode);
   HIPCHK(hipEventRecord(e1, nullptr));
   HIPCHK(hipEventSynchronize(e1));
   float ms = 0;
@@ -392,7 +440,8 @@ RTM_EXPORT int rtm_bandwidth_probe(size_t nbytes, int reps, double* gbps, double
   (void)hipEventDestroy(e1);
   const double per = ms / reps;
   if (ms_per_copy) *ms_per_copy = per;
-  if (gbps) *gbps = 2.0 * (double)(n4 * 16) / (per * 1e-3) / 1e9;
+  const double moved = (mode >= 2 ? 1.0 : 2.0) * (double)(n4 * 16);  // bytes read + written
+  if (gbps) *gbps = moved / (per * 1e-3) / 1e9;
   return OK;
 }
 

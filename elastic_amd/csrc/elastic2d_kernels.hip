@@ -127,34 +127,59 @@ __global__ __launch_bounds__(256) void k_fwd(const ShotDev* __restrict__ shots,
                                              const TileRef* __restrict__ tiles, int ntiles, int L,
                                              int flags, int xcd) {
   using T = Tile<TXR>;
+  constexpr int NIT = (T::N + 255) / 256;  // staged cells per thread
   __shared__ float sh[5][T::N];
   const TileRef tr = tiles[xcd ? xcd_tile(blockIdx.x, ntiles) : (int)blockIdx.x];
   const ShotDev& S = shots[tr.shot];
   const int nz = S.nz, nx = S.nx, ldz = S.ldz;
-  const long long fs = S.fstride, n2 = S.n2;
+  // element offsets of one shot fit in 32 bits (5 fields x n2 < 2^31)
+  const int fs = (int)S.fstride, n2 = (int)S.n2;
   float* const out = lev_ptr(S.lev, S.lev_stride, S.lev_mod, S.lev_off, L);
   const float* const x2 = lev_ptr(S.lev, S.lev_stride, S.lev_mod, S.lev_off, L - 2);
   const float* const y1 = lev_ptr(S.lev, S.lev_stride, S.lev_mod, S.lev_off, L - 1);
   const float* const coef = S.coef;
   const int z0 = tr.z0, x0 = tr.x0;
   const int tz = threadIdx.x, ty = threadIdx.y;
+  const int tid = tz + T::TZ * ty;
 
-  for (int lx = ty; lx < T::LX; lx += T::WY) {
-    const int gx = x0 - T::H + lx;
-    const bool xin = gx >= 0 && gx < nx;
-    for (int lz = tz; lz < T::LZ; lz += T::TZ) {
-      const int gz = z0 - T::H + lz;
-      const bool in = xin && gz >= 0 && gz < nz;
-      const long long gi = (long long)gz + (long long)ldz * gx;
-      const int p = lx * T::LZ + lz;
+  // Every global load of this thread is issued before the first LDS store or
+  // barrier: the staged tile of level L-1 (flat over tile + halo so all
+  // lanes stay busy) and the own-cell operands (level L-2, coefficients).
+  float st[NIT][5];
 #pragma unroll
-      for (int f = 0; f < 5; ++f) sh[f][p] = in ? y1[f * fs + gi] : 0.f;
+  for (int k = 0; k < NIT; ++k) {
+    const int c = tid + 256 * k;
+    const int lx = c / T::LZ, lz = c - lx * T::LZ;
+    const int gx = x0 - T::H + lx, gz = z0 - T::H + lz;
+    const bool in = c < T::N && gx >= 0 && gx < nx && gz >= 0 && gz < nz;
+    const int gi = gz + ldz * gx;
+#pragma unroll
+    for (int f = 0; f < 5; ++f) st[k][f] = in ? y1[f * fs + gi] : 0.f;
+  }
+  const int gz = z0 + tz;
+  const bool zok = gz < nz;
+  float ox[TXR][5], oc[TXR][5];
+#pragma unroll
+  for (int j = 0; j < TXR; ++j) {
+    const int gx = x0 + ty * TXR + j;
+    const bool ok = zok && gx < nx;
+    const int gi = gz + ldz * gx;
+#pragma unroll
+    for (int f = 0; f < 5; ++f) {
+      ox[j][f] = ok ? x2[f * fs + gi] : 0.f;
+      oc[j][f] = ok ? coef[f * n2 + gi] : 0.f;
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < NIT; ++k) {
+    const int c = tid + 256 * k;
+    if (c < T::N) {
+#pragma unroll
+      for (int f = 0; f < 5; ++f) sh[f][c] = st[k][f];
     }
   }
   __syncthreads();
 
-  const int gz = z0 + tz;
-  const bool zok = gz < nz;
   const float kz = ((flags & F_K) && zok) ? S.kz[gz] : 1.f;
   constexpr int SX = T::LZ;
   float sv[4][TXR];
@@ -167,12 +192,11 @@ __global__ __launch_bounds__(256) void k_fwd(const ShotDev* __restrict__ shots,
 #pragma unroll
     for (int a = 0; a < 4; ++a) sv[a][j] = 0.f;
     if (!ok) continue;
-    const long long gi = (long long)gz + (long long)ldz * gx;
-    const float m1x = coef[gi], m1z = coef[n2 + gi], m2 = coef[2 * n2 + gi],
-                m3 = coef[3 * n2 + gi], m2m3 = coef[4 * n2 + gi];
+    const int gi = gz + ldz * gx;
+    const float m1x = oc[j][0], m1z = oc[j][1], m2 = oc[j][2], m3 = oc[j][3], m2m3 = oc[j][4];
     float v[5];
 #pragma unroll
-    for (int f = 0; f < 5; ++f) v[f] = x2[f * fs + gi];
+    for (int f = 0; f < 5; ++f) v[f] = ox[j][f];
     if (flags & F_BORN) {
 #pragma unroll
       for (int f = 0; f < 5; ++f) v[f] = S.bsrc[f * n2 + gi] + v[f];
@@ -205,8 +229,13 @@ __global__ __launch_bounds__(256) void k_fwd(const ShotDev* __restrict__ shots,
 #pragma unroll
       for (int f = 0; f < 5; ++f) v[f] = (v[f] * kz) * kx;
     }
+    if (flags & F_NTS) {
 #pragma unroll
-    for (int f = 0; f < 5; ++f) out[f * fs + gi] = v[f];
+      for (int f = 0; f < 5; ++f) __builtin_nontemporal_store(v[f], out + f * fs + gi);
+    } else {
+#pragma unroll
+      for (int f = 0; f < 5; ++f) out[f * fs + gi] = v[f];
+    }
     if ((flags & (F_REC | F_RECSUB)) && gz == S.zr && gx >= S.rx0 && gx < S.rx0 + S.nrx) {
       float* r = S.rec + (long long)(L - 1) * S.nrx + (gx - S.rx0);
       const long long rs = (long long)S.nt * S.nrx;
@@ -219,7 +248,7 @@ __global__ __launch_bounds__(256) void k_fwd(const ShotDev* __restrict__ shots,
       }
     }
     if (SNAP == SNAP_HIST && (flags & F_SNAP)) {
-      float* h = static_cast<float*>(S.snap) + (long long)(L - 1) * 2 * n2;
+      float* h = static_cast<float*>(S.snap) + (long long)(L - 1) * 2 * (long long)n2;
       h[gi] = v[0];
       h[n2 + gi] = v[1];
     }
@@ -240,7 +269,7 @@ __global__ __launch_bounds__(256) void k_fwd(const ShotDev* __restrict__ shots,
       for (int j = 0; j < TXR; ++j) {
         const int gx = x0 + ty * TXR + j;
         if (!zok || gx >= nx) continue;
-        const long long gi = (long long)gz + (long long)ldz * gx;
+        const int gi = gz + ldz * gx;
 #pragma unroll
         for (int a = 0; a < 4; ++a) sp4[a * n2 + gi] = sv[a][j];
       }
@@ -259,7 +288,7 @@ __global__ __launch_bounds__(256) void k_fwd(const ShotDev* __restrict__ shots,
         for (int j = 0; j < TXR; ++j) {
           const int gx = x0 + ty * TXR + j;
           if (!zok || gx >= nx) continue;
-          const long long gi = (long long)gz + (long long)ldz * gx;
+          const int gi = gz + ldz * gx;
           q4[a * n2 + gi] = SnapQ<SNAP>::enc(sv[a][j] * inv);
         }
       }
@@ -328,18 +357,41 @@ __global__ __launch_bounds__(256) void k_snap_last(const ShotDev* __restrict__ s
   }
 }
 
-// Source Vx (a=0) or Vz (a=1) of imaging level L at (gz, gx), decoded.
+// Source-wavefield view of one imaging level L: per-level base pointers are
+// resolved once per block; elements are then addressed with 32-bit offsets.
 template <int SNAP>
-__device__ __forceinline__ float src_field(const ShotDev& S, int a, int L, long long gi, int gz,
-                                           int gx) {
-  const long long n2 = S.n2;
-  if (SNAP == SNAP_HIST) return static_cast<const float*>(S.snap)[((long long)(L - 1) * 2 + a) * n2 + gi];
-  const long long k0 = L / S.snap_every - 1;
-  if (SNAP == SNAP_F32) return static_cast<const float*>(S.snap)[(k0 * 4 + a) * n2 + gi];
-  const int strip = (gz >> 6) + S.snap_nbz * (gx >> S.snap_sw);
-  const float s = S.snap_scale[(k0 * 4 + a) * S.snap_nstrip + strip];
-  return SnapQ<SNAP>::dec(static_cast<const unsigned short*>(S.snap)[(k0 * 4 + a) * n2 + gi]) * s;
-}
+struct SrcView {
+  const float* f[4];           // fp32 arrays (Vx, Vz, dVx, dVz; HIST: Vx, Vz, Vx(L+1), Vz(L+1))
+  const unsigned short* q[4];  // bf16 / fp16 arrays
+  const float* sc[4];          // per-strip scales
+  int nbz, sw;
+  __device__ SrcView(const ShotDev& S, int L) {
+    const long long n2 = S.n2;
+    nbz = S.snap_nbz;
+    sw = S.snap_sw;
+    if (SNAP == SNAP_HIST) {
+      const float* h = static_cast<const float*>(S.snap) + (long long)(L - 1) * 2 * n2;
+      f[0] = h; f[1] = h + n2; f[2] = h + 2 * n2; f[3] = h + 3 * n2;
+    } else {
+      const long long k0 = L / S.snap_every - 1;
+#pragma unroll
+      for (int a = 0; a < 4; ++a) {
+        if (SNAP == SNAP_F32) {
+          f[a] = static_cast<const float*>(S.snap) + (k0 * 4 + a) * n2;
+        } else {
+          q[a] = static_cast<const unsigned short*>(S.snap) + (k0 * 4 + a) * n2;
+          sc[a] = S.snap_scale + (k0 * 4 + a) * S.snap_nstrip;
+        }
+      }
+    }
+  }
+  // array a at element gi = gz + ldz*gx
+  __device__ __forceinline__ float get(int a, int gi, int gz, int gx) const {
+    if (SNAP == SNAP_HIST || SNAP == SNAP_F32) return f[a][gi];
+    const int strip = (gz >> 6) + nbz * (gx >> sw);
+    return SnapQ<SNAP>::dec(q[a][gi]) * sc[a][strip];
+  }
+};
 
 // ---------------------------------------------------------------------------
 // Adjoint step (gather form of rtm_op3_c_, W/rtmlib-modC.c:120-253, with the
@@ -358,51 +410,80 @@ __global__ __launch_bounds__(256) void k_adj(const ShotDev* __restrict__ shots,
                                              const TileRef* __restrict__ tiles, int ntiles, int L,
                                              int flags, int xcd) {
   using T = Tile<TXR>;
+  constexpr int NIT = (T::N + 255) / 256;
   __shared__ float sh[5][T::N];
   const TileRef tr = tiles[xcd ? xcd_tile(blockIdx.x, ntiles) : (int)blockIdx.x];
   const ShotDev& S = shots[tr.shot];
   const int nz = S.nz, nx = S.nx, ldz = S.ldz;
-  const long long fs = S.fstride, n2 = S.n2;
+  const int fs = (int)S.fstride, n2 = (int)S.n2;
   float* const out = lev_ptr(S.lev, S.lev_stride, S.lev_mod, S.lev_off, L);
   const float* const x2 = lev_ptr(S.lev, S.lev_stride, S.lev_mod, S.lev_off, L + 2);
   const float* const y1 = lev_ptr(S.lev, S.lev_stride, S.lev_mod, S.lev_off, L + 1);
   const float* const coef = S.coef;
   const int z0 = tr.z0, x0 = tr.x0;
   const int tz = threadIdx.x, ty = threadIdx.y;
+  const int tid = tz + T::TZ * ty;
 
-  for (int lx = ty; lx < T::LX; lx += T::WY) {
-    const int gx = x0 - T::H + lx;
-    const bool xin = gx >= 0 && gx < nx;
-    for (int lz = tz; lz < T::LZ; lz += T::TZ) {
-      const int gz = z0 - T::H + lz;
-      const bool in = xin && gz >= 0 && gz < nz;
-      const int p = lx * T::LZ + lz;
-      float p0 = 0.f, p1 = 0.f, p2 = 0.f, p3 = 0.f, p4 = 0.f;
-      if (in) {
-        const long long gi = (long long)gz + (long long)ldz * gx;
-        const float vx = y1[gi], vz = y1[fs + gi], xx = y1[2 * fs + gi], zz = y1[3 * fs + gi],
-                    xz = y1[4 * fs + gi];
-        const float m1x = coef[gi], m1z = coef[n2 + gi], m2 = coef[2 * n2 + gi],
-                    m3 = coef[3 * n2 + gi], m2m3 = coef[4 * n2 + gi];
-        p0 = m2m3 * xx + m2 * zz;
-        p1 = m2 * xx + m2m3 * zz;
-        p2 = m3 * xz;
-        p3 = m1x * vx;
-        p4 = m1z * vz;
-      }
-      sh[0][p] = p0;
-      sh[1][p] = p1;
-      sh[2][p] = p2;
-      sh[3][p] = p3;
-      sh[4][p] = p4;
+  // Staged tile: coefficient-weighted products of level L+1 (flat over tile +
+  // halo); own-cell operands are loaded before the barrier as well.
+  float st[NIT][5];
+#pragma unroll
+  for (int k = 0; k < NIT; ++k) {
+    const int c = tid + 256 * k;
+    const int lx = c / T::LZ, lz = c - lx * T::LZ;
+    const int gx = x0 - T::H + lx, gz = z0 - T::H + lz;
+    const bool in = c < T::N && gx >= 0 && gx < nx && gz >= 0 && gz < nz;
+    float p0 = 0.f, p1 = 0.f, p2 = 0.f, p3 = 0.f, p4 = 0.f;
+    if (in) {
+      const int gi = gz + ldz * gx;
+      const float vx = y1[gi], vz = y1[fs + gi], xx = y1[2 * fs + gi], zz = y1[3 * fs + gi],
+                  xz = y1[4 * fs + gi];
+      const float m1x = coef[gi], m1z = coef[n2 + gi], m2 = coef[2 * n2 + gi],
+                  m3 = coef[3 * n2 + gi], m2m3 = coef[4 * n2 + gi];
+      p0 = m2m3 * xx + m2 * zz;
+      p1 = m2 * xx + m2m3 * zz;
+      p2 = m3 * xz;
+      p3 = m1x * vx;
+      p4 = m1z * vz;
+    }
+    st[k][0] = p0;
+    st[k][1] = p1;
+    st[k][2] = p2;
+    st[k][3] = p3;
+    st[k][4] = p4;
+  }
+  const int gz = z0 + tz;
+  const bool zok = gz < nz;
+  const bool row = (flags & A_INJ) && gz == S.zr;
+  float b[TXR][5];
+#pragma unroll
+  for (int j = 0; j < TXR; ++j) {
+    const int gx = x0 + ty * TXR + j;
+    const bool ok = zok && gx < nx;
+    const int gi = gz + ldz * gx;
+#pragma unroll
+    for (int f = 0; f < 5; ++f) b[j][f] = ok ? x2[f * fs + gi] : 0.f;
+    if (ok && (flags & A_XIN)) {
+#pragma unroll
+      for (int f = 0; f < 5; ++f) b[j][f] = out[f * fs + gi] + b[j][f];
+    } else if (ok && row && gx >= S.rx0 && gx < S.rx0 + S.nrx) {
+      const float* r = S.rec + (long long)(L - 1) * S.nrx + (gx - S.rx0);
+      const long long rs = (long long)S.nt * S.nrx;
+#pragma unroll
+      for (int f = 0; f < 5; ++f) b[j][f] = r[f * rs] + b[j][f];
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < NIT; ++k) {
+    const int c = tid + 256 * k;
+    if (c < T::N) {
+#pragma unroll
+      for (int f = 0; f < 5; ++f) sh[f][c] = st[k][f];
     }
   }
   __syncthreads();
 
-  const int gz = z0 + tz;
-  const bool zok = gz < nz;
   const float kz = ((flags & A_K) && zok) ? S.kz[gz] : 1.f;
-  const bool row = (flags & A_INJ) && gz == S.zr;
   float o[TXR][5];
   constexpr int SX = T::LZ;
 #pragma unroll
@@ -414,27 +495,14 @@ __global__ __launch_bounds__(256) void k_adj(const ShotDev* __restrict__ shots,
 #pragma unroll
     for (int f = 0; f < 5; ++f) o[j][f] = 0.f;
     if (!ok) continue;
-    const long long gi = (long long)gz + (long long)ldz * gx;
-    float b[5];
-    if (flags & A_XIN) {
-#pragma unroll
-      for (int f = 0; f < 5; ++f) b[f] = out[f * fs + gi] + x2[f * fs + gi];
-    } else if (row && gx >= S.rx0 && gx < S.rx0 + S.nrx) {
-      const float* r = S.rec + (long long)(L - 1) * S.nrx + (gx - S.rx0);
-      const long long rs = (long long)S.nt * S.nrx;
-#pragma unroll
-      for (int f = 0; f < 5; ++f) b[f] = r[f * rs] + x2[f * fs + gi];
-    } else {
-#pragma unroll
-      for (int f = 0; f < 5; ++f) b[f] = x2[f * fs + gi];
-    }
-    float v0 = b[0] - sm<SX>(sh[0], p);
+    const int gi = gz + ldz * gx;
+    float v0 = b[j][0] - sm<SX>(sh[0], p);
     v0 -= sp<1>(sh[2], p);
-    float v1 = b[1] - sm<1>(sh[1], p);
+    float v1 = b[j][1] - sm<1>(sh[1], p);
     v1 -= sp<SX>(sh[2], p);
-    const float v2 = b[2] - sp<SX>(sh[3], p);
-    const float v3 = b[3] - sp<1>(sh[4], p);
-    float v4 = b[4] - sm<1>(sh[3], p);
+    const float v2 = b[j][2] - sp<SX>(sh[3], p);
+    const float v3 = b[j][3] - sp<1>(sh[4], p);
+    float v4 = b[j][4] - sm<1>(sh[3], p);
     v4 -= sm<SX>(sh[4], p);
     float v[5] = {v0, v1, v2, v3, v4};
     if (flags & A_K) {
@@ -442,25 +510,37 @@ __global__ __launch_bounds__(256) void k_adj(const ShotDev* __restrict__ shots,
 #pragma unroll
       for (int f = 0; f < 5; ++f) v[f] = (v[f] * kz) * kx;
     }
+    if (flags & A_NTS) {
 #pragma unroll
-    for (int f = 0; f < 5; ++f) {
-      out[f * fs + gi] = v[f];
-      o[j][f] = v[f];
+      for (int f = 0; f < 5; ++f) __builtin_nontemporal_store(v[f], out + f * fs + gi);
+    } else {
+#pragma unroll
+      for (int f = 0; f < 5; ++f) out[f * fs + gi] = v[f];
     }
+#pragma unroll
+    for (int f = 0; f < 5; ++f) o[j][f] = v[f];
   }
 
   if (!(flags & A_IMG)) return;  // uniform over the launch
+  const SrcView<SNAP> sv(S, L);
+  float sx[NIT][2];
+#pragma unroll
+  for (int k = 0; k < NIT; ++k) {
+    const int c = tid + 256 * k;
+    const int lx = c / T::LZ, lz = c - lx * T::LZ;
+    const int gx = x0 - T::H + lx, gz2 = z0 - T::H + lz;
+    const bool in = c < T::N && gx >= 0 && gx < nx && gz2 >= 0 && gz2 < nz;
+    const int gi = gz2 + ldz * gx;
+    sx[k][0] = in ? sv.get(0, gi, gz2, gx) : 0.f;
+    sx[k][1] = in ? sv.get(1, gi, gz2, gx) : 0.f;
+  }
   __syncthreads();
-  for (int lx = ty; lx < T::LX; lx += T::WY) {
-    const int gx = x0 - T::H + lx;
-    const bool xin = gx >= 0 && gx < nx;
-    for (int lz = tz; lz < T::LZ; lz += T::TZ) {
-      const int gz2 = z0 - T::H + lz;
-      const bool in = xin && gz2 >= 0 && gz2 < nz;
-      const long long gi = (long long)gz2 + (long long)ldz * gx;
-      const int p = lx * T::LZ + lz;
-      sh[0][p] = in ? src_field<SNAP>(S, 0, L, gi, gz2, gx) : 0.f;
-      sh[1][p] = in ? src_field<SNAP>(S, 1, L, gi, gz2, gx) : 0.f;
+#pragma unroll
+  for (int k = 0; k < NIT; ++k) {
+    const int c = tid + 256 * k;
+    if (c < T::N) {
+      sh[0][c] = sx[k][0];
+      sh[1][c] = sx[k][1];
     }
   }
   __syncthreads();
@@ -475,17 +555,16 @@ __global__ __launch_bounds__(256) void k_adj(const ShotDev* __restrict__ shots,
     const int gx = x0 + lxo;
     if (!zok || gx >= nx) continue;
     const int p = (lxo + T::H) * T::LZ + (tz + T::H);
-    const long long gi = (long long)gz + (long long)ldz * gx;
+    const int gi = gz + ldz * gx;
     // S1, S2: forward time difference (T, W/image-mod-rtm.f90:288-297)
     float s1 = 0.f, s2 = 0.f;
     if (next) {
-      if (SNAP == SNAP_HIST) {
-        const float* h1 = static_cast<const float*>(S.snap) + (long long)L * 2 * n2;
-        s1 = inv_dt * (h1[gi] - sh[0][p]);
-        s2 = inv_dt * (h1[n2 + gi] - sh[1][p]);
+      if (SNAP == SNAP_HIST) {  // arrays 2, 3 are Vx, Vz of level L+1
+        s1 = inv_dt * (sv.get(2, gi, gz, gx) - sh[0][p]);
+        s2 = inv_dt * (sv.get(3, gi, gz, gx) - sh[1][p]);
       } else {
-        s1 = inv_dt * src_field<SNAP>(S, 2, L, gi, gz, gx);
-        s2 = inv_dt * src_field<SNAP>(S, 3, L, gi, gz, gx);
+        s1 = inv_dt * sv.get(2, gi, gz, gx);
+        s2 = inv_dt * sv.get(3, gi, gz, gx);
       }
     }
     // S4 = vxdx, S5 = vzdz (range 4..n-6), S6 = vxdz + vzdx (range 5..n-6)

@@ -28,6 +28,8 @@ int ensure_device() {
 // numbering; RTM_TXR / RTM_XCD_REMAP override them (autotuner, experiments).
 int default_txr() { return env_int("RTM_TXR", 4); }
 int default_xcd() { return env_int("RTM_XCD_REMAP", 1); }
+// Non-temporal stores of the new level (RTM_NT=1).
+static int nt_stores() { return env_int("RTM_NT", 0); }
 
 // Sponge taper, W/abc-mod.f90:35-50, and per-row/column factors of K
 // (:59-78): rows first then columns, so corners are damped twice.
@@ -315,6 +317,7 @@ int Engine::enqueue_forward(int g, int mode, hipStream_t st) {
     if (mode == 1 || mode == 3) fl |= F_REC;
     if (mode == 2) fl |= F_RECSUB;
     if (mode == 4) fl |= F_SNAP;
+    if (nt_stores()) fl |= F_NTS;
     if (mode == 0) {
       if (cfg_.snap_fmt == SNAP_HIST) fl |= F_SNAP;
       else if ((L - 1) % cfg_.snap_every == 0) fl |= F_SNAP;
@@ -342,6 +345,7 @@ int Engine::enqueue_adjoint(int g, hipStream_t st) {
     if (L % cfg_.snap_every == 0) fl |= A_IMG;
     if (L < cfg_.nt) fl |= A_NEXT;
     if (cfg_.comp) fl |= A_COMP;
+    if (nt_stores()) fl |= A_NTS;
     STCHK(launch_adj(desc, tiles, gr.ntiles, L, fl, cfg_.txr, cfg_.snap_fmt, cfg_.xcd, st));
   }
   if (cfg_.comp)

@@ -312,10 +312,11 @@ class Batch:
             pass
 
 
-def bandwidth_probe(nbytes=1 << 30, reps=20):
-    """Achieved read+write GB/s of a streaming device copy and ms per copy."""
+def bandwidth_probe(nbytes=1 << 30, reps=20, mode=0):
+    """Achieved GB/s (read + written) of a streaming kernel and ms per launch.
+    mode 0 copy, 1 non-temporal copy, 2 read only, 3 write only."""
     g, ms = C.c_double(), C.c_double()
-    _chk(load().rtm_bandwidth_probe(C.c_size_t(nbytes), C.c_int(reps), C.byref(g), C.byref(ms)))
+    _chk(load().rtm_bandwidth_probe(C.c_size_t(nbytes), C.c_int(reps), C.c_int(mode), C.byref(g), C.byref(ms)))
     return g.value, ms.value
 
 

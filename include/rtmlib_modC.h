@@ -199,11 +199,12 @@ int rtm_batch_get_stack(rtm_batch *b, float *img3);            /* 3*(stack_nz,st
 int rtm_batch_info(rtm_batch *b, double *out8);
 int rtm_batch_destroy(rtm_batch *b);
 
-/* Streaming device-to-device copy of nbytes, timed over reps launches with
- * HIP events: achieved read+write GB/s and ms per copy. Used to measure the
- * attainable HBM bandwidth and to calibrate the FETCH_SIZE/WRITE_SIZE
- * counters against a kernel of known traffic. */
-int rtm_bandwidth_probe(size_t nbytes, int reps, double *gbps, double *ms_per_copy);
+/* Streaming kernel over nbytes, timed over reps launches with HIP events:
+ * achieved GB/s (bytes read + written) and ms per launch. mode 0: copy,
+ * 1: copy with non-temporal loads/stores, 2: read only, 3: write only. Used to
+ * measure the attainable HBM bandwidth and to calibrate the FETCH_SIZE /
+ * WRITE_SIZE counters against a kernel of known traffic. */
+int rtm_bandwidth_probe(size_t nbytes, int reps, int mode, double *gbps, double *ms_per_copy);
 
 /* Library lifecycle: optional explicit device selection and release of the
  * cached device state of the calling process. */
