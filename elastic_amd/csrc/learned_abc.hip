@@ -1,0 +1,801 @@
+// learned_abc.hip -- learned absorbing layer: band geometry, features,
+// training (forward, backward, Adam) and inference kernels (gfx950).
+//
+// The network is small (8965 parameters) and evaluated on the ~6% of cells
+// that form the 8-cell band of a grid. Training and inference both use BF16-
+// rounded weights and features with FP32 accumulation; the training step keeps
+// FP32 master weights and Adam moments.
+
+#include "../../include/learned_abc.h"
+#include "engine2d.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <random>
+#include <vector>
+
+namespace erm {
+namespace labc {
+
+constexpr int R = LABC_RIM, W = LABC_WIDTH, NF = LABC_NFEAT, KP = LABC_KPAD, H = LABC_HID,
+              NO = LABC_NOUT;
+constexpr int OW1 = 0, OB1 = NF * H, OW2 = OB1 + H, OB2 = OW2 + H * H, OW3 = OB2 + H,
+              OB3 = OW3 + H * NO, NP = OB3 + NO;
+static_assert(NP == LABC_NPARAM, "parameter count");
+constexpr float Z0 = LABC_Z0;
+
+struct Band {
+  int nz, nx, nT, nL, total;
+};
+__host__ __device__ inline Band band_of(int nz, int nx) {
+  Band b;
+  b.nz = nz;
+  b.nx = nx;
+  b.nT = W * (nx - 2 * R);
+  b.nL = W * (nz - 2 * R - 2 * W);
+  b.total = 2 * b.nT + 2 * b.nL;
+  return b;
+}
+// edge 0 top (inward +z), 1 bottom (-z), 2 left (+x), 3 right (-x); depth 0
+// is the band row/column next to the rim.
+__host__ __device__ inline void cell_coords(const Band& b, int c, int& iz, int& ix, int& edge,
+                                            int& depth) {
+  const int cols = b.nx - 2 * R, rows = b.nz - 2 * R - 2 * W;
+  if (c < b.nT) {
+    depth = c / cols; iz = R + depth; ix = R + c % cols; edge = 0;
+    return;
+  }
+  c -= b.nT;
+  if (c < b.nT) {
+    depth = c / cols; iz = b.nz - R - 1 - depth; ix = R + c % cols; edge = 1;
+    return;
+  }
+  c -= b.nT;
+  if (c < b.nL) {
+    depth = c / rows; ix = R + depth; iz = R + W + c % rows; edge = 2;
+    return;
+  }
+  c -= b.nL;
+  depth = c / rows; ix = b.nx - R - 1 - depth; iz = R + W + c % rows; edge = 3;
+}
+__host__ __device__ inline int cell_index(const Band& b, int iz, int ix) {
+  const int cols = b.nx - 2 * R, rows = b.nz - 2 * R - 2 * W;
+  if (ix < R || ix >= b.nx - R || iz < R || iz >= b.nz - R) return -1;
+  if (iz < R + W) return (iz - R) * cols + (ix - R);
+  if (iz >= b.nz - R - W) return b.nT + (b.nz - R - 1 - iz) * cols + (ix - R);
+  if (ix < R + W) return 2 * b.nT + (ix - R) * rows + (iz - R - W);
+  if (ix >= b.nx - R - W) return 2 * b.nT + b.nL + (b.nx - R - 1 - ix) * rows + (iz - R - W);
+  return -1;
+}
+
+__device__ __forceinline__ float bf16r(float x) {  // round to bf16 (nearest even)
+  unsigned u = __float_as_uint(x);
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return __uint_as_float(u & 0xffff0000u);
+}
+static inline float bf16r_host(float x) {
+  unsigned u;
+  std::memcpy(&u, &x, 4);
+  u += 0x7fffu + ((u >> 16) & 1u);
+  u &= 0xffff0000u;
+  float y;
+  std::memcpy(&y, &u, 4);
+  return y;
+}
+__device__ __forceinline__ float gelu(float x) { return 0.5f * x * (1.f + erff(x * 0.70710678f)); }
+__device__ __forceinline__ float gelu_d(float x) {
+  return 0.5f * (1.f + erff(x * 0.70710678f)) + x * 0.39894228f * expf(-0.5f * x * x);
+}
+
+// Wavefield value of field f at (iz, ix) scaled to velocity units, 0 outside.
+struct Lev {
+  const float* p;  // field f at p + f*fs
+  int fs, ldz, nz, nx;
+  __device__ __forceinline__ float get(int f, int iz, int ix) const {
+    if (iz < 0 || iz >= nz || ix < 0 || ix >= nx) return 0.f;
+    const float v = p[f * fs + iz + ldz * ix];
+    return f >= 2 ? v * (1.f / Z0) : v;
+  }
+};
+
+// Raw (unnormalised) feature k of a band cell and its scale contribution.
+__device__ inline float raw_feature(const Lev& m1, const Lev& m2, int iz, int ix, int edge, int depth,
+                                    int k) {
+  if (k < 225) {
+    const int kn = k / 25, rem = k % 25, m = rem / 5 - 2, f = rem % 5;
+    const int dz = edge == 0 ? 1 : edge == 1 ? -1 : 0;
+    const int dx = edge == 2 ? 1 : edge == 3 ? -1 : 0;
+    const int tz = (edge >= 2) ? 1 : 0, tx = (edge < 2) ? 1 : 0;
+    return m1.get(f, iz + kn * dz + m * tz, ix + kn * dx + m * tx);
+  }
+  if (k < 230) return m2.get(k - 225, iz, ix);
+  return 0.f;
+}
+__device__ inline float onehot_feature(int edge, int depth, int k) {
+  if (k >= 230 && k < 234) return (k - 230) == edge ? 1.f : 0.f;
+  if (k >= 234 && k < 242) return (k - 234) == depth ? 1.f : 0.f;
+  return 0.f;
+}
+
+// Features [ncell][KP] (normalised, bf16-rounded) and scales of every band
+// cell of one grid.
+__global__ void k_features(Lev m1, Lev m2, Band b, float* feat, float* scale) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= b.total) return;
+  int iz, ix, edge, depth;
+  cell_coords(b, c, iz, ix, edge, depth);
+  float s = 0.f;
+  for (int k = 0; k < 230; ++k) s = fmaxf(s, fabsf(raw_feature(m1, m2, iz, ix, edge, depth, k)));
+  const float inv = s > 1e-30f ? 1.f / s : 0.f;
+  float* out = feat + (size_t)c * KP;
+  for (int k = 0; k < KP; ++k) {
+    const float v = k < 230 ? raw_feature(m1, m2, iz, ix, edge, depth, k) * inv : onehot_feature(edge, depth, k);
+    out[k] = bf16r(v);
+  }
+  scale[c] = s > 1e-30f ? s : 0.f;
+}
+
+// Targets: (wide(L) - plain(L)) / (s * unit) on the band cells.
+__global__ void k_targets(const float* wide, int wfs, int wldz, int off, const float* plain, int pfs,
+                          int pldz, Band b, const float* scale, float* tgt) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= b.total) return;
+  int iz, ix, edge, depth;
+  cell_coords(b, c, iz, ix, edge, depth);
+  const float s = scale[c];
+  for (int f = 0; f < NO; ++f) {
+    const float w = wide[f * wfs + (iz + off) + wldz * (ix + off)];
+    const float p = plain[f * pfs + iz + pldz * ix];
+    const float u = f >= 2 ? Z0 : 1.f;
+    tgt[(size_t)c * NO + f] = s > 0.f ? (w - p) / (s * u) : 0.f;
+  }
+}
+
+// Window copy of one level (5 fields): small(iz, ix) = wide(iz+off, ix+off),
+// rim cells zero.
+__global__ void k_window(const float* wide, int wfs, int wldz, int off, float* small, int sfs, int sldz,
+                         int nz, int nx) {
+  const int iz = blockIdx.x * blockDim.x + threadIdx.x, ix = blockIdx.y;
+  if (iz >= nz || ix >= nx) return;
+  const bool rim = iz < R || iz >= nz - R || ix < R || ix >= nx - R;
+  for (int f = 0; f < 5; ++f)
+    small[f * sfs + iz + sldz * ix] = rim ? 0.f : wide[f * wfs + (iz + off) + wldz * (ix + off)];
+}
+
+// Bf16-rounded copy of the master weights (the weights the forward uses).
+__global__ void k_round_params(const float* master, float* w) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < NP) w[i] = bf16r(master[i]);
+}
+
+// One evaluation of the network for up to 32 samples per block of 256
+// threads; weights staged in LDS. With grad != nullptr the block also runs
+// the backward pass of 0.5*mean((y - t)^2) (normaliser inv_count) and
+// accumulates parameter gradients with atomics. Samples are rows idx[i] of
+// feat (or i when idx is null).
+constexpr int SB = 32;  // samples per block
+__global__ __launch_bounds__(256) void k_mlp(const float* __restrict__ w, const float* __restrict__ feat,
+                                             const int* __restrict__ idx, int n, const float* __restrict__ tgt,
+                                             float* __restrict__ out, float* __restrict__ grad,
+                                             float* __restrict__ loss, float inv_count) {
+  __shared__ float sW1[NF * H];
+  __shared__ float sx[SB][KP];
+  __shared__ float sz1[SB][H], sh1[SB][H], sz2[SB][H], sh2[SB][H], sy[SB][8], sd[SB][H];
+  const int t = threadIdx.x;
+  for (int i = t; i < NF * H; i += 256) sW1[i] = w[OW1 + i];
+  const int s0 = blockIdx.x * SB;
+  for (int i = t; i < SB * KP; i += 256) {
+    const int s = i / KP, k = i % KP;
+    const int gs = s0 + s;
+    float v = 0.f;
+    if (gs < n) {
+      const int row = idx ? idx[gs] : gs;
+      v = feat[(size_t)row * KP + k];
+    }
+    sx[s][k] = v;
+  }
+  __syncthreads();
+  const int s = t / 8, q = t % 8;  // sample, group of 4 hidden units
+  const bool valid = s0 + s < n;
+  // layer 1
+  {
+    float acc[4];
+    for (int u = 0; u < 4; ++u) acc[u] = w[OB1 + 4 * q + u];
+    for (int k = 0; k < NF; ++k) {
+      const float x = sx[s][k];
+      for (int u = 0; u < 4; ++u) acc[u] += x * sW1[k * H + 4 * q + u];
+    }
+    for (int u = 0; u < 4; ++u) {
+      sz1[s][4 * q + u] = acc[u];
+      sh1[s][4 * q + u] = gelu(acc[u]);
+    }
+  }
+  __syncthreads();
+  // layer 2
+  {
+    float acc[4];
+    for (int u = 0; u < 4; ++u) acc[u] = w[OB2 + 4 * q + u];
+    for (int k = 0; k < H; ++k) {
+      const float x = sh1[s][k];
+      for (int u = 0; u < 4; ++u) acc[u] += x * w[OW2 + k * H + 4 * q + u];
+    }
+    for (int u = 0; u < 4; ++u) {
+      sz2[s][4 * q + u] = acc[u];
+      sh2[s][4 * q + u] = gelu(acc[u]);
+    }
+  }
+  __syncthreads();
+  // layer 3 (linear)
+  if (q < NO) {
+    float acc = w[OB3 + q];
+    for (int k = 0; k < H; ++k) acc += sh2[s][k] * w[OW3 + k * NO + q];
+    sy[s][q] = acc;
+    if (valid && out) out[(size_t)(s0 + s) * NO + q] = acc;
+  }
+  if (!grad) return;
+  __syncthreads();
+  // dL/dy = (y - t) * inv_count ; loss = 0.5 * sum (y-t)^2 * inv_count
+  if (q < NO) {
+    float d = 0.f;
+    if (valid) {
+      const int row = idx ? idx[s0 + s] : s0 + s;
+      d = sy[s][q] - tgt[(size_t)row * NO + q];
+      atomicAdd(loss, 0.5f * d * d * inv_count);
+    }
+    sy[s][q] = d * inv_count;
+  }
+  __syncthreads();
+  // W3, b3 gradients: thread t < H*NO handles one entry, summed over samples
+  if (t < H * NO) {
+    const int k = t / NO, o = t % NO;
+    float g = 0.f;
+    for (int ss = 0; ss < SB; ++ss) g += sh2[ss][k] * sy[ss][o];
+    atomicAdd(&grad[OW3 + t], g);
+  }
+  if (t < NO) {
+    float g = 0.f;
+    for (int ss = 0; ss < SB; ++ss) g += sy[ss][t];
+    atomicAdd(&grad[OB3 + t], g);
+  }
+  // dz2 = (dy W3^T) * gelu'(z2)
+  for (int u = 0; u < 4; ++u) {
+    const int j = 4 * q + u;
+    float g = 0.f;
+    for (int o = 0; o < NO; ++o) g += sy[s][o] * w[OW3 + j * NO + o];
+    sd[s][j] = g * gelu_d(sz2[s][j]);
+  }
+  __syncthreads();
+  // W2 (1024 entries, 4 per thread), b2
+  for (int e = t; e < H * H; e += 256) {
+    const int k = e / H, j = e % H;
+    float g = 0.f;
+    for (int ss = 0; ss < SB; ++ss) g += sh1[ss][k] * sd[ss][j];
+    atomicAdd(&grad[OW2 + e], g);
+  }
+  if (t < H) {
+    float g = 0.f;
+    for (int ss = 0; ss < SB; ++ss) g += sd[ss][t];
+    atomicAdd(&grad[OB2 + t], g);
+  }
+  // dz1 = (dz2 W2^T) * gelu'(z1) -> reuse sz1 for dz1 after everyone read sd
+  float dz1[4];
+  for (int u = 0; u < 4; ++u) {
+    const int k = 4 * q + u;
+    float g = 0.f;
+    for (int j = 0; j < H; ++j) g += sd[s][j] * w[OW2 + k * H + j];
+    dz1[u] = g * gelu_d(sz1[s][k]);
+  }
+  __syncthreads();
+  for (int u = 0; u < 4; ++u) sz1[s][4 * q + u] = dz1[u];
+  __syncthreads();
+  // W1 (NF*H entries), b1
+  for (int e = t; e < NF * H; e += 256) {
+    const int k = e / H, j = e % H;
+    float g = 0.f;
+    for (int ss = 0; ss < SB; ++ss) g += sx[ss][k] * sz1[ss][j];
+    atomicAdd(&grad[OW1 + e], g);
+  }
+  if (t < H) {
+    float g = 0.f;
+    for (int ss = 0; ss < SB; ++ss) g += sz1[ss][t];
+    atomicAdd(&grad[OB1 + t], g);
+  }
+}
+
+__global__ void k_adam(float* master, float* m, float* v, float* grad, float lr, float b1c, float b2c) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= NP) return;
+  const float g = grad[i];
+  const float mi = 0.9f * m[i] + 0.1f * g;
+  const float vi = 0.999f * v[i] + 0.001f * g * g;
+  m[i] = mi;
+  v[i] = vi;
+  master[i] -= lr * (mi / b1c) / (sqrtf(vi / b2c) + 1e-8f);
+  grad[i] = 0.f;
+}
+
+// Random minibatch indices (LCG per step).
+__global__ void k_sample(int* idx, int batch, int n, unsigned seed) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= batch) return;
+  unsigned x = seed * 2654435761u + (unsigned)i * 2246822519u + 0x9e3779b9u;
+  x ^= x >> 16; x *= 0x7feb352du; x ^= x >> 15; x *= 0x846ca68bu; x ^= x >> 16;
+  idx[i] = (int)(x % (unsigned)n);
+}
+
+// Inference on a grid: corrections delta[cell][5] (physical units) from
+// levels L-1 and L-2, one thread per band cell, weights in LDS.
+__global__ __launch_bounds__(256) void k_infer(const float* __restrict__ w, Lev m1, Lev m2, Band b,
+                                               float* __restrict__ delta) {
+  __shared__ float sW1[NF * H];
+  __shared__ float sW2[H * H];
+  __shared__ float sW3[H * NO];
+  __shared__ float sb[H + H + NO];
+  for (int i = threadIdx.x; i < NF * H; i += 256) sW1[i] = w[OW1 + i];
+  for (int i = threadIdx.x; i < H * H; i += 256) sW2[i] = w[OW2 + i];
+  for (int i = threadIdx.x; i < H * NO; i += 256) sW3[i] = w[OW3 + i];
+  for (int i = threadIdx.x; i < H; i += 256) {
+    sb[i] = w[OB1 + i];
+    sb[H + i] = w[OB2 + i];
+  }
+  if (threadIdx.x < NO) sb[2 * H + threadIdx.x] = w[OB3 + threadIdx.x];
+  __syncthreads();
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= b.total) return;
+  int iz, ix, edge, depth;
+  cell_coords(b, c, iz, ix, edge, depth);
+  float s = 0.f;
+  for (int k = 0; k < 230; ++k) s = fmaxf(s, fabsf(raw_feature(m1, m2, iz, ix, edge, depth, k)));
+  if (!(s > 1e-30f)) {
+    for (int f = 0; f < NO; ++f) delta[(size_t)c * NO + f] = 0.f;
+    return;
+  }
+  const float inv = 1.f / s;
+  float h1[H];
+  for (int j = 0; j < H; ++j) h1[j] = sb[j];
+  for (int k = 0; k < NF; ++k) {
+    const float x = bf16r(k < 230 ? raw_feature(m1, m2, iz, ix, edge, depth, k) * inv : onehot_feature(edge, depth, k));
+    if (x == 0.f) continue;
+#pragma unroll
+    for (int j = 0; j < H; ++j) h1[j] += x * sW1[k * H + j];
+  }
+#pragma unroll
+  for (int j = 0; j < H; ++j) h1[j] = gelu(h1[j]);
+  float h2[H];
+#pragma unroll
+  for (int j = 0; j < H; ++j) h2[j] = sb[H + j];
+  for (int k = 0; k < H; ++k) {
+#pragma unroll
+    for (int j = 0; j < H; ++j) h2[j] += h1[k] * sW2[k * H + j];
+  }
+#pragma unroll
+  for (int j = 0; j < H; ++j) h2[j] = gelu(h2[j]);
+  for (int o = 0; o < NO; ++o) {
+    float y = sb[2 * H + o];
+    for (int k = 0; k < H; ++k) y += h2[k] * sW3[k * NO + o];
+    delta[(size_t)c * NO + o] = y * s * (o >= 2 ? Z0 : 1.f);
+  }
+}
+
+// level(L)[band cell] += delta[cell]
+__global__ void k_apply(float* lev, int fs, int ldz, Band b, const float* delta) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= b.total) return;
+  int iz, ix, edge, depth;
+  cell_coords(b, c, iz, ix, edge, depth);
+  for (int f = 0; f < NO; ++f) lev[f * fs + iz + ldz * ix] += delta[(size_t)c * NO + f];
+}
+
+// Zero the 5-cell rim of one level: the band grid's stencil updates reach
+// row/column 4, which a learned-layer run keeps at zero like the windowed
+// training levels.
+__global__ void k_zero_rim(float* lev, int fs, int ldz, int nz, int nx) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int per = 2 * R * nx + 2 * R * nz;
+  if (i >= per) return;
+  int iz, ix;
+  if (i < 2 * R * nx) {
+    const int r = i / nx;
+    ix = i % nx;
+    iz = r < R ? r : nz - 2 * R + r;
+  } else {
+    const int j = i - 2 * R * nx, r = j / nz;
+    iz = j % nz;
+    ix = r < R ? r : nx - 2 * R + r;
+  }
+  for (int f = 0; f < NO; ++f) lev[f * fs + iz + ldz * ix] = 0.f;
+}
+
+}  // namespace labc
+}  // namespace erm
+
+using namespace erm;
+using namespace erm::labc;
+
+static double g_last_ms = 0.0;
+
+RTM_EXPORT double labc_last_forward_ms(void) { return g_last_ms; }
+
+struct labc_net {
+  DevBuf master, w, m, v, grad, loss;
+  int t = 0;
+};
+
+struct labc_data {
+  int capacity = 0, size = 0;
+  DevBuf feat, tgt;
+};
+
+static int launch_mlp(labc_net* net, const float* feat, const int* idx, int n, const float* tgt, float* out,
+                      bool train, float inv_count, hipStream_t st) {
+  const int blocks = (n + SB - 1) / SB;
+  hipLaunchKernelGGL(k_mlp, dim3(blocks), dim3(256), 0, st, net->w.as<float>(), feat, idx, n, tgt, out,
+                     train ? net->grad.as<float>() : nullptr, net->loss.as<float>(), inv_count);
+  HIPCHK(hipGetLastError());
+  return OK;
+}
+
+static int refresh_weights(labc_net* n, hipStream_t st) {
+  hipLaunchKernelGGL(k_round_params, dim3((NP + 255) / 256), dim3(256), 0, st, n->master.as<float>(),
+                     n->w.as<float>());
+  HIPCHK(hipGetLastError());
+  return OK;
+}
+
+RTM_EXPORT int labc_net_create(labc_net** out, unsigned seed) {
+  if (!out) { set_error("null argument"); return E_ARG; }
+  STCHK(ensure_device());
+  std::unique_ptr<labc_net> n(new labc_net());
+  std::mt19937 rng(seed);
+  std::vector<float> p(NP, 0.f);
+  auto fill = [&](int off, int fan_in, int fan_out, int count) {
+    const float a = std::sqrt(6.0f / (float)(fan_in + fan_out));
+    std::uniform_real_distribution<float> u(-a, a);
+    for (int i = 0; i < count; ++i) p[off + i] = u(rng);
+  };
+  fill(OW1, NF, H, NF * H);
+  fill(OW2, H, H, H * H);
+  fill(OW3, H, NO, H * NO);
+  for (int i = 0; i < H * NO; ++i) p[OW3 + i] *= 0.01f;  // start near "no correction"
+  STCHK(n->master.alloc(NP * 4, "net"));
+  STCHK(n->w.alloc(NP * 4, "net"));
+  STCHK(n->m.alloc(NP * 4, "net"));
+  STCHK(n->v.alloc(NP * 4, "net"));
+  STCHK(n->grad.alloc(NP * 4, "net"));
+  STCHK(n->loss.alloc(4, "net"));
+  HIPCHK(hipMemcpy(n->master.p, p.data(), NP * 4, hipMemcpyHostToDevice));
+  HIPCHK(hipMemset(n->m.p, 0, NP * 4));
+  HIPCHK(hipMemset(n->v.p, 0, NP * 4));
+  HIPCHK(hipMemset(n->grad.p, 0, NP * 4));
+  STCHK(refresh_weights(n.get(), nullptr));
+  HIPCHK(hipDeviceSynchronize());
+  *out = n.release();
+  return OK;
+}
+
+RTM_EXPORT int labc_net_destroy(labc_net* n) {
+  delete n;
+  return OK;
+}
+
+RTM_EXPORT int labc_net_get_params(labc_net* n, float* p) {
+  if (!n || !p) { set_error("null argument"); return E_ARG; }
+  HIPCHK(hipMemcpy(p, n->master.p, NP * 4, hipMemcpyDeviceToHost));
+  return OK;
+}
+
+RTM_EXPORT int labc_net_set_params(labc_net* n, const float* p) {
+  if (!n || !p) { set_error("null argument"); return E_ARG; }
+  HIPCHK(hipMemcpy(n->master.p, p, NP * 4, hipMemcpyHostToDevice));
+  STCHK(refresh_weights(n, nullptr));
+  HIPCHK(hipDeviceSynchronize());
+  return OK;
+}
+
+RTM_EXPORT int labc_cell_count(int nz, int nx) {
+  if (nz < 2 * (R + W) + 1 || nx < 2 * (R + W) + 1) return 0;
+  return band_of(nz, nx).total;
+}
+
+RTM_EXPORT int labc_cell_coords(int nz, int nx, int c, int* iz, int* ix, int* edge, int* depth) {
+  const Band b = band_of(nz, nx);
+  if (c < 0 || c >= b.total || !iz || !ix || !edge || !depth) { set_error("bad cell"); return E_ARG; }
+  cell_coords(b, c, *iz, *ix, *edge, *depth);
+  return OK;
+}
+
+RTM_EXPORT int labc_net_forward(labc_net* n, const float* feat, int cnt, float* out, int mode) {
+  if (!n || !feat || !out || cnt < 1) { set_error("bad argument"); return E_ARG; }
+  if (mode != 0) { set_error("only the VALU forward (mode 0) is built"); return E_ARG; }
+  DevBuf f, o;
+  STCHK(f.alloc((size_t)cnt * KP * 4, "features"));
+  STCHK(o.alloc((size_t)cnt * NO * 4, "outputs"));
+  HIPCHK(hipMemcpy(f.p, feat, (size_t)cnt * KP * 4, hipMemcpyHostToDevice));
+  STCHK(launch_mlp(n, f.as<float>(), nullptr, cnt, nullptr, o.as<float>(), false, 0.f, nullptr));
+  HIPCHK(hipDeviceSynchronize());
+  HIPCHK(hipMemcpy(out, o.p, (size_t)cnt * NO * 4, hipMemcpyDeviceToHost));
+  return OK;
+}
+
+RTM_EXPORT int labc_net_grad(labc_net* n, const float* feat, const float* target, int cnt, float* grad,
+                             float* loss) {
+  if (!n || !feat || !target || !grad || !loss || cnt < 1) { set_error("bad argument"); return E_ARG; }
+  DevBuf f, t;
+  STCHK(f.alloc((size_t)cnt * KP * 4, "features"));
+  STCHK(t.alloc((size_t)cnt * NO * 4, "targets"));
+  HIPCHK(hipMemcpy(f.p, feat, (size_t)cnt * KP * 4, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(t.p, target, (size_t)cnt * NO * 4, hipMemcpyHostToDevice));
+  HIPCHK(hipMemset(n->grad.p, 0, NP * 4));
+  HIPCHK(hipMemset(n->loss.p, 0, 4));
+  STCHK(launch_mlp(n, f.as<float>(), nullptr, cnt, t.as<float>(), nullptr, true, 1.f / (cnt * NO), nullptr));
+  HIPCHK(hipDeviceSynchronize());
+  HIPCHK(hipMemcpy(grad, n->grad.p, NP * 4, hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpy(loss, n->loss.p, 4, hipMemcpyDeviceToHost));
+  HIPCHK(hipMemset(n->grad.p, 0, NP * 4));
+  return OK;
+}
+
+RTM_EXPORT int labc_features(int nz, int nx, const float* lev_m1, const float* lev_m2, float* feat,
+                             float* scale) {
+  const int nc = labc_cell_count(nz, nx);
+  if (nc == 0 || !lev_m1 || !lev_m2 || !feat || !scale) { set_error("bad argument"); return E_ARG; }
+  STCHK(ensure_device());
+  const size_t n2 = (size_t)nz * nx;
+  DevBuf a, b, f, s;
+  STCHK(a.alloc(5 * n2 * 4, "level"));
+  STCHK(b.alloc(5 * n2 * 4, "level"));
+  STCHK(f.alloc((size_t)nc * KP * 4, "features"));
+  STCHK(s.alloc((size_t)nc * 4, "scales"));
+  HIPCHK(hipMemcpy(a.p, lev_m1, 5 * n2 * 4, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(b.p, lev_m2, 5 * n2 * 4, hipMemcpyHostToDevice));
+  Lev l1{a.as<float>(), (int)n2, nz, nz, nx}, l2{b.as<float>(), (int)n2, nz, nz, nx};
+  const Band bd = band_of(nz, nx);
+  hipLaunchKernelGGL(k_features, dim3((nc + 255) / 256), dim3(256), 0, nullptr, l1, l2, bd, f.as<float>(),
+                     s.as<float>());
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipDeviceSynchronize());
+  HIPCHK(hipMemcpy(feat, f.p, (size_t)nc * KP * 4, hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpy(scale, s.p, (size_t)nc * 4, hipMemcpyDeviceToHost));
+  return OK;
+}
+
+RTM_EXPORT int labc_data_create(labc_data** out, int capacity) {
+  if (!out || capacity < 1) { set_error("bad argument"); return E_ARG; }
+  STCHK(ensure_device());
+  std::unique_ptr<labc_data> d(new labc_data());
+  d->capacity = capacity;
+  STCHK(d->feat.alloc((size_t)capacity * KP * 4, "training features"));
+  STCHK(d->tgt.alloc((size_t)capacity * NO * 4, "training targets"));
+  *out = d.release();
+  return OK;
+}
+
+RTM_EXPORT int labc_data_destroy(labc_data* d) {
+  delete d;
+  return OK;
+}
+
+RTM_EXPORT int labc_data_size(labc_data* d) { return d ? d->size : 0; }
+
+RTM_EXPORT int labc_data_get(labc_data* d, int row0, int n, float* feat, float* tgt) {
+  if (!d || row0 < 0 || n < 0 || row0 + n > d->size) { set_error("rows out of range"); return E_ARG; }
+  if (n == 0) return OK;
+  if (feat) HIPCHK(hipMemcpy(feat, d->feat.as<float>() + (size_t)row0 * KP, (size_t)n * KP * 4, hipMemcpyDeviceToHost));
+  if (tgt) HIPCHK(hipMemcpy(tgt, d->tgt.as<float>() + (size_t)row0 * NO, (size_t)n * NO * 4, hipMemcpyDeviceToHost));
+  return OK;
+}
+
+// One grid (shot) with its own ring/coefficients, driven step by step.
+struct LGrid {
+  ShotDev h{};
+  DevBuf desc, tiles, ring, coef, kfac, amp;
+  int ntiles = 0, ldz = 0;
+  long long n2 = 0;
+  int setup(int nz, int nx, int nt, float dt, int zs, int xs, const float* coef5, const float* wavelet, int abc,
+            int zr, int rx0, int nrx, float* rec_dev) {
+    ldz = device_ldz(nz);
+    n2 = (long long)ldz * nx;
+    STCHK(ring.alloc(10 * n2 * 4, "ring"));
+    STCHK(coef.alloc(5 * n2 * 4, "coefficients"));
+    STCHK(fill_masked_coefs(coef5, nz, nx, ldz, coef.as<float>()));
+    STCHK(launch_fill(ring.as<float>(), 10 * n2, 0.f, nullptr));
+    h.nz = nz; h.nx = nx; h.ldz = ldz; h.nt = nt; h.n2 = n2;
+    h.lev = ring.as<float>(); h.lev_stride = 5 * n2; h.fstride = n2; h.lev_mod = 2; h.lev_off = 2;
+    h.coef = coef.as<float>();
+    if (abc > 0) {
+      std::vector<float> hz, hx;
+      sponge_factors(nz, abc, hz);
+      sponge_factors(nx, abc, hx);
+      STCHK(kfac.alloc((size_t)(nz + nx) * 4, "sponge"));
+      HIPCHK(hipMemcpy(kfac.p, hz.data(), nz * 4, hipMemcpyHostToDevice));
+      HIPCHK(hipMemcpy(kfac.as<float>() + nz, hx.data(), nx * 4, hipMemcpyHostToDevice));
+      h.kz = kfac.as<float>();
+      h.kx = h.kz + nz;
+    }
+    std::vector<float> a(nt, 0.f);
+    for (int L = 2; L <= nt; ++L) a[L - 1] = dt * wavelet[L - 2];
+    STCHK(amp.alloc(nt * 4, "amplitudes"));
+    HIPCHK(hipMemcpy(amp.p, a.data(), nt * 4, hipMemcpyHostToDevice));
+    h.zs = zs; h.xs = xs; h.src_amp = amp.as<float>();
+    h.zr = zr; h.rx0 = rx0; h.nrx = nrx; h.rec = rec_dev;
+    h.snap_fmt = SNAP_HIST; h.snap_every = 1;
+    STCHK(desc.alloc(sizeof(ShotDev), "descriptor"));
+    HIPCHK(hipMemcpy(desc.p, &h, sizeof(ShotDev), hipMemcpyHostToDevice));
+    const int tx = tile_x(4);
+    const int nbz = (nz + TILE_Z - 1) / TILE_Z, nbx = (nx + tx - 1) / tx;
+    std::vector<TileRef> t;
+    for (int bx = 0; bx < nbx; ++bx)
+      for (int bz = 0; bz < nbz; ++bz) t.push_back({0, bz * TILE_Z, bx * tx, bz + nbz * bx});
+    ntiles = (int)t.size();
+    STCHK(tiles.alloc(t.size() * sizeof(TileRef), "tiles"));
+    HIPCHK(hipMemcpy(tiles.p, t.data(), t.size() * sizeof(TileRef), hipMemcpyHostToDevice));
+    return OK;
+  }
+  float* level(int L) const { return ring.as<float>() + (size_t)(L % 2) * 5 * n2; }
+  int step(int L, int flags, hipStream_t st) {
+    return launch_fwd(desc.as<ShotDev>(), tiles.as<TileRef>(), ntiles, L, flags, 4, SNAP_HIST, 1, st);
+  }
+  Lev lev_view(int L) const { return Lev{level(L), (int)n2, ldz, h.nz, h.nx}; }
+};
+
+RTM_EXPORT int labc_collect(labc_data* d, int nz_phys, int nx_phys, int pad_wide, int abc_wide, int nt,
+                            float dt, int zs_phys, int xs_phys, const float* coef5_wide,
+                            const float* coef5_small, const float* wavelet, const int* levels, int nlev) {
+  if (!d || !coef5_wide || !coef5_small || !wavelet || !levels || nlev < 1) { set_error("bad argument"); return E_ARG; }
+  const int pad_s = R + W;
+  if (pad_wide < pad_s + 5) { set_error("wide padding must exceed the band padding"); return E_ARG; }
+  const int nzw = nz_phys + 2 * pad_wide, nxw = nx_phys + 2 * pad_wide;
+  const int nzs = nz_phys + 2 * pad_s, nxs = nx_phys + 2 * pad_s;
+  const int off = pad_wide - pad_s;
+  const Band bd = band_of(nzs, nxs);
+  if ((long long)d->size + (long long)nlev * bd.total > d->capacity) {
+    set_error("training set capacity %d exceeded", d->capacity);
+    return E_ARG;
+  }
+  STCHK(ensure_device());
+  LGrid gw, gs;
+  STCHK(gw.setup(nzw, nxw, nt, dt, pad_wide + zs_phys, pad_wide + xs_phys, coef5_wide, wavelet, abc_wide, 0, 0, 0,
+                 nullptr));
+  STCHK(gs.setup(nzs, nxs, nt, dt, pad_s + zs_phys, pad_s + xs_phys, coef5_small, wavelet, 0, 0, 0, 0, nullptr));
+  DevBuf scales;
+  STCHK(scales.alloc((size_t)bd.total * 4, "scales"));
+  std::vector<int> lv(levels, levels + nlev);
+  std::sort(lv.begin(), lv.end());
+  size_t next = 0;
+  const dim3 wgrid((nzs + 255) / 256, nxs);
+  for (int L = 2; L <= nt && next < lv.size(); ++L) {
+    const bool take = lv[next] == L;
+    if (take && L >= 3) {
+      // band grid levels L-1, L-2 = windowed wide levels (rim zero)
+      hipLaunchKernelGGL(k_window, wgrid, dim3(256), 0, nullptr, gw.level(L - 1), (int)gw.n2, gw.ldz, off,
+                         gs.level(L - 1), (int)gs.n2, gs.ldz, nzs, nxs);
+      hipLaunchKernelGGL(k_window, wgrid, dim3(256), 0, nullptr, gw.level(L - 2), (int)gw.n2, gw.ldz, off,
+                         gs.level(L - 2), (int)gs.n2, gs.ldz, nzs, nxs);
+      HIPCHK(hipGetLastError());
+      hipLaunchKernelGGL(k_features, dim3((bd.total + 255) / 256), dim3(256), 0, nullptr, gs.lev_view(L - 1),
+                         gs.lev_view(L - 2), bd, d->feat.as<float>() + (size_t)d->size * KP, scales.as<float>());
+      HIPCHK(hipGetLastError());
+    }
+    STCHK(gw.step(L, (L >= 3 ? F_K : 0) | F_SRC, nullptr));
+    if (take) {
+      if (L >= 3) {
+        STCHK(gs.step(L, F_SRC, nullptr));  // plain update of the band grid
+        hipLaunchKernelGGL(k_targets, dim3((bd.total + 255) / 256), dim3(256), 0, nullptr, gw.level(L),
+                           (int)gw.n2, gw.ldz, off, gs.level(L), (int)gs.n2, gs.ldz, bd,
+                           scales.as<float>(), d->tgt.as<float>() + (size_t)d->size * NO);
+        HIPCHK(hipGetLastError());
+        d->size += bd.total;
+      }
+      ++next;
+    }
+  }
+  HIPCHK(hipDeviceSynchronize());
+  return OK;
+}
+
+RTM_EXPORT int labc_train(labc_net* n, labc_data* d, int steps, int batch, float lr, unsigned seed,
+                          float* losses) {
+  if (!n || !d || d->size < 1 || steps < 1 || batch < 1) { set_error("bad argument"); return E_ARG; }
+  DevBuf idx, lossv;
+  STCHK(idx.alloc((size_t)batch * 4, "batch"));
+  STCHK(lossv.alloc((size_t)steps * 4, "losses"));
+  HIPCHK(hipMemset(lossv.p, 0, (size_t)steps * 4));
+  for (int s = 0; s < steps; ++s) {
+    hipLaunchKernelGGL(k_sample, dim3((batch + 255) / 256), dim3(256), 0, nullptr, idx.as<int>(), batch, d->size,
+                       seed + 7919u * (unsigned)s);
+    const int blocks = (batch + SB - 1) / SB;
+    hipLaunchKernelGGL(k_mlp, dim3(blocks), dim3(256), 0, nullptr, n->w.as<float>(), d->feat.as<float>(),
+                       idx.as<int>(), batch, d->tgt.as<float>(), nullptr, n->grad.as<float>(),
+                       lossv.as<float>() + s, 1.f / (batch * NO));
+    n->t += 1;
+    const float b1c = 1.f - std::pow(0.9f, (float)n->t), b2c = 1.f - std::pow(0.999f, (float)n->t);
+    hipLaunchKernelGGL(k_adam, dim3((NP + 255) / 256), dim3(256), 0, nullptr, n->master.as<float>(),
+                       n->m.as<float>(), n->v.as<float>(), n->grad.as<float>(), lr, b1c, b2c);
+    STCHK(refresh_weights(n, nullptr));
+    HIPCHK(hipGetLastError());
+  }
+  HIPCHK(hipDeviceSynchronize());
+  if (losses) HIPCHK(hipMemcpy(losses, lossv.p, (size_t)steps * 4, hipMemcpyDeviceToHost));
+  return OK;
+}
+
+RTM_EXPORT int labc_eval(labc_net* n, labc_data* d, float* loss) {
+  if (!n || !d || !loss || d->size < 1) { set_error("bad argument"); return E_ARG; }
+  HIPCHK(hipMemset(n->loss.p, 0, 4));
+  DevBuf g;
+  STCHK(g.alloc(NP * 4, "scratch gradient"));
+  HIPCHK(hipMemset(g.p, 0, NP * 4));
+  const int blocks = (d->size + SB - 1) / SB;
+  hipLaunchKernelGGL(k_mlp, dim3(blocks), dim3(256), 0, nullptr, n->w.as<float>(), d->feat.as<float>(), nullptr,
+                     d->size, d->tgt.as<float>(), nullptr, g.as<float>(), n->loss.as<float>(),
+                     1.f / ((float)d->size * NO));
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipDeviceSynchronize());
+  HIPCHK(hipMemcpy(loss, n->loss.p, 4, hipMemcpyDeviceToHost));
+  return OK;
+}
+
+RTM_EXPORT int labc_forward(labc_net* n, const rtm_shot_t* s, const float* coef5, const float* wavelet,
+                            float* rec, float* last) {
+  if (!s || !coef5 || !wavelet || (s->nrx > 0 && !rec)) { set_error("bad argument"); return E_ARG; }
+  if (n && labc_cell_count(s->nz, s->nx) == 0) { set_error("grid too small for the band"); return E_ARG; }
+  STCHK(ensure_device());
+  DevBuf recd, delta;
+  const int nr = std::max(s->nrx, 1);
+  STCHK(recd.alloc((size_t)5 * s->nt * nr * 4, "receivers"));
+  STCHK(launch_fill(recd.as<float>(), (size_t)5 * s->nt * nr, 0.f, nullptr));
+  LGrid g;
+  STCHK(g.setup(s->nz, s->nx, s->nt, s->dt, s->zs, s->xs, coef5, wavelet, n ? 0 : s->abc, s->zr, s->rx0, s->nrx,
+                recd.as<float>()));
+  const Band bd = band_of(s->nz, s->nx);
+  if (n) STCHK(delta.alloc((size_t)bd.total * NO * 4, "corrections"));
+  hipEvent_t e0, e1;
+  HIPCHK(hipEventCreate(&e0));
+  HIPCHK(hipEventCreate(&e1));
+  HIPCHK(hipEventRecord(e0, nullptr));
+  for (int L = 2; L <= s->nt; ++L) {
+    int fl = F_SRC | (s->nrx > 0 ? F_REC : 0);
+    if (!n && L >= 3 && s->abc > 0) fl |= F_K;
+    const bool learn = n && L >= 3;
+    if (learn) {
+      hipLaunchKernelGGL(k_infer, dim3((bd.total + 255) / 256), dim3(256), 0, nullptr, n->w.as<float>(),
+                         g.lev_view(L - 1), g.lev_view(L - 2), bd, delta.as<float>());
+      HIPCHK(hipGetLastError());
+    }
+    STCHK(g.step(L, fl, nullptr));
+    if (learn) {
+      hipLaunchKernelGGL(k_apply, dim3((bd.total + 255) / 256), dim3(256), 0, nullptr, g.level(L), (int)g.n2,
+                         g.ldz, bd, delta.as<float>());
+      const int per = 2 * R * (s->nz + s->nx);
+      hipLaunchKernelGGL(k_zero_rim, dim3((per + 255) / 256), dim3(256), 0, nullptr, g.level(L), (int)g.n2, g.ldz,
+                         s->nz, s->nx);
+      HIPCHK(hipGetLastError());
+    }
+  }
+  HIPCHK(hipEventRecord(e1, nullptr));
+  HIPCHK(hipEventSynchronize(e1));
+  float ms = 0;
+  HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  g_last_ms = ms;
+  if (s->nrx > 0) {
+    const int nt = s->nt;
+    std::vector<float> t((size_t)5 * nt * s->nrx);
+    HIPCHK(hipMemcpy(t.data(), recd.p, t.size() * 4, hipMemcpyDeviceToHost));
+    for (int f = 0; f < 5; ++f)
+      for (int ix = 0; ix < s->nrx; ++ix)
+        for (int it = 0; it < nt; ++it)
+          rec[(size_t)f * nt * s->nrx + it + (size_t)nt * ix] = t[((size_t)f * nt + it) * s->nrx + ix];
+  }
+  if (last) {
+    const size_t nn = (size_t)s->nz * s->nx;
+    for (int lv = 0; lv < 2; ++lv)
+      for (int f = 0; f < 5; ++f)
+        HIPCHK(hipMemcpy2D(last + (size_t)(5 * lv + f) * nn, s->nz * 4, g.level(s->nt - 1 + lv) + f * g.n2,
+                           g.ldz * 4, s->nz * 4, s->nx, hipMemcpyDeviceToHost));
+  }
+  return OK;
+}

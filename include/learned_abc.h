@@ -1,0 +1,103 @@
+/*
+ * learned_abc.h -- learned absorbing layer (part of librtm_hip.so).
+ *
+ * No counterpart in the reference: its only boundary treatment is the
+ * multiplicative sponge K (W/abc-mod.f90:59-78, 40 points). The learned layer
+ * replaces it by an 8-cell band inside the 5-cell rim of the computational
+ * grid. For every band cell a small network maps local wavefield values to
+ * an additive correction of the five fields of the new level:
+ *   X(L)[cell] = plain stencil update (no sponge) + s * NN(features / s)
+ *
+ * Features of a band cell (LABC_NFEAT = 242, padded to LABC_KPAD = 248):
+ *   - the 5 fields of level L-1 on a 9 x 5 patch: 9 cells from the cell
+ *     inwards along the edge normal times 5 tangential offsets (-2..2);
+ *     stresses are divided by LABC_Z0 (an impedance scale, Pa / (m/s));
+ *   - the cell's own 5 fields of level L-2 (same scaling);
+ *   - one-hot edge (top, bottom, left, right) and one-hot depth in the band.
+ * Wavefield features are divided by the cell's scale s = max-abs over the
+ * scaled patch values, so the correction is equivariant to the wavefield
+ * amplitude. Network: 242 -> 32 -> 32 -> 5, exact GELU, BF16 weights and
+ * BF16-rounded features with FP32 accumulation (8965 parameters).
+ * Band cells: top rows [R, R+W) and bottom rows [nz-R-W, nz-R) over columns
+ * [R, nx-R) (corners belong to them), left columns [R, R+W) and right
+ * columns [nx-R-W, nx-R) over rows [R+W, nz-R-W); R = 5, W = 8.
+ */
+#ifndef LEARNED_ABC_H
+#define LEARNED_ABC_H
+
+#include "rtmlib_modC.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define LABC_RIM 5
+#define LABC_WIDTH 8
+#define LABC_NFEAT 242
+#define LABC_KPAD 248
+#define LABC_HID 32
+#define LABC_NOUT 5
+#define LABC_NPARAM (LABC_NFEAT * LABC_HID + LABC_HID + LABC_HID * LABC_HID + LABC_HID + \
+                     LABC_HID * LABC_NOUT + LABC_NOUT)
+#define LABC_Z0 6.0e6f
+
+typedef struct labc_net labc_net;
+
+/* Parameters in order W1 (242x32, row-major [k][j]), b1 (32), W2 (32x32),
+ * b2 (32), W3 (32x5), b3 (5). */
+int labc_net_create(labc_net **net, unsigned seed);
+int labc_net_destroy(labc_net *net);
+int labc_net_get_params(labc_net *net, float *params);
+int labc_net_set_params(labc_net *net, const float *params);
+/* Number of band cells of an nz x nx grid. */
+int labc_cell_count(int nz, int nx);
+/* Band cell c -> (iz, ix, edge 0..3, depth 0..7). */
+int labc_cell_coords(int nz, int nx, int c, int *iz, int *ix, int *edge, int *depth);
+/* Host-side evaluation on a feature matrix feat [n][LABC_KPAD] (unscaled by
+ * s, i.e. the normalised features): out [n][5]. mode 0: the VALU training
+ * forward, 1: the MFMA inference path. */
+int labc_net_forward(labc_net *net, const float *feat, int n, float *out, int mode);
+/* Loss 0.5*mean((out - target)^2) over n*5 values and its gradient [LABC_NPARAM]. */
+int labc_net_grad(labc_net *net, const float *feat, const float *target, int n, float *grad,
+                  float *loss);
+/* Features of the band cells of one grid from two levels (host arrays, each
+ * 5 fields (nz,nx) back to back): feat [ncell][LABC_KPAD], scale [ncell]. */
+int labc_features(int nz, int nx, const float *lev_m1, const float *lev_m2, float *feat,
+                  float *scale);
+
+/* Training set on the device. */
+typedef struct labc_data labc_data;
+int labc_data_create(labc_data **d, int capacity);
+int labc_data_destroy(labc_data *d);
+int labc_data_size(labc_data *d);
+/* Copy rows [row0, row0+n) of the set to the host: feat [n][LABC_KPAD],
+ * tgt [n][5] (either may be NULL). */
+int labc_data_get(labc_data *d, int row0, int n, float *feat, float *tgt);
+/* Teacher-forced pairs from one wide-sponge run: the wide grid (pad_wide
+ * cells of padding, sponge abc_wide) and the band grid (pad = R + W) of the
+ * same physical model share the physical region. At each listed level L the
+ * band grid's levels L-1, L-2 are the windowed wide levels (rim zeroed); the
+ * target is wide(L) - plain band-grid update(L) on the band cells, divided by
+ * the cell scale. coef5_wide / coef5_small: dt/dx-scaled coefficients of the
+ * two grids; wavelet[it-1] = source(it). */
+int labc_collect(labc_data *d, int nz_phys, int nx_phys, int pad_wide, int abc_wide, int nt,
+                 float dt, int zs_phys, int xs_phys, const float *coef5_wide,
+                 const float *coef5_small, const float *wavelet, const int *levels, int nlev);
+/* Adam on random minibatches: loss per step in losses[steps]. */
+int labc_train(labc_net *net, labc_data *d, int steps, int batch, float lr, unsigned seed,
+               float *losses);
+/* Mean loss over the whole data set. */
+int labc_eval(labc_net *net, labc_data *d, float *loss);
+
+/* One forward propagation of a band grid with the learned layer (no sponge)
+ * or, with net = NULL, with the sponge abc: records receivers like
+ * rtm_forward_hip and returns the last two levels. */
+int labc_forward(labc_net *net, const rtm_shot_t *s, const float *coef5, const float *wavelet,
+                 float *rec, float *last);
+/* Device time of the last labc_forward time loop (ms). */
+double labc_last_forward_ms(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* LEARNED_ABC_H */

@@ -204,6 +204,19 @@ def forward(p: Params, wavelet, coefs, dtype=np.float64, want_hist=False, want_h
     return dict(rec=rec, hist_vx=hvx, hist_vz=hvz, hist5=h5 if want_hist5 else None)
 
 
+def fwd_level(p: Params, cur, prev, coefs, apply_k=False, srcamp=None, dtype=np.float64):
+    """One forward level X(L) = K(X(L-2) [+ src] + Op X(L-1)) (W/rtm-mod-adj.f90:225-252).
+    cur: five (nz,nx) arrays of level L-2; prev: level L-1. Returns level L
+    (new arrays). srcamp = dt*source(L-1) or None."""
+    out = [np.array(F(a, dtype), order="F", copy=True) for a in cur]
+    pv = [F(a, dtype) for a in prev]
+    c = [F(a, dtype) for a in coefs]
+    getattr(lib(), "orc_fwd_level_" + _suffix(dtype))(
+        C.byref(p), *[_p(a) for a in out], *[_p(a) for a in pv], *[_p(a) for a in c],
+        C.c_int(int(apply_k)), C.c_int(srcamp is not None), _scal(dtype)(0.0 if srcamp is None else srcamp))
+    return out
+
+
 def image_S(vx, vz, vx1, vz1, dx, dt, dtype=np.float64):
     """S1..S6 at one level (W/image-mod-rtm.f90:26-67). vx1/vz1 may be None (last level)."""
     vx, vz = F(vx, dtype), F(vz, dtype)

@@ -1,0 +1,225 @@
+"""Learned absorbing layer oracle -- TEST INFRASTRUCTURE ONLY.
+
+Only ``tests/`` import this module. It restates, in numpy, what the HIP
+learned-layer kernels compute (elastic_amd/csrc/learned_abc.hip), so that the
+GPU results can be checked:
+
+* band geometry: the 8-cell band inside the 5-cell rim of a grid, cell order
+  top rows, bottom rows (both over all inner columns), left columns, right
+  columns (over the remaining rows);
+* features of a band cell from levels L-1 and L-2 (a 9 x 5 patch of the five
+  fields of L-1 along the inward normal, the cell's own fields of L-2,
+  one-hot edge and depth), divided by the patch max-abs scale s and rounded
+  to bf16; ``dtype=np.float32`` reproduces the kernel's float32 operation
+  order bit for bit, ``np.float64`` is the exact definition;
+* the network 242 -> 32 -> 32 -> 5 with exact GELU on bf16-rounded weights,
+  evaluated in FP64, with the loss 0.5 * mean((y - t)^2) and its gradient
+  (hand-written backward pass);
+* teacher-forced training pairs: the band grid's levels L-1, L-2 are windows
+  of a wide-sponge run, the target is (wide(L) - plain band-grid update(L)) / s
+  per field unit, using the 2D oracle's forward and single-level update.
+
+The reference has no learned boundary: it only has the sponge of
+W/abc-mod.f90:59-78. The network is this project's addition, so there is no
+reference vector for it; the pins are torch's float64 autograd (an
+independent implementation of the same network), central finite differences
+of the loss, and a brute-force construction of the band from its distance
+definition (tests/test_learned_abc_pins.py).
+"""
+from __future__ import annotations
+
+import numpy as np
+from scipy.special import erf
+
+from . import forward, fwd_level, make_params
+
+R, W, NF, KP, H, NO = 5, 8, 242, 248, 32, 5
+NRAW = 230
+Z0 = 6.0e6
+OW1 = 0
+OB1 = NF * H
+OW2 = OB1 + H
+OB2 = OW2 + H * H
+OW3 = OB2 + H
+OB3 = OW3 + H * NO
+NPARAM = OB3 + NO
+
+
+def band_cells(nz, nx):
+    """(iz, ix, edge, depth) of every band cell, in kernel cell order."""
+    cols = np.arange(R, nx - R)
+    rows = np.arange(R + W, nz - R - W)
+    out = []
+    for e in range(4):
+        for d in range(W):
+            if e == 0:
+                out.append((np.full(cols.size, R + d), cols))
+            elif e == 1:
+                out.append((np.full(cols.size, nz - R - 1 - d), cols))
+            elif e == 2:
+                out.append((rows, np.full(rows.size, R + d)))
+            else:
+                out.append((rows, np.full(rows.size, nx - R - 1 - d)))
+    iz = np.concatenate([a for a, _ in out])
+    ix = np.concatenate([b for _, b in out])
+    per = [cols.size] * (2 * W) + [rows.size] * (2 * W)
+    edge = np.concatenate([np.full(n, e) for (e, n) in zip(np.repeat(np.arange(4), W), per)])
+    depth = np.concatenate([np.full(n, d) for (d, n) in zip(np.tile(np.arange(W), 4), per)])
+    return iz.astype(np.int64), ix.astype(np.int64), edge.astype(np.int64), depth.astype(np.int64)
+
+
+def bf16(x):
+    """Round float32 values to the nearest bf16 (ties to even), as float32."""
+    a = np.ascontiguousarray(x, dtype=np.float32)
+    u = a.view(np.uint32).astype(np.uint64)
+    u = (u + 0x7FFF + ((u >> 16) & 1)) & 0xFFFF0000
+    return u.astype(np.uint32).view(np.float32).reshape(a.shape)
+
+
+def _unit(f, dtype):
+    if f < 2:
+        return None
+    return dtype(1.0) / dtype(Z0) if dtype == np.float32 else 1.0 / Z0
+
+
+def raw_patch(lev_m1, lev_m2, cells, dtype=np.float64):
+    """Raw features 0..229 of the band cells: (ncell, 230) in dtype.
+    lev_m1 / lev_m2: five (nz, nx) arrays each."""
+    iz, ix, edge, _ = cells
+    nz, nx = lev_m1[0].shape
+    dz = np.select([edge == 0, edge == 1], [1, -1], 0)
+    dx = np.select([edge == 2, edge == 3], [1, -1], 0)
+    tz = (edge >= 2).astype(np.int64)
+    tx = (edge < 2).astype(np.int64)
+    out = np.zeros((iz.size, NRAW), dtype)
+    for kn in range(9):
+        for m in range(-2, 3):
+            z = iz + kn * dz + m * tz
+            x = ix + kn * dx + m * tx
+            ok = (z >= 0) & (z < nz) & (x >= 0) & (x < nx)
+            zc, xc = np.clip(z, 0, nz - 1), np.clip(x, 0, nx - 1)
+            for f in range(5):
+                v = np.where(ok, np.asarray(lev_m1[f], dtype)[zc, xc], dtype(0))
+                u = _unit(f, dtype)
+                if u is not None:
+                    v = (v * u).astype(dtype)
+                out[:, kn * 25 + (m + 2) * 5 + f] = v
+    for f in range(5):
+        v = np.asarray(lev_m2[f], dtype)[iz, ix]
+        u = _unit(f, dtype)
+        out[:, 225 + f] = v if u is None else (v * u).astype(dtype)
+    return out
+
+
+def features(lev_m1, lev_m2, dtype=np.float64, cells=None):
+    """Normalised, bf16-rounded features (ncell, KP) and scales (ncell,)."""
+    if cells is None:
+        cells = band_cells(*lev_m1[0].shape)
+    raw = raw_patch(lev_m1, lev_m2, cells, dtype)
+    s = np.abs(raw).max(axis=1)
+    live = s > 1e-30
+    inv = np.zeros_like(s)
+    inv[live] = dtype(1.0) / s[live]
+    x = np.zeros((raw.shape[0], KP), dtype)
+    x[:, :NRAW] = (raw * inv[:, None]).astype(dtype)
+    _, _, edge, depth = cells
+    rows = np.arange(raw.shape[0])
+    x[rows, NRAW + edge] = 1.0
+    x[rows, NRAW + 4 + depth] = 1.0
+    feat = bf16(x.astype(np.float32))
+    return feat, np.where(live, s, 0).astype(dtype)
+
+
+def unpack(params):
+    p = bf16(np.asarray(params, np.float32)).astype(np.float64)
+    return (p[OW1:OB1].reshape(NF, H), p[OB1:OW2], p[OW2:OB2].reshape(H, H), p[OB2:OW3],
+            p[OW3:OB3].reshape(H, NO), p[OB3:NPARAM])
+
+
+def gelu(x):
+    return 0.5 * x * (1.0 + erf(x / np.sqrt(2.0)))
+
+
+def gelu_d(x):
+    return 0.5 * (1.0 + erf(x / np.sqrt(2.0))) + x * np.exp(-0.5 * x * x) / np.sqrt(2.0 * np.pi)
+
+
+def mlp(params, feat):
+    """FP64 network output (n, 5) on bf16-rounded weights; feat (n, KP)."""
+    W1, b1, W2, b2, W3, b3 = unpack(params)
+    x = np.asarray(feat, np.float64)[:, :NF]
+    h1 = gelu(x @ W1 + b1)
+    h2 = gelu(h1 @ W2 + b2)
+    return h2 @ W3 + b3
+
+
+def loss_grad(params, feat, tgt):
+    """0.5 * mean((y - t)^2) over n*5 values and its gradient w.r.t. the
+    (rounded) weights, in the header's parameter order."""
+    W1, b1, W2, b2, W3, b3 = unpack(params)
+    x = np.asarray(feat, np.float64)[:, :NF]
+    t = np.asarray(tgt, np.float64)
+    z1 = x @ W1 + b1
+    h1 = gelu(z1)
+    z2 = h1 @ W2 + b2
+    h2 = gelu(z2)
+    y = h2 @ W3 + b3
+    cnt = y.size
+    d = (y - t) / cnt
+    loss = 0.5 * np.sum((y - t) ** 2) / cnt
+    g = np.zeros(NPARAM)
+    g[OW3:OB3] = (h2.T @ d).ravel()
+    g[OB3:NPARAM] = d.sum(0)
+    dz2 = (d @ W3.T) * gelu_d(z2)
+    g[OW2:OB2] = (h1.T @ dz2).ravel()
+    g[OB2:OW3] = dz2.sum(0)
+    dz1 = (dz2 @ W2.T) * gelu_d(z1)
+    g[OW1:OB1] = (x.T @ dz1).ravel()
+    g[OB1:OW2] = dz1.sum(0)
+    return loss, g
+
+
+def window(lev, off, nz, nx):
+    """Band-grid window of a wide level with the 5-cell rim zeroed."""
+    out = []
+    for a in lev:
+        w = np.array(a[off:off + nz, off:off + nx], dtype=np.float64)
+        w[:R, :] = 0
+        w[nz - R:, :] = 0
+        w[:, :R] = 0
+        w[:, nx - R:] = 0
+        out.append(w)
+    return out
+
+
+def teacher_pairs(nz_phys, nx_phys, pad_wide, abc_wide, nt, dt, zs_phys, xs_phys, coefs_wide,
+                  coefs_small, wavelet, levels):
+    """FP64 (features, targets) of the band cells at each level in levels
+    (features of levels L-1, L-2 windowed from the wide run), stacked in
+    level order like labc_collect."""
+    pad_s = R + W
+    nzw, nxw = nz_phys + 2 * pad_wide, nx_phys + 2 * pad_wide
+    nzs, nxs = nz_phys + 2 * pad_s, nx_phys + 2 * pad_s
+    off = pad_wide - pad_s
+    pw = make_params(nzw, nxw, nt, 1.0, dt, pad_wide + xs_phys, pad_wide + zs_phys, 0, 0, 0, abc_wide)
+    hist = forward(pw, wavelet, coefs_wide, want_hist5=True)["hist5"]
+    ps = make_params(nzs, nxs, nt, 1.0, dt, pad_s + xs_phys, pad_s + zs_phys, 0, 0, 0, 0)
+    cells = band_cells(nzs, nxs)
+    iz, ix = cells[0], cells[1]
+    feats, tgts = [], []
+    for L in sorted(levels):
+        if L < 3:
+            continue
+        m1 = window([h[:, :, L - 2] for h in hist], off, nzs, nxs)
+        m2 = window([h[:, :, L - 3] for h in hist], off, nzs, nxs)
+        f, s = features(m1, m2, np.float64, cells)
+        plain = fwd_level(ps, m2, m1, coefs_small, apply_k=False, srcamp=dt * wavelet[L - 2])
+        t = np.zeros((iz.size, NO))
+        for fi in range(5):
+            wide = hist[fi][iz + off, ix + off, L - 1]
+            u = Z0 if fi >= 2 else 1.0
+            with np.errstate(divide="ignore", invalid="ignore"):
+                t[:, fi] = np.where(s > 0, (wide - plain[fi][iz, ix]) / (s * u), 0.0)
+        feats.append(f)
+        tgts.append(t)
+    return np.concatenate(feats), np.concatenate(tgts)

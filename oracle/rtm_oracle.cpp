@@ -761,6 +761,18 @@ static void born(const Params& p, const T* wavelet, const T* m1_x, const T* m1_z
     T* h5[5] = {h0, h1, h2, h3, h4};                                                               \
     orc::forward<T>(*p, wav, m1x, m1z, m2, m3, m2m3, rec, hvx, hvz, h0 ? h5 : nullptr);            \
   }                                                                                                \
+  ORC_EXPORT void orc_fwd_level_##SFX(const orc::Params* p, T* c0, T* c1, T* c2, T* c3, T* c4,     \
+                                      const T* p0, const T* p1, const T* p2, const T* p3,          \
+                                      const T* p4, const T* m1x, const T* m1z, const T* m2,        \
+                                      const T* m3, const T* m2m3, int apply_k, int add_src,        \
+                                      T srcamp) {                                                  \
+    T* cur[5] = {c0, c1, c2, c3, c4};                                                              \
+    const T* prev[5] = {p0, p1, p2, p3, p4};                                                       \
+    std::vector<T> filt(std::max(p->abc + 5, 1));                                                  \
+    orc::abc_filter<T>(p->abc, filt.data());                                                       \
+    orc::fwd_level<T>(*p, cur, prev, m1x, m1z, m2, m3, m2m3, filt.data(), apply_k != 0,           \
+                      add_src != 0, srcamp, nullptr);                                              \
+  }                                                                                                \
   ORC_EXPORT void orc_image_S_##SFX(int nz, int nx, T dx, T dt, const T* vx, const T* vz,          \
                                     const T* vx1, const T* vz1, T* S1, T* S2, T* S3, T* S4, T* S5, \
                                     T* S6) {                                                       \

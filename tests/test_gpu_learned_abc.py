@@ -1,0 +1,135 @@
+"""GPU tests of the learned absorbing layer (elastic_amd/csrc/learned_abc.hip)
+against the numpy oracle (oracle/learned_abc.py): band geometry, features,
+network forward and gradient, teacher-forced pairs from a wide-sponge run,
+training, and propagation with the layer."""
+import numpy as np
+import pytest
+
+import oracle as O
+from oracle import learned_abc as LA
+from elastic_amd import synth
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def L(hip):
+    from elastic_amd import learned
+
+    return learned
+
+
+def test_cell_geometry(L):
+    for nz, nx in ((40, 57), (64, 200)):
+        iz, ix, edge, depth = LA.band_cells(nz, nx)
+        assert L.cell_count(nz, nx) == iz.size
+        for c in list(range(0, iz.size, 7)) + [iz.size - 1]:
+            assert L.cell_coords(nz, nx, c) == (iz[c], ix[c], edge[c], depth[c])
+    assert L.cell_count(20, 20) == 0  # too small for a band on both sides
+
+
+def test_features_bit_exact(L):
+    rng = np.random.default_rng(5)
+    nz, nx = 48, 70
+    m1 = [(rng.standard_normal((nz, nx)) * (1e-3 if f < 2 else 1e4)).astype(np.float32) for f in range(5)]
+    m2 = [(rng.standard_normal((nz, nx)) * (1e-3 if f < 2 else 1e4)).astype(np.float32) for f in range(5)]
+    m1[0][:, 30:] = 0  # some cells with an all-zero patch of Vx
+    f, s = L.features(m1, m2)
+    fo, so = LA.features(m1, m2, np.float32)
+    assert np.array_equal(s, so.astype(np.float32))
+    assert np.array_equal(f, fo)
+
+
+def _params(seed, scale=0.2):
+    return (np.random.default_rng(seed).standard_normal(LA.NPARAM) * scale).astype(np.float32)
+
+
+def _feat(n, seed):
+    rng = np.random.default_rng(seed)
+    f = np.zeros((n, LA.KP), np.float32)
+    f[:, :LA.NRAW] = rng.uniform(-1, 1, (n, LA.NRAW))
+    f[np.arange(n), LA.NRAW + rng.integers(0, 4, n)] = 1
+    f[np.arange(n), LA.NRAW + 4 + rng.integers(0, 8, n)] = 1
+    return LA.bf16(f)
+
+
+def test_forward_and_gradient_vs_oracle(L):
+    net = L.Net(seed=3)
+    p = _params(1)
+    net.params = p
+    assert np.array_equal(net.params, p)
+    for n in (1, 37, 1000):  # partial blocks of 32 samples
+        f = _feat(n, n)
+        y = net.forward(f)
+        yo = LA.mlp(p, f)
+        assert O.rel_l2(y, yo) <= 1e-5, n
+        t = np.random.default_rng(n + 1).standard_normal((n, 5)).astype(np.float32)
+        loss, g = net.grad(f, t)
+        lo, go = LA.loss_grad(p, f, t)
+        assert abs(loss - lo) <= 1e-5 * abs(lo), n
+        assert O.rel_l2(g, go) <= 1e-4, n
+        for a, b in ((LA.OW1, LA.OB1), (LA.OW2, LA.OB2), (LA.OW3, LA.OB3), (LA.OB3, LA.NPARAM)):
+            assert O.rel_l2(g[a:b], go[a:b]) <= 1e-4, (n, a)
+    net.close()
+
+
+def _case(nt=160):
+    nzp, nxp, dx, dt, fp = 30, 40, 10.0, 1e-3, 30.0
+    abc_w = 20
+    pad_w = abc_w + 5 + LA.R + LA.W
+    vp = synth.layered_vp(nzp, nxp, 4, vp_top=2000.0, vp_bot=2600.0, nlayers=2, jump=100.0, anom_sigma=4.0)
+    mw = synth.elastic_from_vp(synth.pad_edge(vp, pad_w))
+    ms = synth.elastic_from_vp(synth.pad_edge(vp, LA.R + LA.W))
+    cw = O.model_op(mw.rho, mw.lam, mw.mu, dx, dt, np.float32)
+    cs = O.model_op(ms.rho, ms.lam, ms.mu, dx, dt, np.float32)
+    wav = O.ricker(nt, dt, fp, nsource=60, dtype=np.float32)
+    return dict(nz_phys=nzp, nx_phys=nxp, pad_wide=pad_w, abc_wide=abc_w, nt=nt, dt=dt, zs_phys=14, xs_phys=9,
+                coefs_wide=cw, coefs_small=cs, wavelet=wav)
+
+
+def test_collect_vs_oracle(L):
+    case = _case()
+    levels = [2, 70, 100, 131, 160]
+    d = L.Data(100000)
+    d.collect(levels=levels, **case)
+    fo, to = LA.teacher_pairs(levels=levels, **case)
+    f, t = d.get()
+    assert f.shape == fo.shape  # level 2 has no pair
+    # features: FP32 run vs FP64 run, then bf16 rounding: within one bf16 step
+    # (plus a floor for values near cancellation)
+    assert np.all(np.abs(f - fo) <= 2.0 ** -7 * np.abs(fo) + 1e-4)
+    assert np.abs(f - fo).mean() < 1e-4
+    # targets: FP32 differences of nearly equal levels; normalised units
+    assert np.abs(t - to).max() <= 1e-4 * max(1.0, np.abs(to).max())
+    assert np.abs(to).max() > 1e-3  # the wave reached the rim
+    d.close()
+
+
+def test_training_reduces_loss(L):
+    case = _case(nt=200)
+    d = L.Data(200000)
+    d.collect(levels=list(range(40, 201, 4)), **case)
+    net = L.Net(seed=7)
+    l0 = net.eval(d)
+    losses = net.train(d, steps=300, batch=2048, lr=2e-3, seed=1)
+    l1 = net.eval(d)
+    assert np.all(np.isfinite(losses))
+    assert l1 < 0.2 * l0, (l0, l1)
+    # the full-set loss agrees with the oracle on the trained weights
+    f, t = d.get()
+    lo, _ = LA.loss_grad(net.params, f, t)
+    assert abs(l1 - lo) <= 1e-3 * lo
+    net.close()
+    d.close()
+
+
+def test_propagation_without_net_equals_forward(L, hip):
+    """labc_forward with no network is the plain sponge forward."""
+    case = _case(nt=150)
+    nz, nx = case["nz_phys"] + 2 * 13, case["nx_phys"] + 2 * 13
+    s = hip.shot(nz, nx, 150, 10.0, case["dt"], 13 + case["xs_phys"], 13 + case["zs_phys"], 15, 13, 40, abc=8)
+    rec, last, _ = L.propagate(None, s, case["coefs_small"], case["wavelet"], want_last=True)
+    rec2, last2 = hip.forward(s, case["coefs_small"], case["wavelet"], want_last=True)
+    assert np.array_equal(rec, rec2)
+    for a, b in zip(last, last2):
+        assert np.array_equal(a, b)
