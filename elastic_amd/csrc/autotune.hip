@@ -1,0 +1,121 @@
+// autotune.hip -- tile-width autotuner of the 2D step kernels.
+//
+// The step kernels are compiled for four tile widths (4*TXR x-cells by 64
+// z-cells, TXR in {1,2,4,8}). Which one is fastest depends on the grid sizes,
+// the snapshot format and the imaging decimation: wider tiles re-read less
+// halo but hold more registers per thread. The tuner runs the real engine on
+// a short copy of the workload (the first shots of the batch, a few dozen
+// levels, the same options) once per width and keeps the fastest. Results
+// are cached for the process, keyed by the workload signature, and, when
+// RTM_TUNE_CACHE names a file, in that file (one "key txr" line per entry).
+#include "engine2d.hpp"
+
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <string>
+
+namespace erm {
+
+static std::mutex g_tune_mu;
+static std::map<std::string, int> g_tune_cache;
+
+static std::string tune_key(const EngineCfg& c, const std::vector<ShotSpec>& sp) {
+  long long cells = 0;
+  int nzmax = 0, nxmax = 0;
+  for (const ShotSpec& s : sp) {
+    cells += (long long)s.nz * s.nx;
+    nzmax = std::max(nzmax, s.nz);
+    nxmax = std::max(nxmax, s.nx);
+  }
+  hipDeviceProp_t prop{};
+  int dev = 0;
+  (void)hipGetDevice(&dev);
+  (void)hipGetDeviceProperties(&prop, dev);
+  char buf[256];
+  // shapes bucketed to 64 cells, mean cells per shot to 2^16
+  std::snprintf(buf, sizeof buf, "%s/nz%d/nx%d/c%lld/n%d/f%d/j%d/k%d/a%d/g%d", prop.gcnArchName, (nzmax + 63) / 64,
+                (nxmax + 63) / 64, (cells / (long long)sp.size()) >> 16, (int)std::min<size_t>(sp.size(), 8),
+                c.snap_fmt, c.snap_every, c.comp, c.adj_abc, c.use_graph);
+  for (char* p = buf; *p; ++p)
+    if (*p == ' ' || *p == ':') *p = '_';
+  return buf;
+}
+
+static void load_file_cache() {
+  static bool done = false;
+  if (done) return;
+  done = true;
+  const char* path = getenv("RTM_TUNE_CACHE");
+  if (!path) return;
+  FILE* f = std::fopen(path, "r");
+  if (!f) return;
+  char key[512];
+  int txr = 0;
+  while (std::fscanf(f, "%511s %d", key, &txr) == 2)
+    if (txr == 1 || txr == 2 || txr == 4 || txr == 8) g_tune_cache[key] = txr;
+  std::fclose(f);
+}
+
+static void append_file_cache(const std::string& key, int txr) {
+  const char* path = getenv("RTM_TUNE_CACHE");
+  if (!path) return;
+  FILE* f = std::fopen(path, "a");
+  if (!f) return;
+  std::fprintf(f, "%s %d\n", key.c_str(), txr);
+  std::fclose(f);
+}
+
+int autotune_txr(const EngineCfg& cfg, const std::vector<ShotSpec>& shots, const std::vector<const float*>& coef5,
+                 const float* wavelet, int* best, double ms_out[4], bool use_cache) {
+  if (shots.empty() || coef5.size() != shots.size() || !wavelet || !best) {
+    set_error("autotune: bad arguments");
+    return E_ARG;
+  }
+  const std::string key = tune_key(cfg, shots);
+  {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    load_file_cache();
+    auto it = g_tune_cache.find(key);
+    if (use_cache && it != g_tune_cache.end()) {
+      *best = it->second;
+      if (ms_out)
+        for (int i = 0; i < 4; ++i) ms_out[i] = -1.0;
+      return OK;
+    }
+  }
+  // A short copy of the workload: up to 8 shots, enough levels for several
+  // snapshot periods, the same options; one group, no stack.
+  const int ns = (int)std::min<size_t>(shots.size(), 8);
+  std::vector<ShotSpec> sp(shots.begin(), shots.begin() + ns);
+  std::vector<const float*> cf(coef5.begin(), coef5.begin() + ns);
+  for (ShotSpec& s : sp) s.img_nz = s.img_nx = 0;
+  EngineCfg c = cfg;
+  c.nt = std::max(48, 8 * c.snap_every);
+  c.nt = std::min(c.nt, cfg.nt);
+  c.stack_nz = c.stack_nx = 0;
+  c.max_group = 0;
+  const int cand[4] = {1, 2, 4, 8};
+  double t[4];
+  int arg = -1;
+  for (int i = 0; i < 4; ++i) {
+    c.txr = cand[i];
+    Engine e;
+    STCHK(e.setup(c, sp, cf, wavelet));
+    STCHK(e.migrate());  // warm-up (and graph instantiation)
+    STCHK(e.migrate());
+    t[i] = e.timing().total_ms;
+    if (arg < 0 || t[i] < t[arg]) arg = i;
+  }
+  *best = cand[arg];
+  if (ms_out)
+    for (int i = 0; i < 4; ++i) ms_out[i] = t[i];
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  g_tune_cache[key] = *best;
+  append_file_cache(key, *best);
+  return OK;
+}
+
+}  // namespace erm

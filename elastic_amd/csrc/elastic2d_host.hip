@@ -290,7 +290,7 @@ struct rtm_ctx {
 
 struct rtm_batch {
   Engine eng;
-  int nshots = 0, nt = 0;
+  int nshots = 0, nt = 0, txr = 0;
   std::vector<int> nz, nx, nrx;
 };
 
@@ -340,6 +340,36 @@ RTM_EXPORT int rtm_forward_hip(const rtm_shot_t* s, const float* coef5, const fl
   if (s->nrx > 0) STCHK(e.get_data(0, rec));
   if (last) STCHK(e.get_last_levels(0, last));
   if (s->nrx > 0) STCHK(host_finite(rec, (size_t)5 * s->nt * s->nrx, "receiver data"));
+  return OK;
+}
+
+RTM_EXPORT int rtm_forward_movie_hip(const rtm_shot_t* s, const float* coef5, const float* wavelet, int nsnap,
+                                     int z0, int x0, int wnz, int wnx, float* movie, int* dsnap) {
+  if (!s || !coef5 || !wavelet || !movie || nsnap < 1 || wnz < 1 || wnx < 1) { set_error("bad argument"); return E_ARG; }
+  const int every = s->nt / nsnap;
+  if (every < 1) { set_error("movie: %d frames exceed nt = %d", nsnap, s->nt); return E_ARG; }
+  STCHK(ensure_device());
+  const size_t frame = (size_t)5 * wnz * wnx;
+  DevBuf m;
+  STCHK(m.alloc(frame * nsnap * sizeof(float), "movie frames"));
+  STCHK(launch_fill(m.as<float>(), frame * nsnap, 0.f, nullptr));
+  HIPCHK(hipDeviceSynchronize());
+  EngineCfg cfg;
+  cfg.nt = s->nt; cfg.dx = s->dx; cfg.dt = s->dt; cfg.abc = s->abc;
+  cfg.txr = default_txr();
+  cfg.xcd = default_xcd();
+  cfg.movie_every = every;
+  cfg.movie_n = nsnap;
+  cfg.movie_z0 = z0; cfg.movie_x0 = x0; cfg.movie_nz = wnz; cfg.movie_nx = wnx;
+  cfg.movie = m.as<float>();
+  ShotSpec p{};
+  p.nz = s->nz; p.nx = s->nx; p.xs = s->xs; p.zs = s->zs; p.zr = s->zr; p.rx0 = s->rx0; p.nrx = s->nrx;
+  Engine e;
+  STCHK(e.setup(cfg, {p}, {coef5}, wavelet));
+  STCHK(e.model_data());
+  store_timing(e.timing());
+  HIPCHK(hipMemcpy(movie, m.p, frame * nsnap * sizeof(float), hipMemcpyDeviceToHost));
+  if (dsnap) *dsnap = every;
   return OK;
 }
 
@@ -543,6 +573,7 @@ RTM_EXPORT int rtm_batch_create(rtm_batch** out, const rtm_batch_cfg_t* cfg, int
   ec.snap_fmt = snap_fmt_of(cfg->snap_prec, ec.snap_every);
   ec.comp = cfg->comp_image;
   ec.txr = cfg->txr > 0 ? cfg->txr : default_txr();
+  const bool tune = cfg->txr == 0 && !getenv("RTM_TXR");
   ec.xcd = default_xcd();
   ec.use_graph = cfg->use_graph;
   ec.max_group = cfg->max_group;
@@ -562,10 +593,33 @@ RTM_EXPORT int rtm_batch_create(rtm_batch** out, const rtm_batch_cfg_t* cfg, int
   }
   b->nshots = nshots;
   b->nt = cfg->nt;
+  if (tune) STCHK(autotune_txr(ec, sp, cf, wavelet, &ec.txr, nullptr, true));
+  b->txr = ec.txr;
   STCHK(b->eng.setup(ec, sp, cf, wavelet));
   *out = b.release();
   return OK;
 }
+
+RTM_EXPORT int rtm_batch_autotune(const rtm_batch_cfg_t* cfg, int nshots, const rtm_batch_shot_t* shots,
+                                  const float* const* coef5, const float* wavelet, int* txr, double* ms4) {
+  if (!cfg || !shots || !coef5 || !wavelet || !txr || nshots < 1) { set_error("bad argument"); return E_ARG; }
+  EngineCfg ec;
+  ec.nt = cfg->nt; ec.dx = cfg->dx; ec.dt = cfg->dt; ec.abc = cfg->abc; ec.adj_abc = cfg->adj_abc;
+  ec.snap_every = cfg->snap_every < 1 ? 1 : cfg->snap_every;
+  ec.snap_fmt = snap_fmt_of(cfg->snap_prec, ec.snap_every);
+  ec.comp = cfg->comp_image;
+  ec.xcd = default_xcd();
+  ec.use_graph = cfg->use_graph;
+  std::vector<ShotSpec> sp(nshots);
+  std::vector<const float*> cf(coef5, coef5 + nshots);
+  for (int i = 0; i < nshots; ++i) {
+    const rtm_batch_shot_t& s = shots[i];
+    sp[i] = ShotSpec{s.nz, s.nx, s.xs, s.zs, s.zr, s.rx0, s.nrx, 0, 0, 0, 0, 0, 0};
+  }
+  return autotune_txr(ec, sp, cf, wavelet, txr, ms4, false);
+}
+
+RTM_EXPORT int rtm_batch_txr(rtm_batch* b) { return b ? b->txr : 0; }
 
 RTM_EXPORT int rtm_batch_set_true_model(rtm_batch* b, const float* const* coef5_true) {
   if (!b || !coef5_true) { set_error("bad argument"); return E_ARG; }

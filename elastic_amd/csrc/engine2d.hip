@@ -25,8 +25,9 @@ int ensure_device() {
 }
 
 // Tile width (x-cells per thread) of the step kernels and XCD-aware tile
-// numbering; RTM_TXR / RTM_XCD_REMAP override them (autotuner, experiments).
-int default_txr() { return env_int("RTM_TXR", 4); }
+// numbering; RTM_TXR / RTM_XCD_REMAP override them (experiments). The batch
+// API tunes the width per workload unless a width is given.
+int default_txr() { return env_int("RTM_TXR", 2); }
 int default_xcd() { return env_int("RTM_XCD_REMAP", 1); }
 // Non-temporal stores of the new level (RTM_NT=1).
 static int nt_stores() { return env_int("RTM_NT", 0); }
@@ -115,6 +116,15 @@ int Engine::setup(const EngineCfg& cfg, const std::vector<ShotSpec>& shots,
   if (cfg_.txr != 1 && cfg_.txr != 2 && cfg_.txr != 4 && cfg_.txr != 8) {
     set_error("tile width %d not in {1,2,4,8}", cfg_.txr);
     return E_ARG;
+  }
+  if (cfg_.movie) {
+    const ShotSpec& p = shots[0];
+    if (shots.size() != 1 || cfg_.movie_every < 1 || cfg_.movie_n < 1 || cfg_.movie_z0 < 0 || cfg_.movie_x0 < 0 ||
+        cfg_.movie_nz < 1 || cfg_.movie_nx < 1 || cfg_.movie_z0 + cfg_.movie_nz > p.nz ||
+        cfg_.movie_x0 + cfg_.movie_nx > p.nx) {
+      set_error("movie: one shot, frames >= 1 and a window inside the grid are required");
+      return E_ARG;
+    }
   }
   if (cfg_.snap_fmt == SNAP_HIST && cfg_.snap_every != 1) cfg_.snap_fmt = SNAP_F32;
   HIPCHK(hipStreamCreateWithFlags(&st_, hipStreamNonBlocking));
@@ -323,6 +333,18 @@ int Engine::enqueue_forward(int g, int mode, hipStream_t st) {
       else if ((L - 1) % cfg_.snap_every == 0) fl |= F_SNAP;
     }
     STCHK(launch_fwd(desc, tiles, gr.ntiles, L, fl, cfg_.txr, snap, cfg_.xcd, st));
+    if (mode == 3 && cfg_.movie && cfg_.movie_every > 0 && L % cfg_.movie_every == 0 &&
+        L / cfg_.movie_every <= cfg_.movie_n) {
+      const int s = gr.shots[0];
+      const long long k = L / cfg_.movie_every - 1;
+      const float* lev = ring_of(s) + (size_t)(L % 2) * 5 * res_[s].n2;
+      for (int f = 0; f < 5; ++f)
+        HIPCHK(hipMemcpy2DAsync(cfg_.movie + ((size_t)k * 5 + f) * cfg_.movie_nz * cfg_.movie_nx,
+                                cfg_.movie_nz * sizeof(float),
+                                lev + f * res_[s].n2 + cfg_.movie_z0 + (size_t)res_[s].ldz * cfg_.movie_x0,
+                                res_[s].ldz * sizeof(float), cfg_.movie_nz * sizeof(float), cfg_.movie_nx,
+                                hipMemcpyDeviceToDevice, st));
+    }
   }
   if (mode == 0 && cfg_.snap_fmt != SNAP_HIST && cfg_.nt % cfg_.snap_every == 0)
     STCHK(launch_snap_last(desc, tiles, gr.ntiles, cfg_.nt, cfg_.txr, cfg_.snap_fmt, st));

@@ -34,6 +34,13 @@ struct EngineCfg {
   int max_group = 0;   // cap on shots per group (0 = memory bound only)
   int stack_nz = 0, stack_nx = 0;  // stacked image (0 = none)
   double mem_fraction = 0.85;      // share of free memory the working set may use
+  // Wavefield movie of a one-shot model_data() run: at every level L that is
+  // a multiple of movie_every (up to movie_n frames) the five fields of the
+  // window [movie_z0, +movie_nz) x [movie_x0, +movie_nx) are copied to
+  // movie (device, [frame][field][x][z]).
+  int movie_every = 0, movie_n = 0;
+  int movie_z0 = 0, movie_x0 = 0, movie_nz = 0, movie_nx = 0;
+  float* movie = nullptr;
 };
 
 struct Timing {
@@ -111,5 +118,10 @@ void sponge_factors(int n, int abc, std::vector<float>& k);
 int ensure_device();
 int default_txr();
 int default_xcd();
+// Tile-width autotuner (autotune.hip): times the engine on a short copy of
+// the workload for TXR 1, 2, 4, 8 and returns the fastest in *best (ms_out:
+// the four times, -1 when the answer came from the cache).
+int autotune_txr(const EngineCfg& cfg, const std::vector<ShotSpec>& shots, const std::vector<const float*>& coef5,
+                 const float* wavelet, int* best, double ms_out[4], bool use_cache);
 
 }  // namespace erm

@@ -1,7 +1,7 @@
 // learned_abc.hip -- learned absorbing layer: band geometry, features,
 // training (forward, backward, Adam) and inference kernels (gfx950).
 //
-// The network is small (8965 parameters) and evaluated on the ~6% of cells
+// The network is small (9093 parameters) and evaluated on the ~6% of cells
 // that form the 8-cell band of a grid. Training and inference both use BF16-
 // rounded weights and features with FP32 accumulation; the training step keeps
 // FP32 master weights and Adam moments.
@@ -113,25 +113,47 @@ __device__ inline float raw_feature(const Lev& m1, const Lev& m2, int iz, int ix
   if (k < 230) return m2.get(k - 225, iz, ix);
   return 0.f;
 }
-__device__ inline float onehot_feature(int edge, int depth, int k) {
+// Material of a cell from the dt/dx-scaled coefficient arrays (0 m1_x,
+// 3 aux_m3, 4 aux_m2m3): P and S Courant numbers vp*dt/dx, vs*dt/dx and the
+// P impedance rho*vp / Z0.
+struct Mat {
+  const float* c;
+  int fs, ldz;
+  __device__ __forceinline__ void at(int iz, int ix, float& cp, float& cs, float& zp) const {
+    const int i = iz + ldz * ix;
+    const float m1 = c[i], m3 = c[3 * fs + i], m23 = c[4 * fs + i];
+    cp = sqrtf(m1 * m23);
+    cs = sqrtf(m1 * m3);
+    zp = m1 > 0.f ? sqrtf(m23 / m1) * (1.f / Z0) : 0.f;
+  }
+};
+
+// Features 230..247: one-hot edge, one-hot depth, material, zero padding.
+__device__ inline float static_feature(int edge, int depth, float cp, float cs, float zp, int k) {
   if (k >= 230 && k < 234) return (k - 230) == edge ? 1.f : 0.f;
   if (k >= 234 && k < 242) return (k - 234) == depth ? 1.f : 0.f;
+  if (k == 242) return cp;
+  if (k == 243) return cs;
+  if (k == 244) return zp;
   return 0.f;
 }
 
 // Features [ncell][KP] (normalised, bf16-rounded) and scales of every band
 // cell of one grid.
-__global__ void k_features(Lev m1, Lev m2, Band b, float* feat, float* scale) {
+__global__ void k_features(Lev m1, Lev m2, Mat mat, Band b, float* feat, float* scale) {
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= b.total) return;
   int iz, ix, edge, depth;
   cell_coords(b, c, iz, ix, edge, depth);
+  float cp, cs, zp;
+  mat.at(iz, ix, cp, cs, zp);
   float s = 0.f;
   for (int k = 0; k < 230; ++k) s = fmaxf(s, fabsf(raw_feature(m1, m2, iz, ix, edge, depth, k)));
   const float inv = s > 1e-30f ? 1.f / s : 0.f;
   float* out = feat + (size_t)c * KP;
   for (int k = 0; k < KP; ++k) {
-    const float v = k < 230 ? raw_feature(m1, m2, iz, ix, edge, depth, k) * inv : onehot_feature(edge, depth, k);
+    const float v = k < 230 ? raw_feature(m1, m2, iz, ix, edge, depth, k) * inv
+                            : static_feature(edge, depth, cp, cs, zp, k);
     out[k] = bf16r(v);
   }
   scale[c] = s > 1e-30f ? s : 0.f;
@@ -327,7 +349,7 @@ __global__ void k_sample(int* idx, int batch, int n, unsigned seed) {
 
 // Inference on a grid: corrections delta[cell][5] (physical units) from
 // levels L-1 and L-2, one thread per band cell, weights in LDS.
-__global__ __launch_bounds__(256) void k_infer(const float* __restrict__ w, Lev m1, Lev m2, Band b,
+__global__ __launch_bounds__(256) void k_infer(const float* __restrict__ w, Lev m1, Lev m2, Mat mat, Band b,
                                                float* __restrict__ delta) {
   __shared__ float sW1[NF * H];
   __shared__ float sW2[H * H];
@@ -346,6 +368,8 @@ __global__ __launch_bounds__(256) void k_infer(const float* __restrict__ w, Lev 
   if (c >= b.total) return;
   int iz, ix, edge, depth;
   cell_coords(b, c, iz, ix, edge, depth);
+  float cp, cs, zp;
+  mat.at(iz, ix, cp, cs, zp);
   float s = 0.f;
   for (int k = 0; k < 230; ++k) s = fmaxf(s, fabsf(raw_feature(m1, m2, iz, ix, edge, depth, k)));
   if (!(s > 1e-30f)) {
@@ -356,7 +380,8 @@ __global__ __launch_bounds__(256) void k_infer(const float* __restrict__ w, Lev 
   float h1[H];
   for (int j = 0; j < H; ++j) h1[j] = sb[j];
   for (int k = 0; k < NF; ++k) {
-    const float x = bf16r(k < 230 ? raw_feature(m1, m2, iz, ix, edge, depth, k) * inv : onehot_feature(edge, depth, k));
+    const float x = bf16r(k < 230 ? raw_feature(m1, m2, iz, ix, edge, depth, k) * inv
+                                  : static_feature(edge, depth, cp, cs, zp, k));
     if (x == 0.f) continue;
 #pragma unroll
     for (int j = 0; j < H; ++j) h1[j] += x * sW1[k * H + j];
@@ -537,22 +562,25 @@ RTM_EXPORT int labc_net_grad(labc_net* n, const float* feat, const float* target
   return OK;
 }
 
-RTM_EXPORT int labc_features(int nz, int nx, const float* lev_m1, const float* lev_m2, float* feat,
-                             float* scale) {
+RTM_EXPORT int labc_features(int nz, int nx, const float* lev_m1, const float* lev_m2, const float* coef5,
+                             float* feat, float* scale) {
   const int nc = labc_cell_count(nz, nx);
-  if (nc == 0 || !lev_m1 || !lev_m2 || !feat || !scale) { set_error("bad argument"); return E_ARG; }
+  if (nc == 0 || !lev_m1 || !lev_m2 || !coef5 || !feat || !scale) { set_error("bad argument"); return E_ARG; }
   STCHK(ensure_device());
   const size_t n2 = (size_t)nz * nx;
-  DevBuf a, b, f, s;
+  DevBuf a, b, cf, f, s;
   STCHK(a.alloc(5 * n2 * 4, "level"));
   STCHK(b.alloc(5 * n2 * 4, "level"));
+  STCHK(cf.alloc(5 * n2 * 4, "coefficients"));
+  HIPCHK(hipMemcpy(cf.p, coef5, 5 * n2 * 4, hipMemcpyHostToDevice));
   STCHK(f.alloc((size_t)nc * KP * 4, "features"));
   STCHK(s.alloc((size_t)nc * 4, "scales"));
   HIPCHK(hipMemcpy(a.p, lev_m1, 5 * n2 * 4, hipMemcpyHostToDevice));
   HIPCHK(hipMemcpy(b.p, lev_m2, 5 * n2 * 4, hipMemcpyHostToDevice));
   Lev l1{a.as<float>(), (int)n2, nz, nz, nx}, l2{b.as<float>(), (int)n2, nz, nz, nx};
   const Band bd = band_of(nz, nx);
-  hipLaunchKernelGGL(k_features, dim3((nc + 255) / 256), dim3(256), 0, nullptr, l1, l2, bd, f.as<float>(),
+  const Mat mat{cf.as<float>(), (int)n2, nz};
+  hipLaunchKernelGGL(k_features, dim3((nc + 255) / 256), dim3(256), 0, nullptr, l1, l2, mat, bd, f.as<float>(),
                      s.as<float>());
   HIPCHK(hipGetLastError());
   HIPCHK(hipDeviceSynchronize());
@@ -638,6 +666,7 @@ struct LGrid {
     return launch_fwd(desc.as<ShotDev>(), tiles.as<TileRef>(), ntiles, L, flags, 4, SNAP_HIST, 1, st);
   }
   Lev lev_view(int L) const { return Lev{level(L), (int)n2, ldz, h.nz, h.nx}; }
+  Mat mat() const { return Mat{coef.as<float>(), (int)n2, ldz}; }
 };
 
 RTM_EXPORT int labc_collect(labc_data* d, int nz_phys, int nx_phys, int pad_wide, int abc_wide, int nt,
@@ -675,7 +704,7 @@ RTM_EXPORT int labc_collect(labc_data* d, int nz_phys, int nx_phys, int pad_wide
                          gs.level(L - 2), (int)gs.n2, gs.ldz, nzs, nxs);
       HIPCHK(hipGetLastError());
       hipLaunchKernelGGL(k_features, dim3((bd.total + 255) / 256), dim3(256), 0, nullptr, gs.lev_view(L - 1),
-                         gs.lev_view(L - 2), bd, d->feat.as<float>() + (size_t)d->size * KP, scales.as<float>());
+                         gs.lev_view(L - 2), gs.mat(), bd, d->feat.as<float>() + (size_t)d->size * KP, scales.as<float>());
       HIPCHK(hipGetLastError());
     }
     STCHK(gw.step(L, (L >= 3 ? F_K : 0) | F_SRC, nullptr));
@@ -761,7 +790,7 @@ RTM_EXPORT int labc_forward(labc_net* n, const rtm_shot_t* s, const float* coef5
     const bool learn = n && L >= 3;
     if (learn) {
       hipLaunchKernelGGL(k_infer, dim3((bd.total + 255) / 256), dim3(256), 0, nullptr, n->w.as<float>(),
-                         g.lev_view(L - 1), g.lev_view(L - 2), bd, delta.as<float>());
+                         g.lev_view(L - 1), g.lev_view(L - 2), g.mat(), bd, delta.as<float>());
       HIPCHK(hipGetLastError());
     }
     STCHK(g.step(L, fl, nullptr));

@@ -20,7 +20,7 @@ import numpy as np
 from . import lib as _lib
 from .lib import _chk, _p, pack_fields, unpack_fields
 
-RIM, WIDTH, NFEAT, KPAD, HID, NOUT = 5, 8, 242, 248, 32, 5
+RIM, WIDTH, NFEAT, KPAD, HID, NOUT = 5, 8, 245, 248, 32, 5
 NPARAM = NFEAT * HID + HID + HID * HID + HID + HID * NOUT + NOUT
 Z0 = 6.0e6
 PAD = RIM + WIDTH  # padding of a band grid around the physical region
@@ -44,20 +44,21 @@ def cell_coords(nz, nx, c):
     return iz.value, ix.value, e.value, d.value
 
 
-def features(lev_m1, lev_m2):
+def features(lev_m1, lev_m2, coef5):
     """Normalised bf16 features (ncell, KPAD) and scales of the band cells
-    of one grid from levels L-1 and L-2 (five (nz, nx) arrays each)."""
+    of one grid from levels L-1 and L-2 and the grid's coefficients (five
+    (nz, nx) arrays each)."""
     nz, nx = np.asarray(lev_m1[0]).shape
     nc = cell_count(nz, nx)
-    a, b = pack_fields(lev_m1), pack_fields(lev_m2)
+    a, b, c = pack_fields(lev_m1), pack_fields(lev_m2), pack_fields(coef5)
     f = np.zeros((nc, KPAD), np.float32)
     s = np.zeros(nc, np.float32)
-    _chk(_L().labc_features(C.c_int(nz), C.c_int(nx), _p(a), _p(b), _p(f), _p(s)))
+    _chk(_L().labc_features(C.c_int(nz), C.c_int(nx), _p(a), _p(b), _p(c), _p(f), _p(s)))
     return f, s
 
 
 class Net:
-    """The 242 -> 32 -> 32 -> 5 GELU network (FP32 master weights, BF16 use)."""
+    """The 245 -> 32 -> 32 -> 5 GELU network (FP32 master weights, BF16 use)."""
 
     def __init__(self, seed=1):
         self.h = C.c_void_p()

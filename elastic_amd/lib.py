@@ -119,6 +119,20 @@ def forward(s: RtmShot, coef5, wavelet, want_last=False):
     return rec
 
 
+def forward_movie(s: RtmShot, coef5, wavelet, nsnap, z0, x0, wnz, wnx):
+    """Frames of one forward run (rtm_forward_movie_hip): returns (movie,
+    dsnap) with movie[k][f] the (wnz, wnx) window of field f at level
+    (k+1)*dsnap, dsnap = nt // nsnap."""
+    lib = load()
+    cb = pack_fields(coef5)
+    w = np.ascontiguousarray(wavelet, np.float32)
+    out = np.zeros((nsnap, 5, wnx, wnz), np.float32)
+    ds = C.c_int()
+    _chk(lib.rtm_forward_movie_hip(C.byref(s), _p(cb), _p(w), C.c_int(nsnap), C.c_int(z0), C.c_int(x0), C.c_int(wnz),
+                                   C.c_int(wnx), _p(out), C.byref(ds)))
+    return np.ascontiguousarray(out.transpose(0, 1, 3, 2)), ds.value
+
+
 def migrate(s: RtmShot, coef5, wavelet, rec):
     lib = load()
     cb = pack_fields(coef5)
@@ -294,6 +308,11 @@ class Batch:
         _chk(load().rtm_batch_get_stack(self._h, _p(img)))
         return tuple(unpack_fields(img))
 
+    @property
+    def txr(self):
+        """Tile width the batch runs with (autotuned when created with txr=0)."""
+        return int(load().rtm_batch_txr(self._h))
+
     def info(self):
         out = np.zeros(8, np.float64)
         _chk(load().rtm_batch_info(self._h, _p(out)))
@@ -310,6 +329,27 @@ class Batch:
             self.close()
         except Exception:
             pass
+
+
+def autotune(nt, dx, dt, shots, coefs, wavelet, abc=0, adj_abc=0, snap_every=1, snap_prec="fp32", comp_image=0,
+             use_graph=0):
+    """Run the tile-width autotuner on a workload (rtm_batch_autotune):
+    returns (best txr, {txr: ms})."""
+    lib = load()
+    n = len(shots)
+    arr = (RtmBatchShot * n)()
+    for i, s in enumerate(shots):
+        for k, _ in RtmBatchShot._fields_:
+            setattr(arr[i], k, int(s.get(k, 0)))
+    cfg = RtmBatchCfg(nt, dx, dt, abc, adj_abc, snap_every, SNAP_PREC[snap_prec] if isinstance(snap_prec, str) else snap_prec,
+                      comp_image, 0, use_graph, 0, 0, 0, 0.0)
+    keep = [pack_fields(c) for c in coefs]
+    ptrs = (C.c_void_p * n)(*[k.ctypes.data for k in keep])
+    w = np.ascontiguousarray(wavelet, np.float32)
+    best = C.c_int()
+    ms = np.zeros(4, np.float64)
+    _chk(lib.rtm_batch_autotune(C.byref(cfg), C.c_int(n), arr, ptrs, _p(w), C.byref(best), _p(ms)))
+    return best.value, dict(zip((1, 2, 4, 8), ms.tolist()))
 
 
 def bandwidth_probe(nbytes=1 << 30, reps=20, mode=0):

@@ -8,16 +8,21 @@
  * an additive correction of the five fields of the new level:
  *   X(L)[cell] = plain stencil update (no sponge) + s * NN(features / s)
  *
- * Features of a band cell (LABC_NFEAT = 242, padded to LABC_KPAD = 248):
- *   - the 5 fields of level L-1 on a 9 x 5 patch: 9 cells from the cell
- *     inwards along the edge normal times 5 tangential offsets (-2..2);
- *     stresses are divided by LABC_Z0 (an impedance scale, Pa / (m/s));
- *   - the cell's own 5 fields of level L-2 (same scaling);
- *   - one-hot edge (top, bottom, left, right) and one-hot depth in the band.
- * Wavefield features are divided by the cell's scale s = max-abs over the
- * scaled patch values, so the correction is equivariant to the wavefield
- * amplitude. Network: 242 -> 32 -> 32 -> 5, exact GELU, BF16 weights and
- * BF16-rounded features with FP32 accumulation (8965 parameters).
+ * Features of a band cell (LABC_NFEAT = 245, padded to LABC_KPAD = 248):
+ *   0..224   the 5 fields of level L-1 on a 9 x 5 patch: 9 cells from the
+ *            cell inwards along the edge normal (slot kn) times 5 tangential
+ *            offsets m = -2..2, feature kn*25 + (m+2)*5 + field; stresses
+ *            are divided by LABC_Z0 (an impedance scale, Pa / (m/s));
+ *   225..229 the cell's own 5 fields of level L-2 (same scaling);
+ *   230..233 one-hot edge (top, bottom, left, right);
+ *   234..241 one-hot depth in the band;
+ *   242..244 the cell's material from the dt/dx-scaled coefficients:
+ *            sqrt(m1_x*aux_m2m3) = vp dt/dx, sqrt(m1_x*aux_m3) = vs dt/dx,
+ *            sqrt(aux_m2m3/m1_x)/LABC_Z0 = rho vp / LABC_Z0.
+ * Wavefield features (0..229) are divided by the cell's scale s = max-abs
+ * over them, so the correction is equivariant to the wavefield amplitude.
+ * Network: 245 -> 32 -> 32 -> 5, exact GELU, BF16 weights and BF16-rounded
+ * features with FP32 accumulation (9093 parameters).
  * Band cells: top rows [R, R+W) and bottom rows [nz-R-W, nz-R) over columns
  * [R, nx-R) (corners belong to them), left columns [R, R+W) and right
  * columns [nx-R-W, nx-R) over rows [R+W, nz-R-W); R = 5, W = 8.
@@ -33,7 +38,7 @@ extern "C" {
 
 #define LABC_RIM 5
 #define LABC_WIDTH 8
-#define LABC_NFEAT 242
+#define LABC_NFEAT 245
 #define LABC_KPAD 248
 #define LABC_HID 32
 #define LABC_NOUT 5
@@ -43,7 +48,7 @@ extern "C" {
 
 typedef struct labc_net labc_net;
 
-/* Parameters in order W1 (242x32, row-major [k][j]), b1 (32), W2 (32x32),
+/* Parameters in order W1 (245x32, row-major [k][j]), b1 (32), W2 (32x32),
  * b2 (32), W3 (32x5), b3 (5). */
 int labc_net_create(labc_net **net, unsigned seed);
 int labc_net_destroy(labc_net *net);
@@ -60,10 +65,11 @@ int labc_net_forward(labc_net *net, const float *feat, int n, float *out, int mo
 /* Loss 0.5*mean((out - target)^2) over n*5 values and its gradient [LABC_NPARAM]. */
 int labc_net_grad(labc_net *net, const float *feat, const float *target, int n, float *grad,
                   float *loss);
-/* Features of the band cells of one grid from two levels (host arrays, each
- * 5 fields (nz,nx) back to back): feat [ncell][LABC_KPAD], scale [ncell]. */
-int labc_features(int nz, int nx, const float *lev_m1, const float *lev_m2, float *feat,
-                  float *scale);
+/* Features of the band cells of one grid from two levels and the grid's
+ * coefficients (host arrays, each 5 fields (nz,nx) back to back):
+ * feat [ncell][LABC_KPAD], scale [ncell]. */
+int labc_features(int nz, int nx, const float *lev_m1, const float *lev_m2, const float *coef5,
+                  float *feat, float *scale);
 
 /* Training set on the device. */
 typedef struct labc_data labc_data;

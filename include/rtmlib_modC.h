@@ -120,6 +120,14 @@ typedef struct {
 int rtm_forward_hip(const rtm_shot_t *s, const float *coef5, const float *wavelet, float *rec,
                     float *last);
 
+/* Wavefield movie of one forward run (the reference's snap_wfld,
+ * W/snap-mod-rtm.f90:154-283): frame k = 1..nsnap is level k*dsnap with
+ * dsnap = nt / nsnap, the five fields of the window [z0, z0+wnz) x
+ * [x0, x0+wnx) (0-based cells). movie is [nsnap][5][wnx][wnz] (each field
+ * block in the reference layout); *dsnap receives the frame spacing. */
+int rtm_forward_movie_hip(const rtm_shot_t *s, const float *coef5, const float *wavelet, int nsnap,
+                          int z0, int x0, int wnz, int wnx, float *movie, int *dsnap);
+
 /* Full RTM of one shot (forward + adjoint + Lame imaging). coef5 holds the
  * five (nz,nx) arrays m1_x, m1_z, aux_m2, aux_m3, aux_m2m3 back to back.
  * rec: 5*(nt,nrx) observed data. img: 3*(nz,nx) = drho, dlambda, dmu. */
@@ -174,7 +182,7 @@ typedef struct {
   int snap_every;   /* imaging decimation j */
   int snap_prec;    /* 0 fp32, 1 bf16, 2 fp16 */
   int comp_image;   /* compensated image sums */
-  int txr;          /* tile width 1/2/4/8 (0 = default) */
+  int txr;          /* tile width 1/2/4/8 (0 = autotuned, or RTM_TXR when set) */
   int use_graph;    /* capture time loops in HIP graphs */
   int max_group;    /* cap on shots per group (0 = memory bound) */
   int stack_nz, stack_nx;  /* stacked image size (0 = none) */
@@ -198,6 +206,15 @@ int rtm_batch_get_stack(rtm_batch *b, float *img3);            /* 3*(stack_nz,st
  * [5] fwd cell-updates [6] adj cell-updates [7] shots per group */
 int rtm_batch_info(rtm_batch *b, double *out8);
 int rtm_batch_destroy(rtm_batch *b);
+/* Tile width the batch runs with. */
+int rtm_batch_txr(rtm_batch *b);
+/* Tile-width autotuner: runs a short copy of the workload (up to 8 of the
+ * shots, max(48, 8*snap_every) levels, same options) with each width 1, 2,
+ * 4, 8 and returns the fastest in *txr and the four times (ms) in ms4. The
+ * result is cached per process and, when RTM_TUNE_CACHE names a file, in it;
+ * rtm_batch_create with cfg->txr == 0 uses the cached or tuned width. */
+int rtm_batch_autotune(const rtm_batch_cfg_t *cfg, int nshots, const rtm_batch_shot_t *shots,
+                       const float *const *coef5, const float *wavelet, int *txr, double *ms4);
 
 /* Streaming kernel over nbytes, timed over reps launches with HIP events:
  * achieved GB/s (bytes read + written) and ms per launch. mode 0: copy,

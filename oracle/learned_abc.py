@@ -8,11 +8,11 @@ GPU results can be checked:
   top rows, bottom rows (both over all inner columns), left columns, right
   columns (over the remaining rows);
 * features of a band cell from levels L-1 and L-2 (a 9 x 5 patch of the five
-  fields of L-1 along the inward normal, the cell's own fields of L-2,
-  one-hot edge and depth), divided by the patch max-abs scale s and rounded
-  to bf16; ``dtype=np.float32`` reproduces the kernel's float32 operation
+  fields of L-1 along the inward normal, the cell's own fields of L-2, divided
+  by their max-abs scale s; one-hot edge and depth; the cell's P and S Courant
+  numbers and P impedance from the coefficients), rounded to bf16; ``dtype=np.float32`` reproduces the kernel's float32 operation
   order bit for bit, ``np.float64`` is the exact definition;
-* the network 242 -> 32 -> 32 -> 5 with exact GELU on bf16-rounded weights,
+* the network 245 -> 32 -> 32 -> 5 with exact GELU on bf16-rounded weights,
   evaluated in FP64, with the loss 0.5 * mean((y - t)^2) and its gradient
   (hand-written backward pass);
 * teacher-forced training pairs: the band grid's levels L-1, L-2 are windows
@@ -33,7 +33,7 @@ from scipy.special import erf
 
 from . import forward, fwd_level, make_params
 
-R, W, NF, KP, H, NO = 5, 8, 242, 248, 32, 5
+R, W, NF, KP, H, NO = 5, 8, 245, 248, 32, 5
 NRAW = 230
 Z0 = 6.0e6
 OW1 = 0
@@ -111,7 +111,21 @@ def raw_patch(lev_m1, lev_m2, cells, dtype=np.float64):
     return out
 
 
-def features(lev_m1, lev_m2, dtype=np.float64, cells=None):
+def material(coefs, cells, dtype=np.float64):
+    """(vp dt/dx, vs dt/dx, rho vp / Z0) of the band cells from the dt/dx-scaled
+    coefficients m1_x, aux_m3, aux_m2m3."""
+    iz, ix = cells[0], cells[1]
+    m1 = np.asarray(coefs[0], dtype)[iz, ix]
+    m3 = np.asarray(coefs[3], dtype)[iz, ix]
+    m23 = np.asarray(coefs[4], dtype)[iz, ix]
+    cp = np.sqrt((m1 * m23).astype(dtype))
+    cs = np.sqrt((m1 * m3).astype(dtype))
+    with np.errstate(divide="ignore", invalid="ignore"):
+        zp = np.where(m1 > 0, np.sqrt((m23 / np.where(m1 > 0, m1, 1)).astype(dtype)) * _unit(2, dtype), 0)
+    return cp, cs, zp.astype(dtype)
+
+
+def features(lev_m1, lev_m2, coefs, dtype=np.float64, cells=None):
     """Normalised, bf16-rounded features (ncell, KP) and scales (ncell,)."""
     if cells is None:
         cells = band_cells(*lev_m1[0].shape)
@@ -126,6 +140,8 @@ def features(lev_m1, lev_m2, dtype=np.float64, cells=None):
     rows = np.arange(raw.shape[0])
     x[rows, NRAW + edge] = 1.0
     x[rows, NRAW + 4 + depth] = 1.0
+    cp, cs, zp = material(coefs, cells, dtype)
+    x[:, 242], x[:, 243], x[:, 244] = cp, cs, zp
     feat = bf16(x.astype(np.float32))
     return feat, np.where(live, s, 0).astype(dtype)
 
@@ -212,7 +228,7 @@ def teacher_pairs(nz_phys, nx_phys, pad_wide, abc_wide, nt, dt, zs_phys, xs_phys
             continue
         m1 = window([h[:, :, L - 2] for h in hist], off, nzs, nxs)
         m2 = window([h[:, :, L - 3] for h in hist], off, nzs, nxs)
-        f, s = features(m1, m2, np.float64, cells)
+        f, s = features(m1, m2, coefs_small, np.float64, cells)
         plain = fwd_level(ps, m2, m1, coefs_small, apply_k=False, srcamp=dt * wavelet[L - 2])
         t = np.zeros((iz.size, NO))
         for fi in range(5):

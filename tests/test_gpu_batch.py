@@ -119,3 +119,27 @@ def test_batch_rejects_bad_shot(hip):
     specs[1] = dict(specs[1], xs=specs[1]["nx"] + 4)
     with pytest.raises(hip.RtmError):
         hip.Batch(NT, DX, DT, specs, cmig, wav, abc=ABC)
+
+
+def test_tile_widths_agree_and_autotuner(hip):
+    """Every tile width computes the same per-cell arithmetic (fp32 snapshots:
+    bitwise equal images); the autotuner returns the fastest width and a batch
+    created with txr=0 runs with it."""
+    specs, ctrue, cmig, wav = _survey(hip)
+    imgs = {}
+    for txr in (1, 2, 4, 8):
+        b = _batch(hip, specs, ctrue, cmig, wav, txr=txr)
+        assert b.txr == txr
+        b.migrate()
+        imgs[txr] = [b.image(i) for i in range(len(specs))]
+        b.close()
+    for txr in (2, 4, 8):
+        for a, c in zip(imgs[1], imgs[txr]):
+            for x, y in zip(a, c):
+                assert np.array_equal(x, y), txr
+    best, ms = hip.autotune(NT, DX, DT, specs, cmig, wav, abc=ABC, adj_abc=1)
+    assert best in (1, 2, 4, 8)
+    assert all(v > 0 for v in ms.values()) and ms[best] == min(ms.values())
+    b = _batch(hip, specs, ctrue, cmig, wav, txr=0)
+    assert b.txr in (1, 2, 4, 8)
+    b.close()

@@ -240,3 +240,30 @@ def test_shot_argument_errors(hip):
     assert ok and abs(lim - 0.5370) < 1e-3
     ok, cr, lim = hip.cfl_check(6000.0, 1e-3, 10.0)
     assert not ok
+
+
+def test_movie_frames_are_forward_levels(hip, tmp_path):
+    """rtm_forward_movie_hip frames equal the FP64 oracle's levels k*dsnap in
+    the window, and snap_wfld writes them as the reference's SEP cubes."""
+    from elastic_amd import sep, snap
+
+    nzp, nxp, abc, nt = 40, 50, 10, 130
+    geo = synth.single_shot_geometry(nzp, nxp, abc, 10.0)
+    m = synth.elastic_from_vp(synth.pad_edge(synth.layered_vp(nzp, nxp, 3), geo.pad))
+    coefs = hip.model_op(m.rho, m.lam, m.mu, 10.0, 1e-3)
+    wav = hip.ricker(nt, 1e-3, 25.0)
+    s = hip.shot(geo.nz, geo.nx, nt, 10.0, 1e-3, geo.xs, geo.zs, geo.zr, geo.rx0, geo.nrx, abc)
+    z0, x0, wnz, wnx = snap.window(geo.nz, geo.nx, abc)
+    movie, dsnap = hip.forward_movie(s, coefs, wav, 6, z0, x0, wnz, wnx)
+    assert dsnap == nt // 6 and movie.shape == (6, 5, wnz, wnx)
+    p = O.make_params(geo.nz, geo.nx, nt, 10.0, 1e-3, geo.xs, geo.zs, geo.zr, geo.rx0, geo.nrx, abc)
+    h5 = O.forward(p, wav, coefs, want_hist5=True)["hist5"]
+    for k in range(6):
+        L = (k + 1) * dsnap
+        for f in range(5):
+            ref = h5[f][z0:z0 + wnz, x0:x0 + wnx, L - 1]
+            assert O.rel_l2(movie[k, f], ref) <= 1e-5, (k, f)
+    paths = snap.snap_wfld(s, coefs, wav, str(tmp_path), nsnap=6)
+    cube, h = sep.read(paths["Vz0"])
+    assert cube.shape == (wnz, wnx, 6) and np.array_equal(cube[:, :, 2], movie[2, 1])
+    assert sep.axes(h)[2][2] == pytest.approx(1e-3 * dsnap)

@@ -34,8 +34,11 @@ def test_features_bit_exact(L):
     m1 = [(rng.standard_normal((nz, nx)) * (1e-3 if f < 2 else 1e4)).astype(np.float32) for f in range(5)]
     m2 = [(rng.standard_normal((nz, nx)) * (1e-3 if f < 2 else 1e4)).astype(np.float32) for f in range(5)]
     m1[0][:, 30:] = 0  # some cells with an all-zero patch of Vx
-    f, s = L.features(m1, m2)
-    fo, so = LA.features(m1, m2, np.float32)
+    vp = synth.layered_vp(nz, nx, 2, vp_top=1800.0, vp_bot=3500.0, nlayers=2, anom_sigma=5.0)
+    m = synth.elastic_from_vp(vp)
+    coefs = O.model_op(m.rho, m.lam, m.mu, 10.0, 1e-3, np.float32)
+    f, s = L.features(m1, m2, coefs)
+    fo, so = LA.features(m1, m2, coefs, np.float32)
     assert np.array_equal(s, so.astype(np.float32))
     assert np.array_equal(f, fo)
 

@@ -118,21 +118,44 @@ def _rand_levels(nz, nx, seed):
     return m1, m2
 
 
+def _coefs(nz, nx, seed, dx=10.0, dt=1e-3):
+    """Piecewise-constant blocks (so that most cells have a constant 2x2
+    neighbourhood) with seeded velocities."""
+    rng = np.random.default_rng(seed)
+    z, x = np.meshgrid(np.arange(nz), np.arange(nx), indexing="ij")
+    vp = np.asfortranarray(rng.uniform(1800, 3500, (4,))[(z >= nz // 2) * 2 + (x >= nx // 3)])
+    m = synth.elastic_from_vp(vp)
+    return O.model_op(m.rho, m.lam, m.mu, dx, dt), m
+
+
 def test_features_scaling_and_layout():
     nz, nx = 36, 44
     m1, m2 = _rand_levels(nz, nx, 8)
-    f, s = LA.features(m1, m2)
+    coefs, model = _coefs(nz, nx, 9)
+    f, s = LA.features(m1, m2, coefs)
     cells = LA.band_cells(nz, nx)
     iz, ix, edge, depth = cells
     # exact scaling law: features are invariant, scales double
-    f2, s2 = LA.features([2 * a for a in m1], [2 * a for a in m2])
+    f2, s2 = LA.features([2 * a for a in m1], [2 * a for a in m2], coefs)
     assert np.array_equal(f, f2) and np.array_equal(s2, 2 * s)
     # max-abs normalisation: the largest raw feature is exactly +-1
     assert np.all(np.abs(f[:, :LA.NRAW]).max(axis=1) == 1.0)
     # one-hots and zero padding
     assert np.array_equal(np.argmax(f[:, 230:234], 1), edge) and np.all(f[:, 230:234].sum(1) == 1)
     assert np.array_equal(np.argmax(f[:, 234:242], 1), depth) and np.all(f[:, 234:242].sum(1) == 1)
-    assert np.all(f[:, 242:] == 0)
+    assert np.all(f[:, 245:] == 0)
+    # material: Courant numbers and impedance of the cell. The coefficients
+    # use averaged rho / harmonic lambda, mu, so compare where the model is
+    # locally constant (its 2x2 neighbourhood) to the closed form
+    vp, vs, rho = model.vp[iz, ix], model.vs[iz, ix], model.rho[iz, ix]
+    flat = np.ones(iz.size, bool)
+    for a in (model.vp,):
+        for dz, dx in ((1, 0), (0, 1), (1, 1)):
+            flat &= a[np.minimum(iz + dz, nz - 1), np.minimum(ix + dx, nx - 1)] == a[iz, ix]
+    assert flat.sum() > 100
+    for k, want in ((242, vp * 1e-3 / 10.0), (243, vs * 1e-3 / 10.0), (244, rho * vp / LA.Z0)):
+        got = f[flat, k]
+        assert np.all(np.abs(got - want[flat]) <= 2.0 ** -8 * np.abs(want[flat]) * 1.01), k
     # spot check: patch slot (kn, m, f) of a left-edge cell reads (iz+m, ix+kn)
     c = np.nonzero(edge == 2)[0][5]
     for kn, m, fi in ((0, 0, 0), (8, -2, 3), (3, 2, 4)):
@@ -141,7 +164,7 @@ def test_features_scaling_and_layout():
     # own cell of level L-2
     assert f[c, 227] == LA.bf16(np.float32([m2[2][iz[c], ix[c]] / LA.Z0 / s[c]]))[0]
     # float32 emulation agrees with the exact definition within one bf16 step
-    f32, s32 = LA.features(m1, m2, np.float32)
+    f32, s32 = LA.features(m1, m2, coefs, np.float32)
     assert np.all(np.abs(f32 - f) <= 2.0 ** -7 * np.abs(f) + 1e-30)
     assert np.allclose(s32, s, rtol=1e-6)
 
@@ -150,7 +173,8 @@ def test_features_outside_grid_are_zero():
     nz, nx = 30, 30
     m1 = [np.ones((nz, nx)) for _ in range(5)]
     m2 = [np.ones((nz, nx)) for _ in range(5)]
-    f, s = LA.features(m1, m2)
+    coefs, _ = _coefs(nz, nx, 1)
+    f, s = LA.features(m1, m2, coefs)
     # every patch of this grid stays inside it (9 inward, 2 tangential from
     # a cell at least 5 from every edge): all raw features are in-grid
     iz, ix, edge, _ = LA.band_cells(nz, nx)
@@ -158,7 +182,7 @@ def test_features_outside_grid_are_zero():
     # a grid too small for the 9-deep patch: the far slots fall outside
     m1[0][:, :] = 0
     m1[0][5, 5] = 1.0
-    fz, _ = LA.features(m1, m2)
+    fz, _ = LA.features(m1, m2, coefs)
     assert fz.shape[0] == iz.size
 
 
@@ -210,7 +234,7 @@ def test_teacher_targets_are_the_rim_contribution():
     zero = [np.zeros((nzs, nxs)) for _ in range(5)]
     contrib = O.fwd_level(ps, zero, rim, case["coefs_small"])
     _, s = LA.features(LA.window([h[:, :, L - 2] for h in hist], off, nzs, nxs),
-                       LA.window([h[:, :, L - 3] for h in hist], off, nzs, nxs))
+                       LA.window([h[:, :, L - 3] for h in hist], off, nzs, nxs), case["coefs_small"])
     iz, ix, _, _ = LA.band_cells(nzs, nxs)
     for fi in range(5):
         u = LA.Z0 if fi >= 2 else 1.0
