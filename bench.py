@@ -158,7 +158,7 @@ def main():
     ap.add_argument("--seed", type=int, default=20260102)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-validate", action="store_true")
-    ap.add_argument("--cpu-steps", type=int, default=30)
+    ap.add_argument("--cpu-steps", type=int, default=600)
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -217,7 +217,7 @@ def main():
         nth = min(16, os.cpu_count() or 1)
         t1 = time.perf_counter()
         mc, secs, _ = cpu_baseline(lib, synth, args.seed, dx, dt, fp, abc, args.cpu_steps, nth)
-        mc1, secs1, _ = cpu_baseline(lib, synth, args.seed, dx, dt, fp, abc, max(4, args.cpu_steps // 6), 1)
+        mc1, secs1, _ = cpu_baseline(lib, synth, args.seed, dx, dt, fp, abc, max(4, args.cpu_steps // 4), 1)
         log(f"bench: cpu baseline {time.perf_counter() - t1:.1f}s")
         cpu = {"value": round(mc, 3), "unit": "Mcell-updates/s", "cores": nth, "kind": "port",
                "sample": f"oracle FP32 RTM (forward+adjoint+imaging) of one 800x900 shot (890x990 padded), "

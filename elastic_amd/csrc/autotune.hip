@@ -7,7 +7,11 @@
 // a short copy of the workload (the first shots of the batch, a few dozen
 // levels, the same options) once per width and keeps the fastest. Results
 // are cached for the process, keyed by the workload signature, and, when
-// RTM_TUNE_CACHE names a file, in that file (one "key txr" line per entry).
+// RTM_TUNE_CACHE names a file, in that file (one "key code" line per entry,
+// code = txr + 16 * two_levels_per_launch). The two-level (temporally
+// blocked) kernels are candidates too: they halve the HBM traffic of the
+// fields but recompute a radius-5 ring of the first level and run at lower
+// occupancy, so which form wins is measured, not assumed.
 #include "engine2d.hpp"
 
 #include <cstdio>
@@ -54,8 +58,10 @@ static void load_file_cache() {
   if (!f) return;
   char key[512];
   int txr = 0;
-  while (std::fscanf(f, "%511s %d", key, &txr) == 2)
-    if (txr == 1 || txr == 2 || txr == 4 || txr == 8) g_tune_cache[key] = txr;
+  while (std::fscanf(f, "%511s %d", key, &txr) == 2) {
+    const int w = txr & 15, tb = txr >> 4;
+    if ((w == 1 || w == 2 || w == 4 || w == 8) && (tb == 0 || (tb == 1 && w <= 4))) g_tune_cache[key] = txr;
+  }
   std::fclose(f);
 }
 
@@ -69,8 +75,8 @@ static void append_file_cache(const std::string& key, int txr) {
 }
 
 int autotune_txr(const EngineCfg& cfg, const std::vector<ShotSpec>& shots, const std::vector<const float*>& coef5,
-                 const float* wavelet, int* best, double ms_out[4], bool use_cache) {
-  if (shots.empty() || coef5.size() != shots.size() || !wavelet || !best) {
+                 const float* wavelet, int* best, int* best_tblock, double ms_out[8], bool use_cache) {
+  if (shots.empty() || coef5.size() != shots.size() || !wavelet || !best || !best_tblock) {
     set_error("autotune: bad arguments");
     return E_ARG;
   }
@@ -80,9 +86,10 @@ int autotune_txr(const EngineCfg& cfg, const std::vector<ShotSpec>& shots, const
     load_file_cache();
     auto it = g_tune_cache.find(key);
     if (use_cache && it != g_tune_cache.end()) {
-      *best = it->second;
+      *best = it->second & 15;
+      *best_tblock = it->second >> 4;
       if (ms_out)
-        for (int i = 0; i < 4; ++i) ms_out[i] = -1.0;
+        for (int i = 0; i < 8; ++i) ms_out[i] = -1.0;
       return OK;
     }
   }
@@ -97,11 +104,17 @@ int autotune_txr(const EngineCfg& cfg, const std::vector<ShotSpec>& shots, const
   c.nt = std::min(c.nt, cfg.nt);
   c.stack_nz = c.stack_nx = 0;
   c.max_group = 0;
+  // candidates: widths 1, 2, 4, 8 with one level per launch (slots 0..3)
+  // and widths 1, 2, 4 with two levels per launch (slots 4..6)
   const int cand[4] = {1, 2, 4, 8};
-  double t[4];
+  double t[8];
   int arg = -1;
-  for (int i = 0; i < 4; ++i) {
-    c.txr = cand[i];
+  for (int i = 0; i < 8; ++i) {
+    t[i] = -1.0;
+    const int tb = i >= 4, txr = cand[i & 3];
+    if (tb && txr > 4) continue;
+    c.txr = txr;
+    c.tblock = tb;
     Engine e;
     STCHK(e.setup(c, sp, cf, wavelet));
     STCHK(e.migrate());  // warm-up (and graph instantiation)
@@ -109,12 +122,14 @@ int autotune_txr(const EngineCfg& cfg, const std::vector<ShotSpec>& shots, const
     t[i] = e.timing().total_ms;
     if (arg < 0 || t[i] < t[arg]) arg = i;
   }
-  *best = cand[arg];
+  *best = cand[arg & 3];
+  *best_tblock = arg >= 4;
   if (ms_out)
-    for (int i = 0; i < 4; ++i) ms_out[i] = t[i];
+    for (int i = 0; i < 8; ++i) ms_out[i] = t[i];
   std::lock_guard<std::mutex> lk(g_tune_mu);
-  g_tune_cache[key] = *best;
-  append_file_cache(key, *best);
+  const int code = *best | (*best_tblock << 4);
+  g_tune_cache[key] = code;
+  append_file_cache(key, code);
   return OK;
 }
 

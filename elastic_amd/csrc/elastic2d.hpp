@@ -66,7 +66,8 @@ enum : int {
   F_SRC = 8,     // inject the explosive source
   F_BORN = 16,   // add the Born sources bsrc
   F_SNAP = 32,   // write the source-wavefield snapshot belonging to this step
-  F_NTS = 64     // non-temporal stores of the new level
+  F_NTS = 64,    // non-temporal stores of the new level
+  F_SNAP2 = 128  // two-level step: also write the snapshot of level L
 };
 // Adjoint flags
 enum : int {
@@ -76,7 +77,8 @@ enum : int {
   A_IMG = 8,   // accumulate the imaging condition at this level
   A_NEXT = 16, // level L+1 exists (S1, S2 nonzero)
   A_COMP = 32, // compensated image accumulation
-  A_NTS = 64   // non-temporal stores of the new level
+  A_NTS = 64,  // non-temporal stores of the new level
+  A_IMG2 = 128 // two-level step: also image level L-1
 };
 
 // Tile geometry: 64 z-cells (one wave64 along the fast axis) by 4*TXR
@@ -90,6 +92,14 @@ int launch_fwd(const ShotDev* shots, const TileRef* tiles, int ntiles, int L, in
                int txr, int snap_fmt, int xcd, hipStream_t st);
 int launch_adj(const ShotDev* shots, const TileRef* tiles, int ntiles, int L, int flags,
                int txr, int snap_fmt, int xcd, hipStream_t st);
+// Levels L and L+1 in one pass (temporal blocking; TXR 1, 2, 4). The ring
+// must hold four levels. No Born sources.
+int launch_fwd2(const ShotDev* shots, const TileRef* tiles, int ntiles, int L, int flags,
+                int txr, int snap_fmt, int xcd, hipStream_t st);
+// Adjoint levels L and L-1 in one pass (TXR 1, 2, 4; four-level ring; not
+// the drop-in history form).
+int launch_adj2(const ShotDev* shots, const TileRef* tiles, int ntiles, int L, int flags,
+                int txr, int snap_fmt, int xcd, hipStream_t st);
 // Snapshot of the last level (no later step writes it): dV = 0.
 int launch_snap_last(const ShotDev* shots, const TileRef* tiles, int ntiles, int L, int txr,
                      int snap_fmt, hipStream_t st);

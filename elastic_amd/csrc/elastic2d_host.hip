@@ -129,6 +129,7 @@ static int single_engine(Engine& e, const rtm_shot_t* s, const float* coef5, con
   cfg.comp = s->comp_image;
   cfg.txr = default_txr();
   cfg.xcd = default_xcd();
+  cfg.tblock = default_tblock();
   cfg.use_graph = getenv("RTM_GRAPH") ? atoi(getenv("RTM_GRAPH")) : 0;
   ShotSpec p{};
   p.nz = s->nz; p.nx = s->nx; p.xs = s->xs; p.zs = s->zs; p.zr = s->zr; p.rx0 = s->rx0; p.nrx = s->nrx;
@@ -290,7 +291,7 @@ struct rtm_ctx {
 
 struct rtm_batch {
   Engine eng;
-  int nshots = 0, nt = 0, txr = 0;
+  int nshots = 0, nt = 0, txr = 0, tblock = 0;
   std::vector<int> nz, nx, nrx;
 };
 
@@ -358,6 +359,7 @@ RTM_EXPORT int rtm_forward_movie_hip(const rtm_shot_t* s, const float* coef5, co
   cfg.nt = s->nt; cfg.dx = s->dx; cfg.dt = s->dt; cfg.abc = s->abc;
   cfg.txr = default_txr();
   cfg.xcd = default_xcd();
+  cfg.tblock = default_tblock();
   cfg.movie_every = every;
   cfg.movie_n = nsnap;
   cfg.movie_z0 = z0; cfg.movie_x0 = x0; cfg.movie_nz = wnz; cfg.movie_nx = wnx;
@@ -573,8 +575,9 @@ RTM_EXPORT int rtm_batch_create(rtm_batch** out, const rtm_batch_cfg_t* cfg, int
   ec.snap_fmt = snap_fmt_of(cfg->snap_prec, ec.snap_every);
   ec.comp = cfg->comp_image;
   ec.txr = cfg->txr > 0 ? cfg->txr : default_txr();
-  const bool tune = cfg->txr == 0 && !getenv("RTM_TXR");
+  const bool tune = cfg->txr == 0 && !getenv("RTM_TXR") && !getenv("RTM_TBLOCK");
   ec.xcd = default_xcd();
+  ec.tblock = default_tblock();
   ec.use_graph = cfg->use_graph;
   ec.max_group = cfg->max_group;
   ec.stack_nz = cfg->stack_nz;
@@ -593,22 +596,25 @@ RTM_EXPORT int rtm_batch_create(rtm_batch** out, const rtm_batch_cfg_t* cfg, int
   }
   b->nshots = nshots;
   b->nt = cfg->nt;
-  if (tune) STCHK(autotune_txr(ec, sp, cf, wavelet, &ec.txr, nullptr, true));
+  if (tune) STCHK(autotune_txr(ec, sp, cf, wavelet, &ec.txr, &ec.tblock, nullptr, true));
   b->txr = ec.txr;
+  b->tblock = ec.tblock;
   STCHK(b->eng.setup(ec, sp, cf, wavelet));
   *out = b.release();
   return OK;
 }
 
 RTM_EXPORT int rtm_batch_autotune(const rtm_batch_cfg_t* cfg, int nshots, const rtm_batch_shot_t* shots,
-                                  const float* const* coef5, const float* wavelet, int* txr, double* ms4) {
-  if (!cfg || !shots || !coef5 || !wavelet || !txr || nshots < 1) { set_error("bad argument"); return E_ARG; }
+                                  const float* const* coef5, const float* wavelet, int* txr, int* tblock,
+                                  double* ms8) {
+  if (!cfg || !shots || !coef5 || !wavelet || !txr || !tblock || nshots < 1) { set_error("bad argument"); return E_ARG; }
   EngineCfg ec;
   ec.nt = cfg->nt; ec.dx = cfg->dx; ec.dt = cfg->dt; ec.abc = cfg->abc; ec.adj_abc = cfg->adj_abc;
   ec.snap_every = cfg->snap_every < 1 ? 1 : cfg->snap_every;
   ec.snap_fmt = snap_fmt_of(cfg->snap_prec, ec.snap_every);
   ec.comp = cfg->comp_image;
   ec.xcd = default_xcd();
+  ec.tblock = default_tblock();
   ec.use_graph = cfg->use_graph;
   std::vector<ShotSpec> sp(nshots);
   std::vector<const float*> cf(coef5, coef5 + nshots);
@@ -616,10 +622,11 @@ RTM_EXPORT int rtm_batch_autotune(const rtm_batch_cfg_t* cfg, int nshots, const 
     const rtm_batch_shot_t& s = shots[i];
     sp[i] = ShotSpec{s.nz, s.nx, s.xs, s.zs, s.zr, s.rx0, s.nrx, 0, 0, 0, 0, 0, 0};
   }
-  return autotune_txr(ec, sp, cf, wavelet, txr, ms4, false);
+  return autotune_txr(ec, sp, cf, wavelet, txr, tblock, ms8, false);
 }
 
 RTM_EXPORT int rtm_batch_txr(rtm_batch* b) { return b ? b->txr : 0; }
+RTM_EXPORT int rtm_batch_tblock(rtm_batch* b) { return b ? b->tblock : 0; }
 
 RTM_EXPORT int rtm_batch_set_true_model(rtm_batch* b, const float* const* coef5_true) {
   if (!b || !coef5_true) { set_error("bad argument"); return E_ARG; }

@@ -39,19 +39,19 @@ struct Tile {
 template <int S>
 __device__ __forceinline__ float sp(const float* f, int p) {
   float r = C1 * (f[p + 5 * S] - f[p - 4 * S]);
-  r += C2 * (f[p + 4 * S] - f[p - 3 * S]);
-  r += C3 * (f[p + 3 * S] - f[p - 2 * S]);
-  r += C4 * (f[p + 2 * S] - f[p - S]);
-  r += C5 * (f[p + S] - f[p]);
+  r = fmaf(C2, f[p + 4 * S] - f[p - 3 * S], r);
+  r = fmaf(C3, f[p + 3 * S] - f[p - 2 * S], r);
+  r = fmaf(C4, f[p + 2 * S] - f[p - S], r);
+  r = fmaf(C5, f[p + S] - f[p], r);
   return r;
 }
 template <int S>
 __device__ __forceinline__ float sm(const float* f, int p) {
   float r = C1 * (f[p + 4 * S] - f[p - 5 * S]);
-  r += C2 * (f[p + 3 * S] - f[p - 4 * S]);
-  r += C3 * (f[p + 2 * S] - f[p - 3 * S]);
-  r += C4 * (f[p + S] - f[p - 2 * S]);
-  r += C5 * (f[p] - f[p - S]);
+  r = fmaf(C2, f[p + 3 * S] - f[p - 4 * S], r);
+  r = fmaf(C3, f[p + 2 * S] - f[p - 3 * S], r);
+  r = fmaf(C4, f[p + S] - f[p - 2 * S], r);
+  r = fmaf(C5, f[p] - f[p - S], r);
   return r;
 }
 
@@ -107,7 +107,9 @@ struct LevelPtr {
   float* p;
 };
 __device__ __forceinline__ float* lev_ptr(float* lev, long long stride, int mod, int off, int L) {
-  return lev + (long long)((L + off) % mod) * stride;
+  int slot = (L + off) % mod;
+  if (slot < 0) slot += mod;  // levels before the first one map into the ring too
+  return lev + (long long)slot * stride;
 }
 
 // ---------------------------------------------------------------------------
@@ -209,21 +211,21 @@ __global__ __launch_bounds__(256) void k_fwd(const ShotDev* __restrict__ shots,
     const float dvz_zp = sp<1>(sh[1], p);   // E, G
     const float dvx_zm = sm<1>(sh[0], p);   // H
     const float dvz_xm = sm<SX>(sh[1], p);  // J
-    v[0] = v[0] + m1x * dxx_xm;
-    v[0] = v[0] + m1x * dxz_zp;
-    v[1] = v[1] + m1z * dxz_xp;
-    v[1] = v[1] + m1z * dzz_zm;
+    v[0] = fmaf(m1x, dxx_xm, v[0]);
+    v[0] = fmaf(m1x, dxz_zp, v[0]);
+    v[1] = fmaf(m1z, dxz_xp, v[1]);
+    v[1] = fmaf(m1z, dzz_zm, v[1]);
     if ((flags & F_SRC) && gz == S.zs && gx == S.xs) {
       const float amp = S.src_amp[L - 1];
       v[2] = v[2] + amp;
       v[3] = v[3] + amp;
     }
-    v[2] = v[2] + m2m3 * dvx_xp;
-    v[2] = v[2] + m2 * dvz_zp;
-    v[3] = v[3] + m2 * dvx_xp;
-    v[3] = v[3] + m2m3 * dvz_zp;
-    v[4] = v[4] + m3 * dvx_zm;
-    v[4] = v[4] + m3 * dvz_xm;
+    v[2] = fmaf(m2m3, dvx_xp, v[2]);
+    v[2] = fmaf(m2, dvz_zp, v[2]);
+    v[3] = fmaf(m2, dvx_xp, v[3]);
+    v[3] = fmaf(m2m3, dvz_zp, v[3]);
+    v[4] = fmaf(m3, dvx_zm, v[4]);
+    v[4] = fmaf(m3, dvz_xm, v[4]);
     if (flags & F_K) {
       const float kx = S.kx[gx];
 #pragma unroll
@@ -293,6 +295,296 @@ __global__ __launch_bounds__(256) void k_fwd(const ShotDev* __restrict__ shots,
         }
       }
     }
+  }
+}
+
+// Store one decimated snapshot k0 of the thread's TXR cells: sv = Vx, Vz,
+// dVx, dVz (fp32, or bf16/fp16 with one max-abs scale per 64 x TXR strip of
+// each array, reduced over the wave).
+template <int TXR, int SNAP>
+__device__ __forceinline__ void store_snap(const ShotDev& S, long long k0, int z0, int x0, int tz, int ty,
+                                           bool zok, int gz, int nx, int ldz, int n2, const float (&sv)[4][TXR]) {
+  if (SNAP == SNAP_F32) {
+    float* sp4 = static_cast<float*>(S.snap) + k0 * 4 * n2;
+#pragma unroll
+    for (int j = 0; j < TXR; ++j) {
+      const int gx = x0 + ty * TXR + j;
+      if (!zok || gx >= nx) continue;
+      const int gi = gz + ldz * gx;
+#pragma unroll
+      for (int a = 0; a < 4; ++a) sp4[a * n2 + gi] = sv[a][j];
+    }
+  } else {
+    unsigned short* q4 = static_cast<unsigned short*>(S.snap) + k0 * 4 * n2;
+    const int strip = (z0 >> 6) + S.snap_nbz * ((x0 >> S.snap_sw) + ty);
+#pragma unroll
+    for (int a = 0; a < 4; ++a) {
+      float m = 0.f;
+#pragma unroll
+      for (int j = 0; j < TXR; ++j) m = fmaxf(m, fabsf(sv[a][j]));
+      m = wave_max(m);
+      const float inv = m > kSnapFloor ? SnapQ<SNAP>::Q / m : 0.f;
+      if (tz == 0) S.snap_scale[(k0 * 4 + a) * S.snap_nstrip + strip] = m / SnapQ<SNAP>::Q;
+#pragma unroll
+      for (int j = 0; j < TXR; ++j) {
+        const int gx = x0 + ty * TXR + j;
+        if (!zok || gx >= nx) continue;
+        q4[a * n2 + gz + ldz * gx] = SnapQ<SNAP>::enc(sv[a][j] * inv);
+      }
+    }
+  }
+}
+
+// Two forward levels per pass (temporal blocking): levels L and L+1 of a
+// 64 x 4*TXR tile from levels L-1 and L-2. Level L-1 is staged in LDS with a
+// radius-10 halo; level L is computed on the tile plus a radius-5 halo
+// (cells outside the grid are zero, as a staged halo would be), kept in LDS
+// in place of L-1, and level L+1 is computed on the tile from it. Each cell
+// of L and L+1 gets exactly the arithmetic of k_fwd, so results are bitwise
+// those of two single-level steps. Per two cell-updates HBM sees 5 fields of
+// L-1, 5 of L-2, 5 coefficients and 10 written fields (100 B instead of
+// 160 B); halo re-reads come from L2. The ring must hold four levels (L and
+// L+1 are written to other slots than L-1 and L-2, which neighbouring tiles
+// still read). Flags: F_K, F_SRC, F_REC / F_RECSUB, F_NTS apply to both
+// levels; F_SNAP writes the snapshot of level L-1 (SNAP_HIST: the history
+// of both levels), F_SNAP2 the snapshot of level L. No Born sources.
+template <int TXR>
+struct Tile2 {
+  static constexpr int TZ = TILE_Z, TX = TILE_WY * TXR, H = 5;
+  static constexpr int LZ1 = TZ + 4 * H, LX1 = TX + 4 * H, N1 = LZ1 * LX1;  // level L-1
+  static constexpr int LZ0 = TZ + 2 * H, LX0 = TX + 2 * H, N0 = LZ0 * LX0;  // level L
+};
+
+template <int TXR, int SNAP>
+__global__ __launch_bounds__(256) void k_fwd2(const ShotDev* __restrict__ shots,
+                                              const TileRef* __restrict__ tiles, int ntiles, int L,
+                                              int flags, int xcd) {
+  using T = Tile2<TXR>;
+  constexpr int NIT1 = (T::N1 + 255) / 256;
+  constexpr int NIT0 = (T::N0 + 255) / 256;
+  constexpr int H = T::H;
+  __shared__ float sh[5][T::N1];
+  const TileRef tr = tiles[xcd ? xcd_tile(blockIdx.x, ntiles) : (int)blockIdx.x];
+  const ShotDev& S = shots[tr.shot];
+  const int nz = S.nz, nx = S.nx, ldz = S.ldz;
+  const int fs = (int)S.fstride, n2 = (int)S.n2;
+  float* const out0 = lev_ptr(S.lev, S.lev_stride, S.lev_mod, S.lev_off, L);
+  float* const out1 = lev_ptr(S.lev, S.lev_stride, S.lev_mod, S.lev_off, L + 1);
+  const float* const x2 = lev_ptr(S.lev, S.lev_stride, S.lev_mod, S.lev_off, L - 2);
+  const float* const y1 = lev_ptr(S.lev, S.lev_stride, S.lev_mod, S.lev_off, L - 1);
+  const float* const coef = S.coef;
+  const int z0 = tr.z0, x0 = tr.x0;
+  const int tz = threadIdx.x, ty = threadIdx.y;
+  const int tid = tz + T::TZ * ty;
+
+  // all global loads first: level L-1 with a radius-10 halo, then level L-2
+  // and the coefficients of the radius-5 extended tile
+  float st[NIT1][5];
+#pragma unroll
+  for (int k = 0; k < NIT1; ++k) {
+    const int c = tid + 256 * k;
+    const int lx = c / T::LZ1, lz = c - lx * T::LZ1;
+    const int gx = x0 - 2 * H + lx, gz = z0 - 2 * H + lz;
+    const bool in = c < T::N1 && gx >= 0 && gx < nx && gz >= 0 && gz < nz;
+    const int gi = gz + ldz * gx;
+#pragma unroll
+    for (int f = 0; f < 5; ++f) st[k][f] = in ? y1[f * fs + gi] : 0.f;
+  }
+  float ex[NIT0][5], ec[NIT0][5];
+#pragma unroll
+  for (int k = 0; k < NIT0; ++k) {
+    const int c = tid + 256 * k;
+    const int lx = c / T::LZ0, lz = c - lx * T::LZ0;
+    const int gx = x0 - H + lx, gz = z0 - H + lz;
+    const bool in = c < T::N0 && gx >= 0 && gx < nx && gz >= 0 && gz < nz;
+    const int gi = gz + ldz * gx;
+#pragma unroll
+    for (int f = 0; f < 5; ++f) {
+      ex[k][f] = in ? x2[f * fs + gi] : 0.f;
+      ec[k][f] = in ? coef[f * n2 + gi] : 0.f;
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < NIT1; ++k) {
+    const int c = tid + 256 * k;
+    if (c < T::N1) {
+#pragma unroll
+      for (int f = 0; f < 5; ++f) sh[f][c] = st[k][f];
+    }
+  }
+  __syncthreads();
+
+  // level L on the extended tile (registers)
+  constexpr int SX1 = T::LZ1;
+  float vl[NIT0][5];
+#pragma unroll
+  for (int k = 0; k < NIT0; ++k) {
+    const int c = tid + 256 * k;
+    const int lx = c / T::LZ0, lz = c - lx * T::LZ0;
+    const int gx = x0 - H + lx, gz = z0 - H + lz;
+    const bool in = c < T::N0 && gx >= 0 && gx < nx && gz >= 0 && gz < nz;
+#pragma unroll
+    for (int f = 0; f < 5; ++f) vl[k][f] = 0.f;
+    if (!in) continue;
+    const int p = (lx + H) * T::LZ1 + (lz + H);
+    const float m1x = ec[k][0], m1z = ec[k][1], m2 = ec[k][2], m3 = ec[k][3], m2m3 = ec[k][4];
+    float v[5];
+#pragma unroll
+    for (int f = 0; f < 5; ++f) v[f] = ex[k][f];
+    const float dxx_xm = sm<SX1>(sh[2], p);
+    const float dxz_zp = sp<1>(sh[4], p);
+    const float dxz_xp = sp<SX1>(sh[4], p);
+    const float dzz_zm = sm<1>(sh[3], p);
+    const float dvx_xp = sp<SX1>(sh[0], p);
+    const float dvz_zp = sp<1>(sh[1], p);
+    const float dvx_zm = sm<1>(sh[0], p);
+    const float dvz_xm = sm<SX1>(sh[1], p);
+    v[0] = fmaf(m1x, dxx_xm, v[0]);
+    v[0] = fmaf(m1x, dxz_zp, v[0]);
+    v[1] = fmaf(m1z, dxz_xp, v[1]);
+    v[1] = fmaf(m1z, dzz_zm, v[1]);
+    if ((flags & F_SRC) && gz == S.zs && gx == S.xs) {
+      const float amp = S.src_amp[L - 1];
+      v[2] = v[2] + amp;
+      v[3] = v[3] + amp;
+    }
+    v[2] = fmaf(m2m3, dvx_xp, v[2]);
+    v[2] = fmaf(m2, dvz_zp, v[2]);
+    v[3] = fmaf(m2, dvx_xp, v[3]);
+    v[3] = fmaf(m2m3, dvz_zp, v[3]);
+    v[4] = fmaf(m3, dvx_zm, v[4]);
+    v[4] = fmaf(m3, dvz_xm, v[4]);
+    if (flags & F_K) {
+      const float kz = S.kz[gz], kx = S.kx[gx];
+#pragma unroll
+      for (int f = 0; f < 5; ++f) v[f] = (v[f] * kz) * kx;
+    }
+#pragma unroll
+    for (int f = 0; f < 5; ++f) vl[k][f] = v[f];
+  }
+  // own cells of level L-1 (the L-2 operand of level L+1) before the tile
+  // is overwritten
+  const int gz = z0 + tz;
+  const bool zok = gz < nz;
+  float om[TXR][5];
+#pragma unroll
+  for (int j = 0; j < TXR; ++j) {
+    const int p = (ty * TXR + j + 2 * H) * T::LZ1 + (tz + 2 * H);
+#pragma unroll
+    for (int f = 0; f < 5; ++f) om[j][f] = sh[f][p];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < NIT0; ++k) {
+    const int c = tid + 256 * k;
+    if (c < T::N0) {
+#pragma unroll
+      for (int f = 0; f < 5; ++f) sh[f][c] = vl[k][f];
+    }
+  }
+  __syncthreads();
+
+  // level L+1 on the tile; level L of the tile written from LDS
+  constexpr int SX0 = T::LZ0;
+  const float kz = ((flags & F_K) && zok) ? S.kz[gz] : 1.f;
+  float sa[4][TXR], sb[4][TXR];
+#pragma unroll
+  for (int j = 0; j < TXR; ++j) {
+    const int lxo = ty * TXR + j;
+    const int gx = x0 + lxo;
+    const bool ok = zok && gx < nx;
+    const int p = (lxo + H) * T::LZ0 + (tz + H);
+#pragma unroll
+    for (int a = 0; a < 4; ++a) sa[a][j] = sb[a][j] = 0.f;
+    if (!ok) continue;
+    const int gi = gz + ldz * gx;
+    float cur[5];
+#pragma unroll
+    for (int f = 0; f < 5; ++f) cur[f] = sh[f][p];
+    const float m1x = coef[gi], m1z = coef[n2 + gi], m2 = coef[2 * n2 + gi], m3 = coef[3 * n2 + gi],
+                m2m3 = coef[4 * n2 + gi];
+    float v[5];
+#pragma unroll
+    for (int f = 0; f < 5; ++f) v[f] = om[j][f];
+    const float dxx_xm = sm<SX0>(sh[2], p);
+    const float dxz_zp = sp<1>(sh[4], p);
+    const float dxz_xp = sp<SX0>(sh[4], p);
+    const float dzz_zm = sm<1>(sh[3], p);
+    const float dvx_xp = sp<SX0>(sh[0], p);
+    const float dvz_zp = sp<1>(sh[1], p);
+    const float dvx_zm = sm<1>(sh[0], p);
+    const float dvz_xm = sm<SX0>(sh[1], p);
+    v[0] = fmaf(m1x, dxx_xm, v[0]);
+    v[0] = fmaf(m1x, dxz_zp, v[0]);
+    v[1] = fmaf(m1z, dxz_xp, v[1]);
+    v[1] = fmaf(m1z, dzz_zm, v[1]);
+    if ((flags & F_SRC) && gz == S.zs && gx == S.xs) {
+      const float amp = S.src_amp[L];
+      v[2] = v[2] + amp;
+      v[3] = v[3] + amp;
+    }
+    v[2] = fmaf(m2m3, dvx_xp, v[2]);
+    v[2] = fmaf(m2, dvz_zp, v[2]);
+    v[3] = fmaf(m2, dvx_xp, v[3]);
+    v[3] = fmaf(m2m3, dvz_zp, v[3]);
+    v[4] = fmaf(m3, dvx_zm, v[4]);
+    v[4] = fmaf(m3, dvz_xm, v[4]);
+    if (flags & F_K) {
+      const float kx = S.kx[gx];
+#pragma unroll
+      for (int f = 0; f < 5; ++f) v[f] = (v[f] * kz) * kx;
+    }
+    if (flags & F_NTS) {
+#pragma unroll
+      for (int f = 0; f < 5; ++f) {
+        __builtin_nontemporal_store(cur[f], out0 + f * fs + gi);
+        __builtin_nontemporal_store(v[f], out1 + f * fs + gi);
+      }
+    } else {
+#pragma unroll
+      for (int f = 0; f < 5; ++f) {
+        out0[f * fs + gi] = cur[f];
+        out1[f * fs + gi] = v[f];
+      }
+    }
+    if ((flags & (F_REC | F_RECSUB)) && gz == S.zr && gx >= S.rx0 && gx < S.rx0 + S.nrx) {
+      const long long rs = (long long)S.nt * S.nrx;
+      float* r0 = S.rec + (long long)(L - 1) * S.nrx + (gx - S.rx0);
+      float* r1 = r0 + S.nrx;
+#pragma unroll
+      for (int f = 0; f < 5; ++f) {
+        if (flags & F_RECSUB) {
+          r0[f * rs] = r0[f * rs] - cur[f];
+          r1[f * rs] = r1[f * rs] - v[f];
+        } else {
+          r0[f * rs] = cur[f];
+          r1[f * rs] = v[f];
+        }
+      }
+    }
+    if (SNAP == SNAP_HIST && (flags & F_SNAP)) {
+      float* h = static_cast<float*>(S.snap) + (long long)(L - 1) * 2 * (long long)n2;
+      h[gi] = cur[0];
+      h[n2 + gi] = cur[1];
+      h[2 * n2 + gi] = v[0];
+      h[3 * n2 + gi] = v[1];
+    }
+    if (SNAP != SNAP_HIST) {
+      sa[0][j] = om[j][0];
+      sa[1][j] = om[j][1];
+      sa[2][j] = cur[0] - om[j][0];
+      sa[3][j] = cur[1] - om[j][1];
+      sb[0][j] = cur[0];
+      sb[1][j] = cur[1];
+      sb[2][j] = v[0] - cur[0];
+      sb[3][j] = v[1] - cur[1];
+    }
+  }
+  if (SNAP != SNAP_HIST) {
+    // snapshot of level L-1 (flag set by the host when (L-1) % j == 0) and of
+    // level L (F_SNAP2, when L % j == 0)
+    if (flags & F_SNAP) store_snap<TXR, SNAP>(S, (L - 1) / S.snap_every - 1, z0, x0, tz, ty, zok, gz, nx, ldz, n2, sa);
+    if (flags & F_SNAP2) store_snap<TXR, SNAP>(S, L / S.snap_every - 1, z0, x0, tz, ty, zok, gz, nx, ldz, n2, sb);
   }
 }
 
@@ -440,8 +732,8 @@ __global__ __launch_bounds__(256) void k_adj(const ShotDev* __restrict__ shots,
                   xz = y1[4 * fs + gi];
       const float m1x = coef[gi], m1z = coef[n2 + gi], m2 = coef[2 * n2 + gi],
                   m3 = coef[3 * n2 + gi], m2m3 = coef[4 * n2 + gi];
-      p0 = m2m3 * xx + m2 * zz;
-      p1 = m2 * xx + m2m3 * zz;
+      p0 = fmaf(m2m3, xx, m2 * zz);
+      p1 = fmaf(m2, xx, m2m3 * zz);
       p2 = m3 * xz;
       p3 = m1x * vx;
       p4 = m1z * vz;
@@ -575,13 +867,13 @@ __global__ __launch_bounds__(256) void k_adj(const ShotDev* __restrict__ shots,
     }
     if (zin5 && gx >= 5 && gx <= nx - 6) {
       s6 = sm<1>(sh[0], p) * inv_dx;
-      s6 += sm<SX>(sh[1], p) * inv_dx;
+      s6 = fmaf(sm<SX>(sh[1], p), inv_dx, s6);
     }
     const float s3 = s4 + s5;
     const float* v = o[j];
-    const float t1 = w * (v[0] * s1 + v[1] * s2);
-    const float t2 = -w * (v[2] * s3 + v[3] * s3);
-    const float t3 = -w * (v[2] * 2.f * s4 + v[3] * 2.f * s5 + v[4] * s6);
+    const float t1 = w * fmaf(v[0], s1, v[1] * s2);
+    const float t2 = -w * fmaf(v[2], s3, v[3] * s3);
+    const float t3 = -w * fmaf(v[4], s6, fmaf(v[3] * 2.f, s5, v[2] * 2.f * s4));
     if (flags & A_COMP) {
       float s = img[gi], c = imgc[gi];
       neumaier(s, c, t1);
@@ -602,6 +894,345 @@ __global__ __launch_bounds__(256) void k_adj(const ShotDev* __restrict__ shots,
       img[n2 + gi] += t2;
       img[2 * n2 + gi] += t3;
     }
+  }
+}
+
+
+// Imaging condition of one adjoint level at the thread's TXR cells: o holds
+// the adjoint fields of the level, the source Vx, Vz of the level are staged
+// in s0 / s1 (LDS, stride LZ, radius-5 halo).
+template <int TXR, int SNAP, int LZ>
+__device__ __forceinline__ void image_cells(const ShotDev& S, const SrcView<SNAP>& sv, const float* s0, const float* s1,
+                                            const float (&o)[TXR][5], bool next, bool comp, int z0, int x0, int tz,
+                                            int ty, int nz, int nx, int ldz, int n2) {
+  constexpr int H = 5;
+  const int gz = z0 + tz;
+  if (gz >= nz) return;
+  const bool zin4 = gz >= 4 && gz <= nz - 6, zin5 = gz >= 5 && gz <= nz - 6;
+  float* const img = S.img;
+  float* const imgc = S.imgc;
+  const float w = S.img_w, inv_dx = S.inv_dx, inv_dt = S.inv_dt;
+#pragma unroll
+  for (int j = 0; j < TXR; ++j) {
+    const int lxo = ty * TXR + j;
+    const int gx = x0 + lxo;
+    if (gx >= nx) continue;
+    const int p = (lxo + H) * LZ + (tz + H);
+    const int gi = gz + ldz * gx;
+    float s1v = 0.f, s2v = 0.f;
+    if (next) {
+      if (SNAP == SNAP_HIST) {
+        s1v = inv_dt * (sv.get(2, gi, gz, gx) - s0[p]);
+        s2v = inv_dt * (sv.get(3, gi, gz, gx) - s1[p]);
+      } else {
+        s1v = inv_dt * sv.get(2, gi, gz, gx);
+        s2v = inv_dt * sv.get(3, gi, gz, gx);
+      }
+    }
+    float s4 = 0.f, s5 = 0.f, s6 = 0.f;
+    if (zin4 && gx >= 4 && gx <= nx - 6) {
+      s4 = sp<LZ>(s0, p) * inv_dx;
+      s5 = sp<1>(s1, p) * inv_dx;
+    }
+    if (zin5 && gx >= 5 && gx <= nx - 6) {
+      s6 = sm<1>(s0, p) * inv_dx;
+      s6 = fmaf(sm<LZ>(s1, p), inv_dx, s6);
+    }
+    const float s3 = s4 + s5;
+    const float* v = o[j];
+    const float t1 = w * fmaf(v[0], s1v, v[1] * s2v);
+    const float t2 = -w * fmaf(v[2], s3, v[3] * s3);
+    const float t3 = -w * fmaf(v[4], s6, fmaf(v[3] * 2.f, s5, v[2] * 2.f * s4));
+    if (comp) {
+      float s = img[gi], c = imgc[gi];
+      neumaier(s, c, t1);
+      img[gi] = s;
+      imgc[gi] = c;
+      s = img[n2 + gi];
+      c = imgc[n2 + gi];
+      neumaier(s, c, t2);
+      img[n2 + gi] = s;
+      imgc[n2 + gi] = c;
+      s = img[2 * n2 + gi];
+      c = imgc[2 * n2 + gi];
+      neumaier(s, c, t3);
+      img[2 * n2 + gi] = s;
+      imgc[2 * n2 + gi] = c;
+    } else {
+      img[gi] += t1;
+      img[n2 + gi] += t2;
+      img[2 * n2 + gi] += t3;
+    }
+  }
+}
+
+// Stage the source Vx, Vz of imaging level L (radius-5 halo) into s0 / s1.
+template <int TXR, int SNAP>
+__device__ __forceinline__ void stage_source(const SrcView<SNAP>& sv, float* s0, float* s1, int z0, int x0, int tid,
+                                             int nz, int nx, int ldz) {
+  using T = Tile<TXR>;
+  constexpr int NIT = (T::N + 255) / 256;
+  float sx[NIT][2];
+#pragma unroll
+  for (int k = 0; k < NIT; ++k) {
+    const int c = tid + 256 * k;
+    const int lx = c / T::LZ, lz = c - lx * T::LZ;
+    const int gx = x0 - T::H + lx, gz2 = z0 - T::H + lz;
+    const bool in = c < T::N && gx >= 0 && gx < nx && gz2 >= 0 && gz2 < nz;
+    const int gi = gz2 + ldz * gx;
+    sx[k][0] = in ? sv.get(0, gi, gz2, gx) : 0.f;
+    sx[k][1] = in ? sv.get(1, gi, gz2, gx) : 0.f;
+  }
+#pragma unroll
+  for (int k = 0; k < NIT; ++k) {
+    const int c = tid + 256 * k;
+    if (c < T::N) {
+      s0[c] = sx[k][0];
+      s1[c] = sx[k][1];
+    }
+  }
+}
+
+// Two adjoint levels per pass (temporal blocking): levels L and L-1 from
+// levels L+1 and L+2, with the arithmetic of k_adj per cell (bitwise the
+// same results). Coefficient-weighted products of level L+1 are staged with
+// a radius-10 halo; level L is computed on the tile plus a radius-5 halo
+// (receiver injection and sponge included, zero outside the grid), its
+// products replace those of L+1 in LDS, and level L-1 is computed on the
+// tile. The imaging condition of level L (A_IMG) and of level L-1 (A_IMG2)
+// is accumulated in the same pass, in that order. The ring must hold four
+// levels. Flags as k_adj (A_NEXT refers to level L; level L-1 always has a
+// next level); no drop-in history form (A_XIN).
+template <int TXR>
+struct Adj2Lds {
+  using T = Tile2<TXR>;
+  static constexpr int P1 = 5 * T::N1;                 // products of L+1
+  static constexpr int P0 = 5 * T::N0 + 5 * T::TZ * T::TX;  // products of L + raw L of the tile
+  static constexpr int IMG = 2 * Tile<TXR>::N;          // staged source Vx, Vz
+  static constexpr int A = P1 > P0 ? P1 : P0;
+  static constexpr int SIZE = A > IMG ? A : IMG;
+};
+
+template <int TXR, int SNAP>
+__global__ __launch_bounds__(256) void k_adj2(const ShotDev* __restrict__ shots,
+                                              const TileRef* __restrict__ tiles, int ntiles, int L,
+                                              int flags, int xcd) {
+  using T = Tile2<TXR>;
+  constexpr int NIT1 = (T::N1 + 255) / 256;
+  constexpr int NIT0 = (T::N0 + 255) / 256;
+  constexpr int H = T::H;
+  constexpr int NT = T::TZ * T::TX;
+  __shared__ float shm[Adj2Lds<TXR>::SIZE];
+  const TileRef tr = tiles[xcd ? xcd_tile(blockIdx.x, ntiles) : (int)blockIdx.x];
+  const ShotDev& S = shots[tr.shot];
+  const int nz = S.nz, nx = S.nx, ldz = S.ldz;
+  const int fs = (int)S.fstride, n2 = (int)S.n2;
+  float* const out0 = lev_ptr(S.lev, S.lev_stride, S.lev_mod, S.lev_off, L);
+  float* const out1 = lev_ptr(S.lev, S.lev_stride, S.lev_mod, S.lev_off, L - 1);
+  const float* const x2 = lev_ptr(S.lev, S.lev_stride, S.lev_mod, S.lev_off, L + 2);
+  const float* const y1 = lev_ptr(S.lev, S.lev_stride, S.lev_mod, S.lev_off, L + 1);
+  const float* const coef = S.coef;
+  const int z0 = tr.z0, x0 = tr.x0;
+  const int tz = threadIdx.x, ty = threadIdx.y;
+  const int tid = tz + T::TZ * ty;
+  const bool inj = flags & A_INJ;
+  const long long rs = (long long)S.nt * S.nrx;
+
+  // products of level L+1 over the radius-10 tile
+  float st[NIT1][5];
+#pragma unroll
+  for (int k = 0; k < NIT1; ++k) {
+    const int c = tid + 256 * k;
+    const int lx = c / T::LZ1, lz = c - lx * T::LZ1;
+    const int gx = x0 - 2 * H + lx, gz = z0 - 2 * H + lz;
+    const bool in = c < T::N1 && gx >= 0 && gx < nx && gz >= 0 && gz < nz;
+    float p0 = 0.f, p1 = 0.f, p2 = 0.f, p3 = 0.f, p4 = 0.f;
+    if (in) {
+      const int gi = gz + ldz * gx;
+      const float vx = y1[gi], vz = y1[fs + gi], xx = y1[2 * fs + gi], zz = y1[3 * fs + gi],
+                  xz = y1[4 * fs + gi];
+      const float m1x = coef[gi], m1z = coef[n2 + gi], m2 = coef[2 * n2 + gi],
+                  m3 = coef[3 * n2 + gi], m2m3 = coef[4 * n2 + gi];
+      p0 = fmaf(m2m3, xx, m2 * zz);
+      p1 = fmaf(m2, xx, m2m3 * zz);
+      p2 = m3 * xz;
+      p3 = m1x * vx;
+      p4 = m1z * vz;
+    }
+    st[k][0] = p0;
+    st[k][1] = p1;
+    st[k][2] = p2;
+    st[k][3] = p3;
+    st[k][4] = p4;
+  }
+  // level L+2 (+ injection of level L) and the coefficients on the radius-5 tile
+  float eb[NIT0][5], ec[NIT0][5];
+#pragma unroll
+  for (int k = 0; k < NIT0; ++k) {
+    const int c = tid + 256 * k;
+    const int lx = c / T::LZ0, lz = c - lx * T::LZ0;
+    const int gx = x0 - H + lx, gz = z0 - H + lz;
+    const bool in = c < T::N0 && gx >= 0 && gx < nx && gz >= 0 && gz < nz;
+    const int gi = gz + ldz * gx;
+#pragma unroll
+    for (int f = 0; f < 5; ++f) {
+      eb[k][f] = in ? x2[f * fs + gi] : 0.f;
+      ec[k][f] = in ? coef[f * n2 + gi] : 0.f;
+    }
+    if (in && inj && gz == S.zr && gx >= S.rx0 && gx < S.rx0 + S.nrx) {
+      const float* r = S.rec + (long long)(L - 1) * S.nrx + (gx - S.rx0);
+#pragma unroll
+      for (int f = 0; f < 5; ++f) eb[k][f] = r[f * rs] + eb[k][f];
+    }
+  }
+  // level L+1 (+ injection of level L-1) at the tile: b of level L-1
+  const int gz = z0 + tz;
+  const bool zok = gz < nz;
+  float b1[TXR][5];
+#pragma unroll
+  for (int j = 0; j < TXR; ++j) {
+    const int gx = x0 + ty * TXR + j;
+    const bool ok = zok && gx < nx;
+    const int gi = gz + ldz * gx;
+#pragma unroll
+    for (int f = 0; f < 5; ++f) b1[j][f] = ok ? y1[f * fs + gi] : 0.f;
+    if (ok && inj && gz == S.zr && gx >= S.rx0 && gx < S.rx0 + S.nrx) {
+      const float* r = S.rec + (long long)(L - 2) * S.nrx + (gx - S.rx0);
+#pragma unroll
+      for (int f = 0; f < 5; ++f) b1[j][f] = r[f * rs] + b1[j][f];
+    }
+  }
+  float* const sh1[5] = {shm, shm + T::N1, shm + 2 * T::N1, shm + 3 * T::N1, shm + 4 * T::N1};
+#pragma unroll
+  for (int k = 0; k < NIT1; ++k) {
+    const int c = tid + 256 * k;
+    if (c < T::N1) {
+#pragma unroll
+      for (int f = 0; f < 5; ++f) sh1[f][c] = st[k][f];
+    }
+  }
+  __syncthreads();
+
+  // level L on the radius-5 tile
+  constexpr int SX1 = T::LZ1;
+  float vl[NIT0][5];
+#pragma unroll
+  for (int k = 0; k < NIT0; ++k) {
+    const int c = tid + 256 * k;
+    const int lx = c / T::LZ0, lz = c - lx * T::LZ0;
+    const int gx = x0 - H + lx, gzz = z0 - H + lz;
+    const bool in = c < T::N0 && gx >= 0 && gx < nx && gzz >= 0 && gzz < nz;
+#pragma unroll
+    for (int f = 0; f < 5; ++f) vl[k][f] = 0.f;
+    if (!in) continue;
+    const int p = (lx + H) * T::LZ1 + (lz + H);
+    float v0 = eb[k][0] - sm<SX1>(sh1[0], p);
+    v0 -= sp<1>(sh1[2], p);
+    float v1 = eb[k][1] - sm<1>(sh1[1], p);
+    v1 -= sp<SX1>(sh1[2], p);
+    const float v2 = eb[k][2] - sp<SX1>(sh1[3], p);
+    const float v3 = eb[k][3] - sp<1>(sh1[4], p);
+    float v4 = eb[k][4] - sm<1>(sh1[3], p);
+    v4 -= sm<SX1>(sh1[4], p);
+    float v[5] = {v0, v1, v2, v3, v4};
+    if (flags & A_K) {
+      const float kz = S.kz[gzz], kx = S.kx[gx];
+#pragma unroll
+      for (int f = 0; f < 5; ++f) v[f] = (v[f] * kz) * kx;
+    }
+#pragma unroll
+    for (int f = 0; f < 5; ++f) vl[k][f] = v[f];
+  }
+  __syncthreads();
+  // products of level L (radius-5 tile) and the raw level L of the tile
+  float* const sh0[5] = {shm, shm + T::N0, shm + 2 * T::N0, shm + 3 * T::N0, shm + 4 * T::N0};
+  float* const raw = shm + 5 * T::N0;  // [f][NT], tile cell (lxo, tz) at lxo*TZ + tz
+#pragma unroll
+  for (int k = 0; k < NIT0; ++k) {
+    const int c = tid + 256 * k;
+    if (c < T::N0) {
+      const float m1x = ec[k][0], m1z = ec[k][1], m2 = ec[k][2], m3 = ec[k][3], m2m3 = ec[k][4];
+      const float vx = vl[k][0], vz = vl[k][1], xx = vl[k][2], zz = vl[k][3], xz = vl[k][4];
+      sh0[0][c] = fmaf(m2m3, xx, m2 * zz);
+      sh0[1][c] = fmaf(m2, xx, m2m3 * zz);
+      sh0[2][c] = m3 * xz;
+      sh0[3][c] = m1x * vx;
+      sh0[4][c] = m1z * vz;
+      const int lx = c / T::LZ0, lz = c - lx * T::LZ0;
+      const int lxo = lx - H, lzo = lz - H;
+      if (lxo >= 0 && lxo < T::TX && lzo >= 0 && lzo < T::TZ) {
+#pragma unroll
+        for (int f = 0; f < 5; ++f) raw[f * NT + lxo * T::TZ + lzo] = vl[k][f];
+      }
+    }
+  }
+  __syncthreads();
+
+  // level L-1 on the tile; level L of the tile written from LDS
+  constexpr int SX0 = T::LZ0;
+  const float kz = ((flags & A_K) && zok) ? S.kz[gz] : 1.f;
+  float oa[TXR][5], ob[TXR][5];
+#pragma unroll
+  for (int j = 0; j < TXR; ++j) {
+    const int lxo = ty * TXR + j;
+    const int gx = x0 + lxo;
+    const bool ok = zok && gx < nx;
+    const int p = (lxo + H) * T::LZ0 + (tz + H);
+#pragma unroll
+    for (int f = 0; f < 5; ++f) oa[j][f] = ob[j][f] = 0.f;
+    if (!ok) continue;
+    const int gi = gz + ldz * gx;
+#pragma unroll
+    for (int f = 0; f < 5; ++f) oa[j][f] = raw[f * NT + lxo * T::TZ + tz];
+    float v0 = b1[j][0] - sm<SX0>(sh0[0], p);
+    v0 -= sp<1>(sh0[2], p);
+    float v1 = b1[j][1] - sm<1>(sh0[1], p);
+    v1 -= sp<SX0>(sh0[2], p);
+    const float v2 = b1[j][2] - sp<SX0>(sh0[3], p);
+    const float v3 = b1[j][3] - sp<1>(sh0[4], p);
+    float v4 = b1[j][4] - sm<1>(sh0[3], p);
+    v4 -= sm<SX0>(sh0[4], p);
+    float v[5] = {v0, v1, v2, v3, v4};
+    if (flags & A_K) {
+      const float kx = S.kx[gx];
+#pragma unroll
+      for (int f = 0; f < 5; ++f) v[f] = (v[f] * kz) * kx;
+    }
+    if (flags & A_NTS) {
+#pragma unroll
+      for (int f = 0; f < 5; ++f) {
+        __builtin_nontemporal_store(oa[j][f], out0 + f * fs + gi);
+        __builtin_nontemporal_store(v[f], out1 + f * fs + gi);
+      }
+    } else {
+#pragma unroll
+      for (int f = 0; f < 5; ++f) {
+        out0[f * fs + gi] = oa[j][f];
+        out1[f * fs + gi] = v[f];
+      }
+    }
+#pragma unroll
+    for (int f = 0; f < 5; ++f) ob[j][f] = v[f];
+  }
+
+  // imaging of level L, then of level L-1 (uniform flags over the launch)
+  const bool comp = flags & A_COMP;
+  float* const s0 = shm;
+  float* const s1 = shm + Tile<TXR>::N;
+  if (flags & A_IMG) {
+    const SrcView<SNAP> sv(S, L);
+    __syncthreads();
+    stage_source<TXR, SNAP>(sv, s0, s1, z0, x0, tid, nz, nx, ldz);
+    __syncthreads();
+    image_cells<TXR, SNAP, Tile<TXR>::LZ>(S, sv, s0, s1, oa, (flags & A_NEXT) != 0, comp, z0, x0, tz, ty, nz, nx,
+                                         ldz, n2);
+  }
+  if (flags & A_IMG2) {
+    const SrcView<SNAP> sv(S, L - 1);
+    __syncthreads();
+    stage_source<TXR, SNAP>(sv, s0, s1, z0, x0, tid, nz, nx, ldz);
+    __syncthreads();
+    image_cells<TXR, SNAP, Tile<TXR>::LZ>(S, sv, s0, s1, ob, true, comp, z0, x0, tz, ty, nz, nx, ldz, n2);
   }
 }
 
@@ -652,15 +1283,15 @@ __global__ __launch_bounds__(256) void k_born_sources(const ShotDev* __restrict_
     }
     if (gz >= 5 && gz <= nz - 6 && gx >= 5 && gx <= nx - 6) {
       s6 = sm<1>(sh[0], p) * S.inv_dx;
-      s6 += sm<SX>(sh[1], p) * S.inv_dx;
+      s6 = fmaf(sm<SX>(sh[1], p), S.inv_dx, s6);
     }
     const float s3 = s4 + s5;
     const float d1 = S.dm[gi], d2 = S.dm[n2 + gi], d3 = S.dm[2 * n2 + gi];
     float* src = S.bsrc;
     src[gi] = d1 * s1;
     src[n2 + gi] = d1 * s2;
-    src[2 * n2 + gi] = -(d2 * s3) - (d3 * 2.f * s4);
-    src[3 * n2 + gi] = -(d2 * s3) - (d3 * 2.f * s5);
+    src[2 * n2 + gi] = fmaf(-(d3 * 2.f), s4, -(d2 * s3));
+    src[3 * n2 + gi] = fmaf(-(d3 * 2.f), s5, -(d2 * s3));
     src[4 * n2 + gi] = -(d3 * s6);
   }
 }
@@ -710,6 +1341,17 @@ static void fwd_dispatch(int snap, dim3 g, hipStream_t st, const ShotDev* s, con
   }
 }
 template <int TXR>
+static void fwd2_dispatch(int snap, dim3 g, hipStream_t st, const ShotDev* s, const TileRef* t,
+                          int n, int L, int fl, int xcd) {
+  const dim3 b(TILE_Z, TILE_WY);
+  switch (snap) {
+    case SNAP_HIST: hipLaunchKernelGGL((k_fwd2<TXR, SNAP_HIST>), g, b, 0, st, s, t, n, L, fl, xcd); break;
+    case SNAP_F32: hipLaunchKernelGGL((k_fwd2<TXR, SNAP_F32>), g, b, 0, st, s, t, n, L, fl, xcd); break;
+    case SNAP_BF16: hipLaunchKernelGGL((k_fwd2<TXR, SNAP_BF16>), g, b, 0, st, s, t, n, L, fl, xcd); break;
+    default: hipLaunchKernelGGL((k_fwd2<TXR, SNAP_F16>), g, b, 0, st, s, t, n, L, fl, xcd); break;
+  }
+}
+template <int TXR>
 static void adj_dispatch(int snap, dim3 g, hipStream_t st, const ShotDev* s, const TileRef* t,
                          int n, int L, int fl, int xcd) {
   const dim3 b(TILE_Z, TILE_WY);
@@ -718,6 +1360,17 @@ static void adj_dispatch(int snap, dim3 g, hipStream_t st, const ShotDev* s, con
     case SNAP_F32: hipLaunchKernelGGL((k_adj<TXR, SNAP_F32>), g, b, 0, st, s, t, n, L, fl, xcd); break;
     case SNAP_BF16: hipLaunchKernelGGL((k_adj<TXR, SNAP_BF16>), g, b, 0, st, s, t, n, L, fl, xcd); break;
     default: hipLaunchKernelGGL((k_adj<TXR, SNAP_F16>), g, b, 0, st, s, t, n, L, fl, xcd); break;
+  }
+}
+template <int TXR>
+static void adj2_dispatch(int snap, dim3 g, hipStream_t st, const ShotDev* s, const TileRef* t,
+                          int n, int L, int fl, int xcd) {
+  const dim3 b(TILE_Z, TILE_WY);
+  switch (snap) {
+    case SNAP_HIST: hipLaunchKernelGGL((k_adj2<TXR, SNAP_HIST>), g, b, 0, st, s, t, n, L, fl, xcd); break;
+    case SNAP_F32: hipLaunchKernelGGL((k_adj2<TXR, SNAP_F32>), g, b, 0, st, s, t, n, L, fl, xcd); break;
+    case SNAP_BF16: hipLaunchKernelGGL((k_adj2<TXR, SNAP_BF16>), g, b, 0, st, s, t, n, L, fl, xcd); break;
+    default: hipLaunchKernelGGL((k_adj2<TXR, SNAP_F16>), g, b, 0, st, s, t, n, L, fl, xcd); break;
   }
 }
 template <int TXR>
@@ -747,6 +1400,21 @@ int launch_fwd(const ShotDev* shots, const TileRef* tiles, int ntiles, int L, in
   return OK;
 }
 
+int launch_fwd2(const ShotDev* shots, const TileRef* tiles, int ntiles, int L, int flags, int txr,
+                int snap, int xcd, hipStream_t st) {
+  if (ntiles <= 0) return OK;
+  if (flags & F_BORN) { set_error("two-level forward step has no Born sources"); return E_ARG; }
+  const dim3 g(ntiles);
+  switch (txr) {
+    case 1: fwd2_dispatch<1>(snap, g, st, shots, tiles, ntiles, L, flags, xcd); break;
+    case 2: fwd2_dispatch<2>(snap, g, st, shots, tiles, ntiles, L, flags, xcd); break;
+    case 4: fwd2_dispatch<4>(snap, g, st, shots, tiles, ntiles, L, flags, xcd); break;
+    default: set_error("two-level forward step: unsupported tile width %d", txr); return E_ARG;
+  }
+  HIPCHK(hipGetLastError());
+  return OK;
+}
+
 int launch_adj(const ShotDev* shots, const TileRef* tiles, int ntiles, int L, int flags, int txr,
                int snap, int xcd, hipStream_t st) {
   if (ntiles <= 0) return OK;
@@ -757,6 +1425,21 @@ int launch_adj(const ShotDev* shots, const TileRef* tiles, int ntiles, int L, in
     case 4: adj_dispatch<4>(snap, g, st, shots, tiles, ntiles, L, flags, xcd); break;
     case 8: adj_dispatch<8>(snap, g, st, shots, tiles, ntiles, L, flags, xcd); break;
     default: set_error("unsupported tile width %d", txr); return E_ARG;
+  }
+  HIPCHK(hipGetLastError());
+  return OK;
+}
+
+int launch_adj2(const ShotDev* shots, const TileRef* tiles, int ntiles, int L, int flags, int txr,
+                int snap, int xcd, hipStream_t st) {
+  if (ntiles <= 0) return OK;
+  if (flags & A_XIN) { set_error("two-level adjoint step has no drop-in history form"); return E_ARG; }
+  const dim3 g(ntiles);
+  switch (txr) {
+    case 1: adj2_dispatch<1>(snap, g, st, shots, tiles, ntiles, L, flags, xcd); break;
+    case 2: adj2_dispatch<2>(snap, g, st, shots, tiles, ntiles, L, flags, xcd); break;
+    case 4: adj2_dispatch<4>(snap, g, st, shots, tiles, ntiles, L, flags, xcd); break;
+    default: set_error("two-level adjoint step: unsupported tile width %d", txr); return E_ARG;
   }
   HIPCHK(hipGetLastError());
   return OK;

@@ -31,6 +31,9 @@ int default_txr() { return env_int("RTM_TXR", 2); }
 int default_xcd() { return env_int("RTM_XCD_REMAP", 1); }
 // Non-temporal stores of the new level (RTM_NT=1).
 static int nt_stores() { return env_int("RTM_NT", 0); }
+// Two levels per launch (temporal blocking) when RTM_TBLOCK=1; the batch
+// autotuner chooses it per workload.
+int default_tblock() { return env_int("RTM_TBLOCK", 0); }
 
 // Sponge taper, W/abc-mod.f90:35-50, and per-row/column factors of K
 // (:59-78): rows first then columns, so corners are damped twice.
@@ -199,7 +202,7 @@ int Engine::alloc_working() {
     }
     r.snap_bytes = (r.snap_bytes + 255) / 256 * 256;
     r.scale_bytes = (r.scale_bytes + 255) / 256 * 256;
-    r.work_bytes = (size_t)10 * r.n2 * sizeof(float) + r.snap_bytes + r.scale_bytes;
+    r.work_bytes = (size_t)5 * kRing * r.n2 * sizeof(float) + r.snap_bytes + r.scale_bytes;
     largest = std::max(largest, r.work_bytes);
     total += r.work_bytes;
   }
@@ -268,7 +271,9 @@ int Engine::fill_desc(int g, bool true_model, bool born_mode) {
     ShotDev d{};
     d.nz = p.nz; d.nx = p.nx; d.ldz = r.ldz; d.nt = cfg_.nt; d.n2 = r.n2;
     d.lev = ring_of(s);
-    d.lev_stride = 5 * r.n2; d.fstride = r.n2; d.lev_mod = 2; d.lev_off = 2;
+    // slot of level L = (L + kRing) % kRing: non-negative for the levels -1
+    // and 0 that the first steps read
+    d.lev_stride = 5 * r.n2; d.fstride = r.n2; d.lev_mod = kRing; d.lev_off = kRing;
     if (true_model && !r.coef_true.p) { set_error("no true model set"); return E_ARG; }
     d.coef = true_model ? r.coef_true.as<float>() : r.coef.as<float>();
     d.kz = r.kfac.p ? r.kfac.as<float>() : nullptr;
@@ -310,15 +315,19 @@ int Engine::enqueue_forward(int g, int mode, hipStream_t st) {
   const TileRef* tiles = d_tiles_.as<TileRef>() + gr.tile0;
   const int snap = (mode == 0) ? cfg_.snap_fmt : SNAP_HIST;
   for (int s : gr.shots) {
-    STCHK(launch_fill(ring_of(s), 10 * res_[s].n2, 0.f, st));
+    STCHK(launch_fill(ring_of(s), (size_t)5 * kRing * res_[s].n2, 0.f, st));
     if ((mode == 0 && cfg_.snap_fmt == SNAP_HIST) || mode == 4)
       STCHK(launch_fill(reinterpret_cast<float*>(snap_of(s)), 2 * res_[s].n2, 0.f, st));  // level 1
   }
   const bool born = mode == 5;
   const int first = born ? 1 : 2;
-  for (int L = first; L <= cfg_.nt; ++L) {
+  // levels L >= 3 go two per launch (temporal blocking) where the kernel
+  // supports the tile width; Born scattering sources are per level
+  const bool pairs = cfg_.tblock && !born && cfg_.txr <= 4;
+  for (int L = first; L <= cfg_.nt;) {
+    const bool two = pairs && L >= 3 && L + 1 <= cfg_.nt;
     int fl = 0;
-    if (born || L >= 3) fl |= F_K;  // level 2 of the source run is the bare injection
+    if (cfg_.abc > 0 && (born || L >= 3)) fl |= F_K;  // level 2 of the source run is the bare injection
     if (!born) fl |= F_SRC;
     if (born) {
       fl |= F_BORN | F_REC;
@@ -329,22 +338,32 @@ int Engine::enqueue_forward(int g, int mode, hipStream_t st) {
     if (mode == 4) fl |= F_SNAP;
     if (nt_stores()) fl |= F_NTS;
     if (mode == 0) {
-      if (cfg_.snap_fmt == SNAP_HIST) fl |= F_SNAP;
-      else if ((L - 1) % cfg_.snap_every == 0) fl |= F_SNAP;
+      if (cfg_.snap_fmt == SNAP_HIST) {
+        fl |= F_SNAP;
+      } else {
+        if ((L - 1) % cfg_.snap_every == 0) fl |= F_SNAP;      // snapshot of level L-1
+        if (two && L % cfg_.snap_every == 0) fl |= F_SNAP2;    // and of level L
+      }
     }
-    STCHK(launch_fwd(desc, tiles, gr.ntiles, L, fl, cfg_.txr, snap, cfg_.xcd, st));
-    if (mode == 3 && cfg_.movie && cfg_.movie_every > 0 && L % cfg_.movie_every == 0 &&
-        L / cfg_.movie_every <= cfg_.movie_n) {
-      const int s = gr.shots[0];
-      const long long k = L / cfg_.movie_every - 1;
-      const float* lev = ring_of(s) + (size_t)(L % 2) * 5 * res_[s].n2;
-      for (int f = 0; f < 5; ++f)
-        HIPCHK(hipMemcpy2DAsync(cfg_.movie + ((size_t)k * 5 + f) * cfg_.movie_nz * cfg_.movie_nx,
-                                cfg_.movie_nz * sizeof(float),
-                                lev + f * res_[s].n2 + cfg_.movie_z0 + (size_t)res_[s].ldz * cfg_.movie_x0,
-                                res_[s].ldz * sizeof(float), cfg_.movie_nz * sizeof(float), cfg_.movie_nx,
-                                hipMemcpyDeviceToDevice, st));
+    if (two)
+      STCHK(launch_fwd2(desc, tiles, gr.ntiles, L, fl, cfg_.txr, snap, cfg_.xcd, st));
+    else
+      STCHK(launch_fwd(desc, tiles, gr.ntiles, L, fl, cfg_.txr, snap, cfg_.xcd, st));
+    for (int Lm = L; Lm <= L + (two ? 1 : 0); ++Lm) {
+      if (mode == 3 && cfg_.movie && cfg_.movie_every > 0 && Lm % cfg_.movie_every == 0 &&
+          Lm / cfg_.movie_every <= cfg_.movie_n) {
+        const int s = gr.shots[0];
+        const long long k = Lm / cfg_.movie_every - 1;
+        const float* lev = ring_of(s) + (size_t)(Lm % kRing) * 5 * res_[s].n2;
+        for (int f = 0; f < 5; ++f)
+          HIPCHK(hipMemcpy2DAsync(cfg_.movie + ((size_t)k * 5 + f) * cfg_.movie_nz * cfg_.movie_nx,
+                                  cfg_.movie_nz * sizeof(float),
+                                  lev + f * res_[s].n2 + cfg_.movie_z0 + (size_t)res_[s].ldz * cfg_.movie_x0,
+                                  res_[s].ldz * sizeof(float), cfg_.movie_nz * sizeof(float), cfg_.movie_nx,
+                                  hipMemcpyDeviceToDevice, st));
+      }
     }
+    L += two ? 2 : 1;
   }
   if (mode == 0 && cfg_.snap_fmt != SNAP_HIST && cfg_.nt % cfg_.snap_every == 0)
     STCHK(launch_snap_last(desc, tiles, gr.ntiles, cfg_.nt, cfg_.txr, cfg_.snap_fmt, st));
@@ -357,18 +376,25 @@ int Engine::enqueue_adjoint(int g, hipStream_t st) {
   const TileRef* tiles = d_tiles_.as<TileRef>() + gr.tile0;
   for (size_t k = 0; k < gr.shots.size(); ++k) {
     Res& r = res_[gr.shots[k]];
-    STCHK(launch_fill(ring_of(gr.shots[k]), 10 * r.n2, 0.f, st));
+    STCHK(launch_fill(ring_of(gr.shots[k]), (size_t)5 * kRing * r.n2, 0.f, st));
     STCHK(launch_fill(r.img.as<float>(), 3 * r.n2, 0.f, st));
     if (cfg_.comp) STCHK(launch_fill(r.imgc.as<float>(), 3 * r.n2, 0.f, st));
   }
-  for (int L = cfg_.nt; L >= 1; --L) {
+  const bool pairs = cfg_.tblock && cfg_.txr <= 4;
+  for (int L = cfg_.nt; L >= 1;) {
+    const bool two = pairs && L - 1 >= 1;
     int fl = A_INJ;
     if (cfg_.adj_abc && cfg_.abc > 0) fl |= A_K;
     if (L % cfg_.snap_every == 0) fl |= A_IMG;
+    if (two && (L - 1) % cfg_.snap_every == 0) fl |= A_IMG2;
     if (L < cfg_.nt) fl |= A_NEXT;
     if (cfg_.comp) fl |= A_COMP;
     if (nt_stores()) fl |= A_NTS;
-    STCHK(launch_adj(desc, tiles, gr.ntiles, L, fl, cfg_.txr, cfg_.snap_fmt, cfg_.xcd, st));
+    if (two)
+      STCHK(launch_adj2(desc, tiles, gr.ntiles, L, fl, cfg_.txr, cfg_.snap_fmt, cfg_.xcd, st));
+    else
+      STCHK(launch_adj(desc, tiles, gr.ntiles, L, fl, cfg_.txr, cfg_.snap_fmt, cfg_.xcd, st));
+    L -= two ? 2 : 1;
   }
   if (cfg_.comp)
     for (size_t k = 0; k < gr.shots.size(); ++k) {
@@ -376,6 +402,22 @@ int Engine::enqueue_adjoint(int g, hipStream_t st) {
       STCHK(launch_image_finalize(r.img.as<float>(), r.imgc.as<float>(), 3 * r.n2, st));
     }
   return OK;
+}
+
+// Step launches of one forward run of a mode / of one adjoint run (the
+// loops of enqueue_forward / enqueue_adjoint).
+long long Engine::fwd_launch_count(int mode) const {
+  const bool born = mode == 5;
+  const bool pairs = cfg_.tblock && !born && cfg_.txr <= 4;
+  long long n = 0;
+  for (int L = born ? 1 : 2; L <= cfg_.nt; ++n) L += (pairs && L >= 3 && L + 1 <= cfg_.nt) ? 2 : 1;
+  return n;
+}
+long long Engine::adj_launch_count() const {
+  const bool pairs = cfg_.tblock && cfg_.txr <= 4;
+  long long n = 0;
+  for (int L = cfg_.nt; L >= 1; ++n) L -= (pairs && L - 1 >= 1) ? 2 : 1;
+  return n;
 }
 
 // which: 0 imaging forward, 1 adjoint+imaging, 2 data synthesis (both runs)
@@ -470,7 +512,7 @@ int Engine::model_data() {
       STCHK(launch_fill(res_[s].rec.as<float>(), (size_t)5 * cfg_.nt * std::max(specs_[s].nrx, 1), 0.f, st_));
     STCHK(enqueue_forward(g, 3, st_));
     for (int s : groups_[g].shots) tm_.fwd_cells += cells_of(specs_[s]) * (cfg_.nt - 1);
-    tm_.fwd_launches += cfg_.nt - 1;
+    tm_.fwd_launches += fwd_launch_count(3);
     last_group_ = g;
   }
   HIPCHK(hipEventRecord(ev_[1], st_));
@@ -505,8 +547,8 @@ int Engine::migrate() {
     tm_.fwd_ms += a;
     tm_.adj_ms += b;
     tm_.total_ms += c;
-    tm_.fwd_launches += cfg_.nt - 1;
-    tm_.adj_launches += cfg_.nt;
+    tm_.fwd_launches += fwd_launch_count(0);
+    tm_.adj_launches += adj_launch_count();
     for (int s : groups_[g].shots) {
       tm_.fwd_cells += cells_of(specs_[s]) * (cfg_.nt - 1);
       tm_.adj_cells += cells_of(specs_[s]) * cfg_.nt;
@@ -541,7 +583,7 @@ int Engine::born(const std::vector<const float*>& dm) {
   float ms = 0;
   HIPCHK(hipEventElapsedTime(&ms, ev_[0], ev_[1]));
   tm_.fwd_ms = tm_.total_ms = ms;
-  tm_.fwd_launches = cfg_.nt;
+  tm_.fwd_launches = fwd_launch_count(5);
   last_group_ = 0;
   return OK;
 }
@@ -579,10 +621,10 @@ int Engine::get_last_levels(int shot, float* last10) {
   const size_t nn = (size_t)p.nz * p.nx;
   const int nt = cfg_.nt;
   for (int lv = 0; lv < 2; ++lv) {
-    const int L = nt - 1 + lv;  // levels nt-1, nt (slot L % 2)
+    const int L = nt - 1 + lv;  // levels nt-1, nt (slot L % kRing)
     for (int f = 0; f < 5; ++f)
       HIPCHK(hipMemcpy2D(last10 + (size_t)(5 * lv + f) * nn, p.nz * sizeof(float),
-                         base + (size_t)(L % 2) * 5 * r.n2 + f * r.n2, r.ldz * sizeof(float), p.nz * sizeof(float),
+                         base + (size_t)(L % kRing) * 5 * r.n2 + f * r.n2, r.ldz * sizeof(float), p.nz * sizeof(float),
                          p.nx, hipMemcpyDeviceToHost));
   }
   return OK;

@@ -31,6 +31,7 @@ struct EngineCfg {
   int txr = 4;         // tile width (x cells per thread)
   int xcd = 1;         // XCD-aware tile numbering
   int use_graph = 0;   // capture the time loops in HIP graphs
+  int tblock = 0;      // two levels per launch (temporal blocking)
   int max_group = 0;   // cap on shots per group (0 = memory bound only)
   int stack_nz = 0, stack_nx = 0;  // stacked image (0 = none)
   double mem_fraction = 0.85;      // share of free memory the working set may use
@@ -83,7 +84,12 @@ class Engine {
     DevBuf coef, coef_true, kfac, rec, img, imgc;
   };
   float* ring_of(int s) const { return reinterpret_cast<float*>(work_.as<char>() + res_[s].work_off); }
-  char* snap_of(int s) const { return work_.as<char>() + res_[s].work_off + (size_t)10 * res_[s].n2 * sizeof(float); }
+  // The ring holds kRing levels (four, so that a two-level forward step
+  // writes levels other than the two it reads).
+  static constexpr int kRing = 4;
+  char* snap_of(int s) const {
+    return work_.as<char>() + res_[s].work_off + (size_t)5 * kRing * res_[s].n2 * sizeof(float);
+  }
   struct Group {
     std::vector<int> shots;
     int tile0 = 0, ntiles = 0;  // slice of the tile table
@@ -96,6 +102,8 @@ class Engine {
   int enqueue_forward(int g, int mode, hipStream_t st);
   int enqueue_adjoint(int g, hipStream_t st);
   int run_phase(int g, int which, hipStream_t st);
+  long long fwd_launch_count(int mode) const;
+  long long adj_launch_count() const;
   void destroy_graphs();
 
   EngineCfg cfg_;
@@ -118,10 +126,13 @@ void sponge_factors(int n, int abc, std::vector<float>& k);
 int ensure_device();
 int default_txr();
 int default_xcd();
-// Tile-width autotuner (autotune.hip): times the engine on a short copy of
-// the workload for TXR 1, 2, 4, 8 and returns the fastest in *best (ms_out:
-// the four times, -1 when the answer came from the cache).
+int default_tblock();
+// Tile autotuner (autotune.hip): times the engine on a short copy of the
+// workload for TXR 1, 2, 4, 8 with one level per launch and TXR 1, 2, 4 with
+// two, and returns the fastest in *best / *best_tblock (ms_out[tblock*4 +
+// log2 txr]: the times, -1 for a candidate not run or when the answer came
+// from the cache).
 int autotune_txr(const EngineCfg& cfg, const std::vector<ShotSpec>& shots, const std::vector<const float*>& coef5,
-                 const float* wavelet, int* best, double ms_out[4], bool use_cache);
+                 const float* wavelet, int* best, int* best_tblock, double ms_out[8], bool use_cache);
 
 }  // namespace erm

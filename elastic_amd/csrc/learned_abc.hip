@@ -707,7 +707,7 @@ RTM_EXPORT int labc_collect(labc_data* d, int nz_phys, int nx_phys, int pad_wide
                          gs.lev_view(L - 2), gs.mat(), bd, d->feat.as<float>() + (size_t)d->size * KP, scales.as<float>());
       HIPCHK(hipGetLastError());
     }
-    STCHK(gw.step(L, (L >= 3 ? F_K : 0) | F_SRC, nullptr));
+    STCHK(gw.step(L, (L >= 3 && abc_wide > 0 ? F_K : 0) | F_SRC, nullptr));
     if (take) {
       if (L >= 3) {
         STCHK(gs.step(L, F_SRC, nullptr));  // plain update of the band grid

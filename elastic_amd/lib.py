@@ -313,6 +313,11 @@ class Batch:
         """Tile width the batch runs with (autotuned when created with txr=0)."""
         return int(load().rtm_batch_txr(self._h))
 
+    @property
+    def tblock(self):
+        """1 when the batch runs two levels per launch (temporal blocking)."""
+        return int(load().rtm_batch_tblock(self._h))
+
     def info(self):
         out = np.zeros(8, np.float64)
         _chk(load().rtm_batch_info(self._h, _p(out)))
@@ -333,8 +338,8 @@ class Batch:
 
 def autotune(nt, dx, dt, shots, coefs, wavelet, abc=0, adj_abc=0, snap_every=1, snap_prec="fp32", comp_image=0,
              use_graph=0):
-    """Run the tile-width autotuner on a workload (rtm_batch_autotune):
-    returns (best txr, {txr: ms})."""
+    """Run the tile autotuner on a workload (rtm_batch_autotune): returns
+    ((txr, levels per launch - 1), {(txr, tblock): ms})."""
     lib = load()
     n = len(shots)
     arr = (RtmBatchShot * n)()
@@ -346,10 +351,11 @@ def autotune(nt, dx, dt, shots, coefs, wavelet, abc=0, adj_abc=0, snap_every=1, 
     keep = [pack_fields(c) for c in coefs]
     ptrs = (C.c_void_p * n)(*[k.ctypes.data for k in keep])
     w = np.ascontiguousarray(wavelet, np.float32)
-    best = C.c_int()
-    ms = np.zeros(4, np.float64)
-    _chk(lib.rtm_batch_autotune(C.byref(cfg), C.c_int(n), arr, ptrs, _p(w), C.byref(best), _p(ms)))
-    return best.value, dict(zip((1, 2, 4, 8), ms.tolist()))
+    best, tb = C.c_int(), C.c_int()
+    ms = np.zeros(8, np.float64)
+    _chk(lib.rtm_batch_autotune(C.byref(cfg), C.c_int(n), arr, ptrs, _p(w), C.byref(best), C.byref(tb), _p(ms)))
+    times = {(t, k): ms[4 * k + i] for k in (0, 1) for i, t in enumerate((1, 2, 4, 8)) if ms[4 * k + i] >= 0}
+    return (best.value, tb.value), times
 
 
 def bandwidth_probe(nbytes=1 << 30, reps=20, mode=0):

@@ -206,15 +206,20 @@ int rtm_batch_get_stack(rtm_batch *b, float *img3);            /* 3*(stack_nz,st
  * [5] fwd cell-updates [6] adj cell-updates [7] shots per group */
 int rtm_batch_info(rtm_batch *b, double *out8);
 int rtm_batch_destroy(rtm_batch *b);
-/* Tile width the batch runs with. */
+/* Tile width and levels per launch (1 or 2) the batch runs with. */
 int rtm_batch_txr(rtm_batch *b);
-/* Tile-width autotuner: runs a short copy of the workload (up to 8 of the
- * shots, max(48, 8*snap_every) levels, same options) with each width 1, 2,
- * 4, 8 and returns the fastest in *txr and the four times (ms) in ms4. The
- * result is cached per process and, when RTM_TUNE_CACHE names a file, in it;
- * rtm_batch_create with cfg->txr == 0 uses the cached or tuned width. */
+int rtm_batch_tblock(rtm_batch *b);
+/* Tile autotuner: runs a short copy of the workload (up to 8 of the shots,
+ * max(48, 8*snap_every) levels, same options) with tile widths 1, 2, 4, 8 at
+ * one level per launch and 1, 2, 4 at two levels per launch (temporal
+ * blocking), and returns the fastest in *txr, *tblock; ms8[tblock*4 +
+ * log2(txr)] receives the times (ms, -1 for candidates not run). The result
+ * is cached per process and, when RTM_TUNE_CACHE names a file, in it;
+ * rtm_batch_create with cfg->txr == 0 uses the cached or tuned choice unless
+ * RTM_TXR / RTM_TBLOCK are set. */
 int rtm_batch_autotune(const rtm_batch_cfg_t *cfg, int nshots, const rtm_batch_shot_t *shots,
-                       const float *const *coef5, const float *wavelet, int *txr, double *ms4);
+                       const float *const *coef5, const float *wavelet, int *txr, int *tblock,
+                       double *ms8);
 
 /* Streaming kernel over nbytes, timed over reps launches with HIP events:
  * achieved GB/s (bytes read + written) and ms per launch. mode 0: copy,

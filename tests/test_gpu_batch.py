@@ -138,8 +138,31 @@ def test_tile_widths_agree_and_autotuner(hip):
             for x, y in zip(a, c):
                 assert np.array_equal(x, y), txr
     best, ms = hip.autotune(NT, DX, DT, specs, cmig, wav, abc=ABC, adj_abc=1)
-    assert best in (1, 2, 4, 8)
+    assert best[0] in (1, 2, 4, 8) and best[1] in (0, 1) and len(ms) == 7
     assert all(v > 0 for v in ms.values()) and ms[best] == min(ms.values())
     b = _batch(hip, specs, ctrue, cmig, wav, txr=0)
-    assert b.txr in (1, 2, 4, 8)
+    assert b.txr in (1, 2, 4, 8) and b.tblock in (0, 1)
     b.close()
+
+
+@pytest.mark.parametrize("prec,j", [("fp32", 1), ("bf16", 4), ("fp32", 3)])
+def test_two_level_forward_equals_single_level(hip, monkeypatch, prec, j):
+    """The temporally blocked forward (two levels per launch, default) gives
+    the same observed data, snapshots and images as one level per launch."""
+    specs, ctrue, cmig, wav = _survey(hip)
+    out = {}
+    for tb in ("0", "1"):
+        monkeypatch.setenv("RTM_TBLOCK", tb)
+        for txr in (1, 2, 4, 8):
+            b = _batch(hip, specs, ctrue, cmig, wav, txr=txr, snap_every=j, snap_prec=prec)
+            b.migrate()
+            out[tb, txr] = ([b.get_data(i) for i in range(len(specs))], [b.image(i) for i in range(len(specs))])
+            b.close()
+    for txr in (1, 2, 4, 8):
+        d0, i0 = out["0", txr]
+        d1, i1 = out["1", txr]
+        for a, c in zip(d0, d1):
+            assert np.array_equal(a, c), txr
+        for a, c in zip(i0, i1):
+            for x, y in zip(a, c):
+                assert np.array_equal(x, y), txr
