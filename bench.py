@@ -67,7 +67,7 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def build_batch(lib, synth, nz, nx, nshots, nt, dx, dt, fp, abc, variant, seed, txr=0):
+def build_batch(lib, synth, nz, nx, nshots, nt, dx, dt, fp, abc, variant, seed, txr=0, graph=True):
     vp, vps, layout = synth.survey(nz, nx, nshots, seed, abc, dx, dt)
     specs, ctrue, cmig = [], [], []
     for xs, x0, w in layout:
@@ -76,7 +76,9 @@ def build_batch(lib, synth, nz, nx, nshots, nt, dx, dt, fp, abc, variant, seed, 
         ctrue.append(ct)
         cmig.append(cm)
     wav = lib.ricker(nt, dt, fp)
-    opts = VARIANTS[variant]
+    opts = dict(VARIANTS[variant])
+    if not graph:
+        opts["use_graph"] = 0
     b = lib.Batch(nt, dx, dt, specs, cmig, wav, abc=abc, adj_abc=1, stack_shape=(nz, nx), txr=txr, **opts)
     b.set_true_model(ctrue)
     return b, specs, cmig, wav, layout
@@ -159,6 +161,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-validate", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=600)
+    ap.add_argument("--no-graph", action="store_true",
+                    help="launch every level from the host instead of replaying HIP graphs (same kernels; "
+                         "for tracing tools that do not follow graph launches)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -173,7 +178,7 @@ def main():
     t0 = time.perf_counter()
     log(f"bench: building {args.shots}-shot survey {args.nz}x{args.nx}, nt={args.nt}, variant={args.variant}")
     b, specs, cmig, wav, layout = build_batch(lib, synth, args.nz, args.nx, args.shots, args.nt, dx, dt, fp, abc,
-                                              args.variant, args.seed, args.txr)
+                                              args.variant, args.seed, args.txr, graph=not args.no_graph)
     log(f"bench: setup {time.perf_counter() - t0:.1f}s; synthesizing observed data on the GPU")
     t1 = time.perf_counter()
     b.synthesize()
@@ -204,7 +209,15 @@ def main():
     achieved = adj_gbs if dom == "adj" else fwd_gbs
     launches = info["adj_launches"] if dom == "adj" else info["fwd_launches"]
     ms_launch = (info["adj_ms"] if dom == "adj" else info["fwd_ms"]) / launches
-    traffic = load_traffic(args.variant)
+    # HBM bytes per launch of the dominant kernel: measured bytes per
+    # cell-update (rocprofv3 FETCH_SIZE / WRITE_SIZE passes, calibrated on a
+    # copy kernel) times this run's cell-updates per launch
+    tr = load_traffic(args.variant)
+    traffic = None
+    if tr and tr.get("kernel") == ("k_adj" if dom == "adj" else "k_fwd"):
+        cells_per_launch = (info["adj_cells"] / info["adj_launches"]) if dom == "adj" else \
+            (info["fwd_cells"] / info["fwd_launches"])
+        traffic = round(tr["bytes_per_cell_update"] * cells_per_launch)
     b.close()
 
     img_errs = None
@@ -247,7 +260,9 @@ def main():
         "valid": (max_err is not None and max_err <= 1e-3),
         "roofline": {"bound": "hbm", "kernel": "k_adj" if dom == "adj" else "k_fwd", "achieved": round(achieved, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": traffic, "ms_per_launch": round(ms_launch, 5),
+                     "traffic": traffic,
+                     "traffic_per_cell_update": tr["bytes_per_cell_update"] if traffic else None,
+                     "ms_per_launch": round(ms_launch, 5),
                      "algo_bytes_per_cell": fb if dom == "fwd" else abytes,
                      "other": {"k_fwd_GBs": round(fwd_gbs, 1), "k_adj_GBs": round(adj_gbs, 1),
                                "fwd_ms": round(info["fwd_ms"], 2), "adj_ms": round(info["adj_ms"], 2)}},

@@ -25,6 +25,7 @@ constexpr int OW1 = 0, OB1 = NF * H, OW2 = OB1 + H, OB2 = OW2 + H * H, OW3 = OB2
               OB3 = OW3 + H * NO, NP = OB3 + NO;
 static_assert(NP == LABC_NPARAM, "parameter count");
 constexpr float Z0 = LABC_Z0;
+constexpr int NRAW = 230;  // wavefield features (scaled by the cell's max-abs)
 
 struct Band {
   int nz, nx, nT, nL, total;
@@ -348,59 +349,66 @@ __global__ void k_sample(int* idx, int batch, int n, unsigned seed) {
 }
 
 // Inference on a grid: corrections delta[cell][5] (physical units) from
-// levels L-1 and L-2, one thread per band cell, weights in LDS.
+// levels L-1 and L-2. A block evaluates 8 band cells with 32 lanes each:
+// the lanes gather the cell's features into LDS (max-abs scale reduced over
+// the 32 lanes), then lane j computes hidden unit j of each layer; weights
+// are read through the caches (the same 36 KB for every block).
+constexpr int ICELLS = 8;
 __global__ __launch_bounds__(256) void k_infer(const float* __restrict__ w, Lev m1, Lev m2, Mat mat, Band b,
                                                float* __restrict__ delta) {
-  __shared__ float sW1[NF * H];
-  __shared__ float sW2[H * H];
-  __shared__ float sW3[H * NO];
-  __shared__ float sb[H + H + NO];
-  for (int i = threadIdx.x; i < NF * H; i += 256) sW1[i] = w[OW1 + i];
-  for (int i = threadIdx.x; i < H * H; i += 256) sW2[i] = w[OW2 + i];
-  for (int i = threadIdx.x; i < H * NO; i += 256) sW3[i] = w[OW3 + i];
-  for (int i = threadIdx.x; i < H; i += 256) {
-    sb[i] = w[OB1 + i];
-    sb[H + i] = w[OB2 + i];
+  __shared__ float sx[ICELLS][KP];
+  __shared__ float sh1[ICELLS][H];
+  __shared__ float sh2[ICELLS][H];
+  const int lc = threadIdx.x >> 5, j = threadIdx.x & 31;
+  const int c = blockIdx.x * ICELLS + lc;
+  const bool live = c < b.total;  // uniform over the cell's 32 lanes
+  int iz = 0, ix = 0, edge = 0, depth = 0;
+  if (live) cell_coords(b, c, iz, ix, edge, depth);
+  float m = 0.f;
+  for (int k = j; k < NRAW; k += 32) {
+    const float v = live ? raw_feature(m1, m2, iz, ix, edge, depth, k) : 0.f;
+    sx[lc][k] = v;
+    m = fmaxf(m, fabsf(v));
   }
-  if (threadIdx.x < NO) sb[2 * H + threadIdx.x] = w[OB3 + threadIdx.x];
+#pragma unroll
+  for (int o = 16; o > 0; o >>= 1) m = fmaxf(m, __shfl_xor(m, o, 32));
+  const float inv = m > 1e-30f ? 1.f / m : 0.f;
+  float cp = 0.f, cs = 0.f, zp = 0.f;
+  if (live) mat.at(iz, ix, cp, cs, zp);
+  for (int k = j; k < KP; k += 32)
+    sx[lc][k] = bf16r(k < NRAW ? sx[lc][k] * inv : static_feature(edge, depth, cp, cs, zp, k));
   __syncthreads();
+  float a = w[OB1 + j];
+  for (int k = 0; k < NF; ++k) a = fmaf(sx[lc][k], w[OW1 + k * H + j], a);
+  sh1[lc][j] = gelu(a);
+  __syncthreads();
+  float a2 = w[OB2 + j];
+#pragma unroll 8
+  for (int k = 0; k < H; ++k) a2 = fmaf(sh1[lc][k], w[OW2 + k * H + j], a2);
+  sh2[lc][j] = gelu(a2);
+  __syncthreads();
+  if (live && j < NO) {
+    float y = w[OB3 + j];
+#pragma unroll 8
+    for (int k = 0; k < H; ++k) y = fmaf(sh2[lc][k], w[OW3 + k * NO + j], y);
+    delta[(size_t)c * NO + j] = m > 1e-30f ? y * m * (j >= 2 ? Z0 : 1.f) : 0.f;
+  }
+}
+
+// Multiplicative noise on the band cells of one level (training
+// augmentation): v *= 1 + amp * u, u uniform in [-1, 1) from a hash of
+// (seed, cell, field).
+__global__ void k_noise(float* lev, int fs, int ldz, Band b, float amp, unsigned seed) {
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= b.total) return;
   int iz, ix, edge, depth;
   cell_coords(b, c, iz, ix, edge, depth);
-  float cp, cs, zp;
-  mat.at(iz, ix, cp, cs, zp);
-  float s = 0.f;
-  for (int k = 0; k < 230; ++k) s = fmaxf(s, fabsf(raw_feature(m1, m2, iz, ix, edge, depth, k)));
-  if (!(s > 1e-30f)) {
-    for (int f = 0; f < NO; ++f) delta[(size_t)c * NO + f] = 0.f;
-    return;
-  }
-  const float inv = 1.f / s;
-  float h1[H];
-  for (int j = 0; j < H; ++j) h1[j] = sb[j];
-  for (int k = 0; k < NF; ++k) {
-    const float x = bf16r(k < 230 ? raw_feature(m1, m2, iz, ix, edge, depth, k) * inv
-                                  : static_feature(edge, depth, cp, cs, zp, k));
-    if (x == 0.f) continue;
-#pragma unroll
-    for (int j = 0; j < H; ++j) h1[j] += x * sW1[k * H + j];
-  }
-#pragma unroll
-  for (int j = 0; j < H; ++j) h1[j] = gelu(h1[j]);
-  float h2[H];
-#pragma unroll
-  for (int j = 0; j < H; ++j) h2[j] = sb[H + j];
-  for (int k = 0; k < H; ++k) {
-#pragma unroll
-    for (int j = 0; j < H; ++j) h2[j] += h1[k] * sW2[k * H + j];
-  }
-#pragma unroll
-  for (int j = 0; j < H; ++j) h2[j] = gelu(h2[j]);
-  for (int o = 0; o < NO; ++o) {
-    float y = sb[2 * H + o];
-    for (int k = 0; k < H; ++k) y += h2[k] * sW3[k * NO + o];
-    delta[(size_t)c * NO + o] = y * s * (o >= 2 ? Z0 : 1.f);
+  for (int f = 0; f < NO; ++f) {
+    unsigned x = seed * 0x9e3779b9u ^ ((unsigned)c * 5u + (unsigned)f) * 0x85ebca6bu;
+    x ^= x >> 16; x *= 0x7feb352du; x ^= x >> 15; x *= 0x846ca68bu; x ^= x >> 16;
+    const float u = (float)(x >> 8) * (2.f / 16777216.f) - 1.f;
+    float* p = lev + f * fs + iz + ldz * ix;
+    *p = *p * (1.f + amp * u);
   }
 }
 
@@ -533,7 +541,7 @@ RTM_EXPORT int labc_cell_coords(int nz, int nx, int c, int* iz, int* ix, int* ed
 
 RTM_EXPORT int labc_net_forward(labc_net* n, const float* feat, int cnt, float* out, int mode) {
   if (!n || !feat || !out || cnt < 1) { set_error("bad argument"); return E_ARG; }
-  if (mode != 0) { set_error("only the VALU forward (mode 0) is built"); return E_ARG; }
+  if (mode != 0) { set_error("mode must be 0 (the training forward)"); return E_ARG; }
   DevBuf f, o;
   STCHK(f.alloc((size_t)cnt * KP * 4, "features"));
   STCHK(o.alloc((size_t)cnt * NO * 4, "outputs"));
@@ -621,8 +629,12 @@ struct LGrid {
   DevBuf desc, tiles, ring, coef, kfac, amp;
   int ntiles = 0, ldz = 0;
   long long n2 = 0;
+  // band_abc / band_off: multiply the sponge by the band grid's sponge of
+  // width band_abc on the sub-grid [band_off, band_off + band_n) (clamped
+  // outside it), so that a wide grid damps the band as the band grid does.
   int setup(int nz, int nx, int nt, float dt, int zs, int xs, const float* coef5, const float* wavelet, int abc,
-            int zr, int rx0, int nrx, float* rec_dev) {
+            int zr, int rx0, int nrx, float* rec_dev, int band_abc = 0, int band_off = 0, int band_nz = 0,
+            int band_nx = 0) {
     ldz = device_ldz(nz);
     n2 = (long long)ldz * nx;
     STCHK(ring.alloc(10 * n2 * 4, "ring"));
@@ -632,10 +644,19 @@ struct LGrid {
     h.nz = nz; h.nx = nx; h.ldz = ldz; h.nt = nt; h.n2 = n2;
     h.lev = ring.as<float>(); h.lev_stride = 5 * n2; h.fstride = n2; h.lev_mod = 2; h.lev_off = 2;
     h.coef = coef.as<float>();
-    if (abc > 0) {
-      std::vector<float> hz, hx;
-      sponge_factors(nz, abc, hz);
-      sponge_factors(nx, abc, hx);
+    if (abc > 0 || band_abc > 0) {
+      std::vector<float> hz(nz, 1.f), hx(nx, 1.f);
+      if (abc > 0) {
+        sponge_factors(nz, abc, hz);
+        sponge_factors(nx, abc, hx);
+      }
+      if (band_abc > 0) {
+        std::vector<float> bz, bx;
+        sponge_factors(band_nz, band_abc, bz);
+        sponge_factors(band_nx, band_abc, bx);
+        for (int i = 0; i < nz; ++i) hz[i] *= bz[std::min(std::max(i - band_off, 0), band_nz - 1)];
+        for (int i = 0; i < nx; ++i) hx[i] *= bx[std::min(std::max(i - band_off, 0), band_nx - 1)];
+      }
       STCHK(kfac.alloc((size_t)(nz + nx) * 4, "sponge"));
       HIPCHK(hipMemcpy(kfac.p, hz.data(), nz * 4, hipMemcpyHostToDevice));
       HIPCHK(hipMemcpy(kfac.as<float>() + nz, hx.data(), nx * 4, hipMemcpyHostToDevice));
@@ -671,7 +692,8 @@ struct LGrid {
 
 RTM_EXPORT int labc_collect(labc_data* d, int nz_phys, int nx_phys, int pad_wide, int abc_wide, int nt,
                             float dt, int zs_phys, int xs_phys, const float* coef5_wide,
-                            const float* coef5_small, const float* wavelet, const int* levels, int nlev) {
+                            const float* coef5_small, const float* wavelet, const int* levels, int nlev,
+                            float noise, unsigned seed, int abc_band) {
   if (!d || !coef5_wide || !coef5_small || !wavelet || !levels || nlev < 1) { set_error("bad argument"); return E_ARG; }
   const int pad_s = R + W;
   if (pad_wide < pad_s + 5) { set_error("wide padding must exceed the band padding"); return E_ARG; }
@@ -685,9 +707,11 @@ RTM_EXPORT int labc_collect(labc_data* d, int nz_phys, int nx_phys, int pad_wide
   }
   STCHK(ensure_device());
   LGrid gw, gs;
+  if (abc_band < 0 || abc_band > W) { set_error("band sponge width %d not in [0, %d]", abc_band, W); return E_ARG; }
   STCHK(gw.setup(nzw, nxw, nt, dt, pad_wide + zs_phys, pad_wide + xs_phys, coef5_wide, wavelet, abc_wide, 0, 0, 0,
+                 nullptr, abc_band, off, nzs, nxs));
+  STCHK(gs.setup(nzs, nxs, nt, dt, pad_s + zs_phys, pad_s + xs_phys, coef5_small, wavelet, abc_band, 0, 0, 0,
                  nullptr));
-  STCHK(gs.setup(nzs, nxs, nt, dt, pad_s + zs_phys, pad_s + xs_phys, coef5_small, wavelet, 0, 0, 0, 0, nullptr));
   DevBuf scales;
   STCHK(scales.alloc((size_t)bd.total * 4, "scales"));
   std::vector<int> lv(levels, levels + nlev);
@@ -702,15 +726,22 @@ RTM_EXPORT int labc_collect(labc_data* d, int nz_phys, int nx_phys, int pad_wide
                          gs.level(L - 1), (int)gs.n2, gs.ldz, nzs, nxs);
       hipLaunchKernelGGL(k_window, wgrid, dim3(256), 0, nullptr, gw.level(L - 2), (int)gw.n2, gw.ldz, off,
                          gs.level(L - 2), (int)gs.n2, gs.ldz, nzs, nxs);
+      if (noise > 0.f) {
+        hipLaunchKernelGGL(k_noise, dim3((bd.total + 255) / 256), dim3(256), 0, nullptr, gs.level(L - 1), (int)gs.n2,
+                           gs.ldz, bd, noise, seed * 7919u + 2u * (unsigned)L);
+        hipLaunchKernelGGL(k_noise, dim3((bd.total + 255) / 256), dim3(256), 0, nullptr, gs.level(L - 2), (int)gs.n2,
+                           gs.ldz, bd, noise, seed * 7919u + 2u * (unsigned)L + 1u);
+      }
       HIPCHK(hipGetLastError());
       hipLaunchKernelGGL(k_features, dim3((bd.total + 255) / 256), dim3(256), 0, nullptr, gs.lev_view(L - 1),
                          gs.lev_view(L - 2), gs.mat(), bd, d->feat.as<float>() + (size_t)d->size * KP, scales.as<float>());
       HIPCHK(hipGetLastError());
     }
-    STCHK(gw.step(L, (L >= 3 && abc_wide > 0 ? F_K : 0) | F_SRC, nullptr));
+    STCHK(gw.step(L, (L >= 3 && (abc_wide > 0 || abc_band > 0) ? F_K : 0) | F_SRC, nullptr));
     if (take) {
       if (L >= 3) {
-        STCHK(gs.step(L, F_SRC, nullptr));  // plain update of the band grid
+        // plain update of the band grid (with its sponge when abc_band > 0)
+        STCHK(gs.step(L, F_SRC | (abc_band > 0 ? F_K : 0), nullptr));
         hipLaunchKernelGGL(k_targets, dim3((bd.total + 255) / 256), dim3(256), 0, nullptr, gw.level(L),
                            (int)gw.n2, gw.ldz, off, gs.level(L), (int)gs.n2, gs.ldz, bd,
                            scales.as<float>(), d->tgt.as<float>() + (size_t)d->size * NO);
@@ -776,7 +807,7 @@ RTM_EXPORT int labc_forward(labc_net* n, const rtm_shot_t* s, const float* coef5
   STCHK(recd.alloc((size_t)5 * s->nt * nr * 4, "receivers"));
   STCHK(launch_fill(recd.as<float>(), (size_t)5 * s->nt * nr, 0.f, nullptr));
   LGrid g;
-  STCHK(g.setup(s->nz, s->nx, s->nt, s->dt, s->zs, s->xs, coef5, wavelet, n ? 0 : s->abc, s->zr, s->rx0, s->nrx,
+  STCHK(g.setup(s->nz, s->nx, s->nt, s->dt, s->zs, s->xs, coef5, wavelet, s->abc, s->zr, s->rx0, s->nrx,
                 recd.as<float>()));
   const Band bd = band_of(s->nz, s->nx);
   if (n) STCHK(delta.alloc((size_t)bd.total * NO * 4, "corrections"));
@@ -786,10 +817,10 @@ RTM_EXPORT int labc_forward(labc_net* n, const rtm_shot_t* s, const float* coef5
   HIPCHK(hipEventRecord(e0, nullptr));
   for (int L = 2; L <= s->nt; ++L) {
     int fl = F_SRC | (s->nrx > 0 ? F_REC : 0);
-    if (!n && L >= 3 && s->abc > 0) fl |= F_K;
+    if (L >= 3 && s->abc > 0) fl |= F_K;  // with a network: the sponge and the layer together
     const bool learn = n && L >= 3;
     if (learn) {
-      hipLaunchKernelGGL(k_infer, dim3((bd.total + 255) / 256), dim3(256), 0, nullptr, n->w.as<float>(),
+      hipLaunchKernelGGL(k_infer, dim3((bd.total + ICELLS - 1) / ICELLS), dim3(256), 0, nullptr, n->w.as<float>(),
                          g.lev_view(L - 1), g.lev_view(L - 2), g.mat(), bd, delta.as<float>());
       HIPCHK(hipGetLastError());
     }

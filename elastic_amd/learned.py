@@ -142,20 +142,22 @@ class Data:
         return f, t
 
     def collect(self, nz_phys, nx_phys, pad_wide, abc_wide, nt, dt, zs_phys, xs_phys, coefs_wide, coefs_small,
-                wavelet, levels):
+                wavelet, levels, noise=0.0, seed=0, abc_band=0):
         """Teacher-forced pairs of one wide-sponge run (see labc_collect)."""
         cw, cs = pack_fields(coefs_wide), pack_fields(coefs_small)
         w = np.ascontiguousarray(wavelet, np.float32)
         lv = np.ascontiguousarray(levels, np.int32)
         _chk(_L().labc_collect(self.h, C.c_int(nz_phys), C.c_int(nx_phys), C.c_int(pad_wide), C.c_int(abc_wide),
                                C.c_int(nt), C.c_float(dt), C.c_int(zs_phys), C.c_int(xs_phys), _p(cw), _p(cs),
-                               _p(w), _p(lv), C.c_int(lv.size)))
+                               _p(w), _p(lv), C.c_int(lv.size), C.c_float(noise), C.c_uint(seed),
+                               C.c_int(abc_band)))
 
 
 def propagate(net, s: _lib.RtmShot, coef5, wavelet, want_last=False):
-    """Forward propagation of a band grid with the learned layer (net) or,
-    with net None, with the sponge s.abc. Returns receivers (5, nrx, nt)
-    [and the last two levels] and the device time of the time loop (ms)."""
+    """Forward propagation of a band grid with the learned layer (net, and
+    the sponge s.abc when > 0) or, with net None, with the sponge s.abc alone.
+    Returns receivers (5, nrx, nt) [and the last two levels] and the device
+    time of the time loop (ms)."""
     lib = _L()
     cb = pack_fields(coef5)
     w = np.ascontiguousarray(wavelet, np.float32)

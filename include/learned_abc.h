@@ -58,9 +58,9 @@ int labc_net_set_params(labc_net *net, const float *params);
 int labc_cell_count(int nz, int nx);
 /* Band cell c -> (iz, ix, edge 0..3, depth 0..7). */
 int labc_cell_coords(int nz, int nx, int c, int *iz, int *ix, int *edge, int *depth);
-/* Host-side evaluation on a feature matrix feat [n][LABC_KPAD] (unscaled by
- * s, i.e. the normalised features): out [n][5]. mode 0: the VALU training
- * forward, 1: the MFMA inference path. */
+/* Evaluation of the network on a feature matrix feat [n][LABC_KPAD] (the
+ * normalised features): out [n][5]. mode must be 0 (the training forward,
+ * 32 samples per block); propagation uses its own inference kernel. */
 int labc_net_forward(labc_net *net, const float *feat, int n, float *out, int mode);
 /* Loss 0.5*mean((out - target)^2) over n*5 values and its gradient [LABC_NPARAM]. */
 int labc_net_grad(labc_net *net, const float *feat, const float *target, int n, float *grad,
@@ -85,19 +85,28 @@ int labc_data_get(labc_data *d, int row0, int n, float *feat, float *tgt);
  * band grid's levels L-1, L-2 are the windowed wide levels (rim zeroed); the
  * target is wide(L) - plain band-grid update(L) on the band cells, divided by
  * the cell scale. coef5_wide / coef5_small: dt/dx-scaled coefficients of the
- * two grids; wavelet[it-1] = source(it). */
+ * two grids; wavelet[it-1] = source(it). noise > 0 multiplies the band cells
+ * of the two windowed levels by 1 + noise*u (u uniform in [-1,1), hashed
+ * from seed, level, cell and field) before the features and the plain update
+ * are formed, so the network learns to pull perturbed states back.
+ * abc_band > 0 (<= LABC_WIDTH) trains the layer to work with a sponge of that
+ * width in the band: the band grid's update applies it, and the wide grid
+ * applies the same factors at the band's position (continued outwards), so
+ * the target is the exterior contribution of a damped wavefield. */
 int labc_collect(labc_data *d, int nz_phys, int nx_phys, int pad_wide, int abc_wide, int nt,
                  float dt, int zs_phys, int xs_phys, const float *coef5_wide,
-                 const float *coef5_small, const float *wavelet, const int *levels, int nlev);
+                 const float *coef5_small, const float *wavelet, const int *levels, int nlev,
+                 float noise, unsigned seed, int abc_band);
 /* Adam on random minibatches: loss per step in losses[steps]. */
 int labc_train(labc_net *net, labc_data *d, int steps, int batch, float lr, unsigned seed,
                float *losses);
 /* Mean loss over the whole data set. */
 int labc_eval(labc_net *net, labc_data *d, float *loss);
 
-/* One forward propagation of a band grid with the learned layer (no sponge)
- * or, with net = NULL, with the sponge abc: records receivers like
- * rtm_forward_hip and returns the last two levels. */
+/* One forward propagation of a band grid with the learned layer (net) and
+ * the sponge s->abc (0 = none; both may act together), or, with
+ * net = NULL, with the sponge alone: records receivers like rtm_forward_hip
+ * and returns the last two levels. */
 int labc_forward(labc_net *net, const rtm_shot_t *s, const float *coef5, const float *wavelet,
                  float *rec, float *last);
 /* Device time of the last labc_forward time loop (ms). */

@@ -31,7 +31,7 @@ from __future__ import annotations
 import numpy as np
 from scipy.special import erf
 
-from . import forward, fwd_level, make_params
+from . import abc_K, forward, fwd_level, make_params
 
 R, W, NF, KP, H, NO = 5, 8, 245, 248, 32, 5
 NRAW = 230
@@ -208,18 +208,57 @@ def window(lev, off, nz, nx):
     return out
 
 
+def sponge_axes(n1, n2, abc):
+    """Per-row and per-column factors of the sponge K on an (n1, n2) grid
+    (K(field) = field * kz[iz] * kx[ix], W/abc-mod.f90:59-78)."""
+    if abc <= 0:
+        return np.ones(n1), np.ones(n2)
+    k = abc_K(np.ones((n1, n2)), abc)
+    kx = k[n1 // 2, :].copy()
+    kz = k[:, n2 // 2] / kx[n2 // 2]
+    return kz, kx
+
+
+def _forward_custom_k(p, wavelet, coefs, kz, kx, nt):
+    """Forward levels 1..nt (W/rtm-mod-adj.f90:178-260) with a separable
+    damping kz[iz]*kx[ix] after every update from level 3 on; returns the
+    five (nz, nx, nt) histories."""
+    nz, nx = p.nz, p.nx
+    hist = [np.zeros((nz, nx, nt), order="F") for _ in range(5)]
+    lev = {0: [np.zeros((nz, nx)) for _ in range(5)], 1: [np.zeros((nz, nx)) for _ in range(5)]}
+    for L in range(2, nt + 1):
+        new = fwd_level(p, lev[L - 2], lev[L - 1], coefs, apply_k=False, srcamp=p.dt * wavelet[L - 2])
+        if L >= 3:
+            new = [(a * kz[:, None]) * kx[None, :] for a in new]
+        for f in range(5):
+            hist[f][:, :, L - 1] = new[f]
+        lev[L] = new
+        lev.pop(L - 2)
+    return hist
+
+
 def teacher_pairs(nz_phys, nx_phys, pad_wide, abc_wide, nt, dt, zs_phys, xs_phys, coefs_wide,
-                  coefs_small, wavelet, levels):
+                  coefs_small, wavelet, levels, abc_band=0):
     """FP64 (features, targets) of the band cells at each level in levels
     (features of levels L-1, L-2 windowed from the wide run), stacked in
-    level order like labc_collect."""
+    level order like labc_collect. abc_band > 0: the band grid's sponge of
+    that width acts in the band grid's update, and the wide run is damped by
+    the same factors at the band's position (clamped outside the band grid)
+    on top of its own sponge."""
     pad_s = R + W
     nzw, nxw = nz_phys + 2 * pad_wide, nx_phys + 2 * pad_wide
     nzs, nxs = nz_phys + 2 * pad_s, nx_phys + 2 * pad_s
     off = pad_wide - pad_s
     pw = make_params(nzw, nxw, nt, 1.0, dt, pad_wide + xs_phys, pad_wide + zs_phys, 0, 0, 0, abc_wide)
-    hist = forward(pw, wavelet, coefs_wide, want_hist5=True)["hist5"]
-    ps = make_params(nzs, nxs, nt, 1.0, dt, pad_s + xs_phys, pad_s + zs_phys, 0, 0, 0, 0)
+    if abc_band <= 0:
+        hist = forward(pw, wavelet, coefs_wide, want_hist5=True)["hist5"]
+    else:
+        kzw, kxw = sponge_axes(nzw, nxw, abc_wide)
+        kzb, kxb = sponge_axes(nzs, nxs, abc_band)
+        kz = kzw * kzb[np.clip(np.arange(nzw) - off, 0, nzs - 1)]
+        kx = kxw * kxb[np.clip(np.arange(nxw) - off, 0, nxs - 1)]
+        hist = _forward_custom_k(pw, np.asarray(wavelet, np.float64), coefs_wide, kz, kx, nt)
+    ps = make_params(nzs, nxs, nt, 1.0, dt, pad_s + xs_phys, pad_s + zs_phys, 0, 0, 0, abc_band)
     cells = band_cells(nzs, nxs)
     iz, ix = cells[0], cells[1]
     feats, tgts = [], []
@@ -229,7 +268,7 @@ def teacher_pairs(nz_phys, nx_phys, pad_wide, abc_wide, nt, dt, zs_phys, xs_phys
         m1 = window([h[:, :, L - 2] for h in hist], off, nzs, nxs)
         m2 = window([h[:, :, L - 3] for h in hist], off, nzs, nxs)
         f, s = features(m1, m2, coefs_small, np.float64, cells)
-        plain = fwd_level(ps, m2, m1, coefs_small, apply_k=False, srcamp=dt * wavelet[L - 2])
+        plain = fwd_level(ps, m2, m1, coefs_small, apply_k=abc_band > 0, srcamp=dt * wavelet[L - 2])
         t = np.zeros((iz.size, NO))
         for fi in range(5):
             wide = hist[fi][iz + off, ix + off, L - 1]

@@ -240,3 +240,35 @@ def test_teacher_targets_are_the_rim_contribution():
         u = LA.Z0 if fi >= 2 else 1.0
         want = np.where(s > 0, contrib[fi][iz, ix] / np.where(s > 0, s * u, 1.0), 0.0)
         assert np.allclose(t[:, fi], want, rtol=1e-9, atol=1e-12 * np.abs(want).max())
+
+
+def test_custom_damping_forward_matches_oracle_sponge():
+    """The separable-damping forward used for band-sponge teachers equals the
+    oracle's own sponge forward when the damping is that sponge."""
+    case = _small_case()
+    nzw = case["nz_phys"] + 2 * case["pad_wide"]
+    nxw = case["nx_phys"] + 2 * case["pad_wide"]
+    p = O.make_params(nzw, nxw, 80, 1.0, case["dt"], case["pad_wide"] + 6, case["pad_wide"] + 12, 0, 0, 0,
+                      case["abc_wide"])
+    kz, kx = LA.sponge_axes(nzw, nxw, case["abc_wide"])
+    # the factors reproduce K on a field
+    f = np.random.default_rng(1).standard_normal((nzw, nxw))
+    assert np.allclose(O.abc_K(f, case["abc_wide"]), f * kz[:, None] * kx[None, :], rtol=1e-13, atol=0)
+    h = LA._forward_custom_k(p, np.asarray(case["wavelet"]), case["coefs_wide"], kz, kx, 80)
+    ref = O.forward(p, case["wavelet"], case["coefs_wide"], want_hist5=True)["hist5"]
+    for a, b in zip(h, ref):
+        assert O.rel_l2(a, b) <= 1e-12
+
+
+def test_band_sponge_targets_local_to_the_rim():
+    """With the band sponge in both runs the targets are still the exterior
+    contribution only: zero beyond the stencil radius of the rim."""
+    case = _small_case()
+    levels = [60, 90, 119]
+    f, t = LA.teacher_pairs(levels=levels, abc_band=8, **case)
+    nzs, nxs = case["nz_phys"] + 2 * (LA.R + LA.W), case["nx_phys"] + 2 * (LA.R + LA.W)
+    iz, ix, _, _ = LA.band_cells(nzs, nxs)
+    dist = np.minimum(np.minimum(iz, nzs - 1 - iz), np.minimum(ix, nxs - 1 - ix)) - LA.R
+    d = np.tile(dist, len(levels))
+    assert np.abs(t[d >= 5]).max() <= 1e-12 * np.abs(t).max()
+    assert np.abs(t[d < 5]).max() > 1e-3
