@@ -65,6 +65,7 @@ __device__ __forceinline__ int xcd_tile(int L, int N) {
 }
 
 __device__ __forceinline__ void neumaier(float& s, float& c, float t) {
+#pragma clang fp contract(off)
   const float u = s + t;
   if (fabsf(s) >= fabsf(t))
     c += (s - u) + t;
@@ -765,6 +766,43 @@ __global__ __launch_bounds__(256) void k_adj(const ShotDev* __restrict__ shots,
       for (int f = 0; f < 5; ++f) b[j][f] = r[f * rs] + b[j][f];
     }
   }
+  // Imaging inputs: the source Vx, Vz of level L with their halo, the
+  // own-cell S1/S2 operands and the image (and compensation) values to be
+  // updated. For fp32 snapshots they are loaded here, before the barrier,
+  // with the step's own operands (measured faster); bf16 / fp16 snapshots
+  // need their strip scales too and are loaded after the step (fewer live
+  // registers, higher occupancy, measured faster for them).
+  const bool imaging = flags & A_IMG;  // uniform over the launch
+  constexpr bool kEarly = SNAP == SNAP_HIST || SNAP == SNAP_F32;
+  const SrcView<SNAP> sv(S, L);
+  float sx[NIT][2], nt2[TXR][2], im[TXR][3], imc[TXR][3];
+  auto load_imaging = [&]() {
+#pragma unroll
+    for (int k = 0; k < NIT; ++k) {
+      const int c = tid + 256 * k;
+      const int lx = c / T::LZ, lz = c - lx * T::LZ;
+      const int gx = x0 - T::H + lx, gz2 = z0 - T::H + lz;
+      const bool in = c < T::N && gx >= 0 && gx < nx && gz2 >= 0 && gz2 < nz;
+      const int gi = gz2 + ldz * gx;
+      sx[k][0] = in ? sv.get(0, gi, gz2, gx) : 0.f;
+      sx[k][1] = in ? sv.get(1, gi, gz2, gx) : 0.f;
+    }
+    const bool next = flags & A_NEXT;
+#pragma unroll
+    for (int j = 0; j < TXR; ++j) {
+      const int gx = x0 + ty * TXR + j;
+      const bool ok = zok && gx < nx;
+      const int gi = gz + ldz * gx;
+      nt2[j][0] = ok && next ? sv.get(2, gi, gz, gx) : 0.f;
+      nt2[j][1] = ok && next ? sv.get(3, gi, gz, gx) : 0.f;
+#pragma unroll
+      for (int a = 0; a < 3; ++a) {
+        im[j][a] = ok ? S.img[a * n2 + gi] : 0.f;
+        imc[j][a] = ok && (flags & A_COMP) ? S.imgc[a * n2 + gi] : 0.f;
+      }
+    }
+  };
+  if (kEarly && imaging) load_imaging();
 #pragma unroll
   for (int k = 0; k < NIT; ++k) {
     const int c = tid + 256 * k;
@@ -813,26 +851,17 @@ __global__ __launch_bounds__(256) void k_adj(const ShotDev* __restrict__ shots,
     for (int f = 0; f < 5; ++f) o[j][f] = v[f];
   }
 
-  if (!(flags & A_IMG)) return;  // uniform over the launch
-  const SrcView<SNAP> sv(S, L);
-  float sx[NIT][2];
-#pragma unroll
-  for (int k = 0; k < NIT; ++k) {
-    const int c = tid + 256 * k;
-    const int lx = c / T::LZ, lz = c - lx * T::LZ;
-    const int gx = x0 - T::H + lx, gz2 = z0 - T::H + lz;
-    const bool in = c < T::N && gx >= 0 && gx < nx && gz2 >= 0 && gz2 < nz;
-    const int gi = gz2 + ldz * gx;
-    sx[k][0] = in ? sv.get(0, gi, gz2, gx) : 0.f;
-    sx[k][1] = in ? sv.get(1, gi, gz2, gx) : 0.f;
-  }
+  if (!imaging) return;
+  if (!kEarly) load_imaging();
+  // the source tile replaces the products in LDS
   __syncthreads();
+  float* const ss[2] = {sh[0], sh[1]};
 #pragma unroll
   for (int k = 0; k < NIT; ++k) {
     const int c = tid + 256 * k;
     if (c < T::N) {
-      sh[0][c] = sx[k][0];
-      sh[1][c] = sx[k][1];
+      ss[0][c] = sx[k][0];
+      ss[1][c] = sx[k][1];
     }
   }
   __syncthreads();
@@ -843,6 +872,9 @@ __global__ __launch_bounds__(256) void k_adj(const ShotDev* __restrict__ shots,
   const float w = S.img_w, inv_dx = S.inv_dx, inv_dt = S.inv_dt;
 #pragma unroll
   for (int j = 0; j < TXR; ++j) {
+    // the imaging arithmetic is fused only where written (fmaf), so every
+    // kernel variant rounds it the same way
+#pragma clang fp contract(off)
     const int lxo = ty * TXR + j;
     const int gx = x0 + lxo;
     if (!zok || gx >= nx) continue;
@@ -852,51 +884,41 @@ __global__ __launch_bounds__(256) void k_adj(const ShotDev* __restrict__ shots,
     float s1 = 0.f, s2 = 0.f;
     if (next) {
       if (SNAP == SNAP_HIST) {  // arrays 2, 3 are Vx, Vz of level L+1
-        s1 = inv_dt * (sv.get(2, gi, gz, gx) - sh[0][p]);
-        s2 = inv_dt * (sv.get(3, gi, gz, gx) - sh[1][p]);
+        s1 = inv_dt * (nt2[j][0] - ss[0][p]);
+        s2 = inv_dt * (nt2[j][1] - ss[1][p]);
       } else {
-        s1 = inv_dt * sv.get(2, gi, gz, gx);
-        s2 = inv_dt * sv.get(3, gi, gz, gx);
+        s1 = inv_dt * nt2[j][0];
+        s2 = inv_dt * nt2[j][1];
       }
     }
     // S4 = vxdx, S5 = vzdz (range 4..n-6), S6 = vxdz + vzdx (range 5..n-6)
     float s4 = 0.f, s5 = 0.f, s6 = 0.f;
     if (zin4 && gx >= 4 && gx <= nx - 6) {
-      s4 = sp<SX>(sh[0], p) * inv_dx;
-      s5 = sp<1>(sh[1], p) * inv_dx;
+      s4 = sp<SX>(ss[0], p) * inv_dx;
+      s5 = sp<1>(ss[1], p) * inv_dx;
     }
     if (zin5 && gx >= 5 && gx <= nx - 6) {
-      s6 = sm<1>(sh[0], p) * inv_dx;
-      s6 = fmaf(sm<SX>(sh[1], p), inv_dx, s6);
+      s6 = sm<1>(ss[0], p) * inv_dx;
+      s6 = fmaf(sm<SX>(ss[1], p), inv_dx, s6);
     }
     const float s3 = s4 + s5;
     const float* v = o[j];
-    const float t1 = w * fmaf(v[0], s1, v[1] * s2);
-    const float t2 = -w * fmaf(v[2], s3, v[3] * s3);
-    const float t3 = -w * fmaf(v[4], s6, fmaf(v[3] * 2.f, s5, v[2] * 2.f * s4));
+    const float t[3] = {w * fmaf(v[0], s1, v[1] * s2), -w * fmaf(v[2], s3, v[3] * s3),
+                        -w * fmaf(v[4], s6, fmaf(v[3] * 2.f, s5, v[2] * 2.f * s4))};
     if (flags & A_COMP) {
-      float s = img[gi], c = imgc[gi];
-      neumaier(s, c, t1);
-      img[gi] = s;
-      imgc[gi] = c;
-      s = img[n2 + gi];
-      c = imgc[n2 + gi];
-      neumaier(s, c, t2);
-      img[n2 + gi] = s;
-      imgc[n2 + gi] = c;
-      s = img[2 * n2 + gi];
-      c = imgc[2 * n2 + gi];
-      neumaier(s, c, t3);
-      img[2 * n2 + gi] = s;
-      imgc[2 * n2 + gi] = c;
+#pragma unroll
+      for (int a = 0; a < 3; ++a) {
+        float sa = im[j][a], ca = imc[j][a];
+        neumaier(sa, ca, t[a]);
+        img[a * n2 + gi] = sa;
+        imgc[a * n2 + gi] = ca;
+      }
     } else {
-      img[gi] += t1;
-      img[n2 + gi] += t2;
-      img[2 * n2 + gi] += t3;
+#pragma unroll
+      for (int a = 0; a < 3; ++a) img[a * n2 + gi] = im[j][a] + t[a];
     }
   }
 }
-
 
 // Imaging condition of one adjoint level at the thread's TXR cells: o holds
 // the adjoint fields of the level, the source Vx, Vz of the level are staged
@@ -914,6 +936,7 @@ __device__ __forceinline__ void image_cells(const ShotDev& S, const SrcView<SNAP
   const float w = S.img_w, inv_dx = S.inv_dx, inv_dt = S.inv_dt;
 #pragma unroll
   for (int j = 0; j < TXR; ++j) {
+#pragma clang fp contract(off)
     const int lxo = ty * TXR + j;
     const int gx = x0 + lxo;
     if (gx >= nx) continue;
@@ -1267,6 +1290,7 @@ __global__ __launch_bounds__(256) void k_born_sources(const ShotDev* __restrict_
   if (gz >= nz) return;
   constexpr int SX = T::LZ;
   for (int j = 0; j < 4; ++j) {
+#pragma clang fp contract(off)
     const int lxo = ty * 4 + j;
     const int gx = x0 + lxo;
     if (gx >= nx) break;

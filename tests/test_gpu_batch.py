@@ -121,14 +121,15 @@ def test_batch_rejects_bad_shot(hip):
         hip.Batch(NT, DX, DT, specs, cmig, wav, abc=ABC)
 
 
-def test_tile_widths_agree_and_autotuner(hip):
+@pytest.mark.parametrize("j", [1, 3])
+def test_tile_widths_agree_and_autotuner(hip, j):
     """Every tile width computes the same per-cell arithmetic (fp32 snapshots:
     bitwise equal images); the autotuner returns the fastest width and a batch
     created with txr=0 runs with it."""
     specs, ctrue, cmig, wav = _survey(hip)
     imgs = {}
     for txr in (1, 2, 4, 8):
-        b = _batch(hip, specs, ctrue, cmig, wav, txr=txr)
+        b = _batch(hip, specs, ctrue, cmig, wav, txr=txr, snap_every=j)
         assert b.txr == txr
         b.migrate()
         imgs[txr] = [b.image(i) for i in range(len(specs))]
