@@ -60,6 +60,30 @@ struct Ac3Args {
 
 __device__ __forceinline__ float bf16_to_f(unsigned short h) { return __uint_as_float(((unsigned)h) << 16); }
 
+// Halo cells of the (z, x) plane tile that are not owned by a thread of the
+// block: 10 z-halo rows of the 16 interior columns, then the 10 x-halo
+// columns over the full 74 rows.
+constexpr int AHALO = 2 * AH * AX + 2 * AH * ALZ;
+constexpr int ANH = (AHALO + 255) / 256;
+
+__device__ __forceinline__ void halo_cell(int h, int& lx, int& lz) {
+  if (h < 2 * AH * AX) {
+    lx = AH + h / (2 * AH);
+    const int r = h % (2 * AH);
+    lz = r < AH ? r : AZ + r;
+  } else {
+    const int h2 = h - 2 * AH * AX;
+    const int cc = h2 / ALZ;
+    lx = cc < AH ? cc : AX + cc;
+    lz = h2 % ALZ;
+  }
+}
+
+// The march along y is software-pipelined: the global loads of plane y+1
+// (its new queue entry, its LDS halo, and the own-cell p_prev, c, snapshot
+// and image operands) are issued before plane y is computed, so their
+// latency hides behind the arithmetic of the current plane.
+template <bool IMG>
 __global__ __launch_bounds__(256) void k_ac3_step(const Ac3Args a) {
   __shared__ float sh[ALX * ALZ];
   const int nbz = (a.n1 - 10 + AZ - 1) / AZ;
@@ -73,44 +97,79 @@ __global__ __launch_bounds__(256) void k_ac3_step(const Ac3Args a) {
   const int y0 = AH + by * a.ychunk;
   const int y1 = min(y0 + a.ychunk, a.n3 - AH);
   const int tz = threadIdx.x, ty = threadIdx.y;
+  const int tid = tz + AZ * ty;
   const int gz = z0 + tz;
   const bool zok = gz < a.n1 - AH;
-  const long long s2 = a.s2, s3 = a.s3;
-  // register queue: q[k][j] = p at plane y-5+k, column x0+ty*ATXR+j
+  // element offsets of one field fit in 32 bits (the host checks n < 2^31)
+  const int s2 = (int)a.s2, s3 = (int)a.s3;
+  constexpr bool imaging = IMG;  // receiver pass at an imaging level
+  // per-thread halo cells (plane independent): LDS slot (-1: none) and
+  // in-plane offset (-1: outside the grid, staged as zero)
+  int hoff[ANH], hlds[ANH];
+#pragma unroll
+  for (int k = 0; k < ANH; ++k) {
+    const int h = tid + 256 * k;
+    int lx = 0, lz = 0;
+    hlds[k] = -1;
+    hoff[k] = -1;
+    if (h < AHALO) {
+      halo_cell(h, lx, lz);
+      hlds[k] = lx * ALZ + lz;
+      const int gx = x0 - AH + lx, g1 = z0 - AH + lz;
+      if (gx >= 0 && gx < a.n2 && g1 >= 0 && g1 < a.n1) hoff[k] = g1 + gx * s2;
+    }
+  }
+  int coff[ATXR];  // own columns (-1: outside the interior)
+#pragma unroll
+  for (int j = 0; j < ATXR; ++j) {
+    const int gx = x0 + ty * ATXR + j;
+    coff[j] = (zok && gx < a.n2 - AH) ? gz + gx * s2 : -1;
+  }
+  // register queue: q[k][j] = p at plane y-5+k
   float q[11][ATXR];
 #pragma unroll
   for (int k = 0; k < 10; ++k)
 #pragma unroll
+    for (int j = 0; j < ATXR; ++j) q[k][j] = coff[j] >= 0 ? a.p[coff[j] + (y0 - AH + k) * s3] : 0.f;
+  float hv[ANH], ppv[ATXR], cv[ATXR], snv[ATXR], imv[ATXR];
+  auto load_plane = [&](int y, float (&h)[ANH], float (&qn)[ATXR], float (&pv)[ATXR], float (&c)[ATXR],
+                        float (&sn)[ATXR], float (&im)[ATXR]) {
+    const int py = y * s3;
+#pragma unroll
     for (int j = 0; j < ATXR; ++j) {
-      const int gx = x0 + ty * ATXR + j;
-      const bool ok = zok && gx < a.n2 - AH;
-      q[k][j] = ok ? a.p[gz + gx * s2 + (long long)(y0 - AH + k) * s3] : 0.f;
+      const bool ok = coff[j] >= 0;
+      const int o = coff[j] + py;
+      qn[j] = ok ? a.p[o + AH * s3] : 0.f;
+      pv[j] = ok ? a.pp[o] : 0.f;
+      c[j] = ok ? a.c[o] : 0.f;
+      sn[j] = 0.f;
+      im[j] = 0.f;
+      if (imaging && ok) {
+        sn[j] = a.snap_bf16 ? bf16_to_f(static_cast<const unsigned short*>(a.snap)[o]) * a.snap_scale
+                            : static_cast<const float*>(a.snap)[o];
+        im[j] = a.img[o];
+      }
     }
+#pragma unroll
+    for (int k = 0; k < ANH; ++k) h[k] = hoff[k] >= 0 ? a.p[hoff[k] + py] : 0.f;
+  };
+  float qn[ATXR];
+  if (y0 < y1) load_plane(y0, hv, qn, ppv, cv, snv, imv);
   float mx = 0.f;
   for (int y = y0; y < y1; ++y) {
 #pragma unroll
-    for (int j = 0; j < ATXR; ++j) {
-      const int gx = x0 + ty * ATXR + j;
-      const bool ok = zok && gx < a.n2 - AH;
-      q[10][j] = ok ? a.p[gz + gx * s2 + (long long)(y + AH) * s3] : 0.f;
-    }
-    // stage plane y: own cells from the queue, halo from global memory
-    const long long py = (long long)y * s3;
+    for (int j = 0; j < ATXR; ++j) q[10][j] = qn[j];
+    __syncthreads();  // the previous plane's readers are done
 #pragma unroll
     for (int j = 0; j < ATXR; ++j) sh[(ty * ATXR + j + AH) * ALZ + tz + AH] = q[5][j];
-    for (int lx = ty; lx < ALX; lx += AWY) {
-      const int gx = x0 - AH + lx;
-      const bool xin = gx >= 0 && gx < a.n2;
-      const bool xhalo = lx < AH || lx >= AH + AX;
-      for (int lz = tz; lz < ALZ; lz += AZ) {
-        const bool zhalo = lz < AH || lz >= AH + AZ;
-        if (!xhalo && !zhalo) continue;  // interior already from the queue
-        const int g1 = z0 - AH + lz;
-        const bool in = xin && g1 >= 0 && g1 < a.n1;
-        sh[lx * ALZ + lz] = in ? a.p[g1 + gx * s2 + py] : 0.f;
-      }
-    }
+#pragma unroll
+    for (int k = 0; k < ANH; ++k)
+      if (hlds[k] >= 0) sh[hlds[k]] = hv[k];
     __syncthreads();
+    // issue the next plane's loads before this plane's arithmetic
+    float hv2[ANH], pp2[ATXR], c2[ATXR], sn2[ATXR], im2[ATXR];
+    if (y + 1 < y1) load_plane(y + 1, hv2, qn, pp2, c2, sn2, im2);
+    const int py = y * s3;
     if (zok) {
       const float kzv = a.kz ? a.kz[gz] : 1.f;
       const float kyv = a.kz ? a.ky[y] : 1.f;
@@ -118,7 +177,7 @@ __global__ __launch_bounds__(256) void k_ac3_step(const Ac3Args a) {
       for (int j = 0; j < ATXR; ++j) {
         const int lxo = ty * ATXR + j;
         const int gx = x0 + lxo;
-        if (gx >= a.n2 - AH) continue;
+        if (coff[j] < 0) continue;
         const int p0 = (lxo + AH) * ALZ + tz + AH;
         float lap = 3.f * W0 * q[5][j];
         lap += W1 * ((sh[p0 + 1] + sh[p0 - 1]) + (sh[p0 + ALZ] + sh[p0 - ALZ]) + (q[6][j] + q[4][j]));
@@ -126,9 +185,9 @@ __global__ __launch_bounds__(256) void k_ac3_step(const Ac3Args a) {
         lap += W3 * ((sh[p0 + 3] + sh[p0 - 3]) + (sh[p0 + 3 * ALZ] + sh[p0 - 3 * ALZ]) + (q[8][j] + q[2][j]));
         lap += W4 * ((sh[p0 + 4] + sh[p0 - 4]) + (sh[p0 + 4 * ALZ] + sh[p0 - 4 * ALZ]) + (q[9][j] + q[1][j]));
         lap += W5 * ((sh[p0 + 5] + sh[p0 - 5]) + (sh[p0 + 5 * ALZ] + sh[p0 - 5 * ALZ]) + (q[10][j] + q[0][j]));
-        const long long gi = gz + gx * s2 + py;
+        const int gi = coff[j] + py;
         const float kk = a.kz ? kzv * a.kx[gx] * kyv : 1.f;
-        float v = kk * (2.f * q[5][j] - a.pp[gi] + a.c[gi] * lap);
+        float v = kk * (2.f * q[5][j] - ppv[j] + cv[j] * lap);
         if (a.mode == 0) {
           if (gz == a.sz && gx == a.sx && y == a.sy) v += a.amp;
         } else if (a.rec_in && gz == a.zr) {
@@ -138,19 +197,24 @@ __global__ __launch_bounds__(256) void k_ac3_step(const Ac3Args a) {
         if (a.mode == 0) {
           if (a.rec_out && gz == a.zr) a.rec_out[(long long)y * a.n2 + gx] = v;
           mx = fmaxf(mx, fabsf(v));
-        } else if (a.img) {
-          const float s = a.snap_bf16
-                              ? bf16_to_f(static_cast<const unsigned short*>(a.snap)[gi]) * a.snap_scale
-                              : static_cast<const float*>(a.snap)[gi];
-          a.img[gi] += s * v;
+        } else if (imaging) {
+          a.img[gi] = imv[j] + snv[j] * v;
         }
       }
     }
-    __syncthreads();
 #pragma unroll
     for (int k = 0; k < 10; ++k)
 #pragma unroll
       for (int j = 0; j < ATXR; ++j) q[k][j] = q[k + 1][j];
+#pragma unroll
+    for (int k = 0; k < ANH; ++k) hv[k] = hv2[k];
+#pragma unroll
+    for (int j = 0; j < ATXR; ++j) {
+      ppv[j] = pp2[j];
+      cv[j] = c2[j];
+      snv[j] = sn2[j];
+      imv[j] = im2[j];
+    }
   }
   if (a.mode == 0 && a.smax) {
 #pragma unroll
@@ -263,7 +327,10 @@ static int ac3_launch(ac3d* a, const float* p, float* pp, const float* c, int mo
   x.ychunk = 64;
   const int nbz = (g.n1 - 10 + AZ - 1) / AZ, nbx = (g.n2 - 10 + AX - 1) / AX;
   const int nby = (g.n3 - 10 + x.ychunk - 1) / x.ychunk;
-  hipLaunchKernelGGL(k_ac3_step, dim3(nbz * nbx * nby), dim3(AZ, AWY), 0, a->st, x);
+  if (mode == 1 && img)
+    hipLaunchKernelGGL(k_ac3_step<true>, dim3(nbz * nbx * nby), dim3(AZ, AWY), 0, a->st, x);
+  else
+    hipLaunchKernelGGL(k_ac3_step<false>, dim3(nbz * nbx * nby), dim3(AZ, AWY), 0, a->st, x);
   HIPCHK(hipGetLastError());
   return OK;
 }
@@ -333,6 +400,10 @@ RTM_EXPORT int ac3d_create(ac3d** out, const ac3d_cfg_t* cfg, const float* c, co
   std::unique_ptr<ac3d> a(new ac3d());
   a->cfg = g;
   a->n = (size_t)g.n1 * g.n2 * g.n3;
+  if (a->n >= (size_t)1 << 31) {  // the step kernel addresses a field with 32-bit offsets
+    set_error("3D grid of %zu cells exceeds 2^31 cells per field", a->n);
+    return E_ARG;
+  }
   a->plane = (long long)g.n2 * g.n3;
   HIPCHK(hipStreamCreateWithFlags(&a->st, hipStreamNonBlocking));
   for (auto& e : a->ev) HIPCHK(hipEventCreate(&e));

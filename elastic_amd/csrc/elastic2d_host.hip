@@ -54,7 +54,10 @@ static int op3(int nt, int nz, int nx, float* const* hist, const float* const* c
     if (!hist[f] || !coefs[f]) { set_error("null array argument"); return E_ARG; }
   if (nt < 3) return OK;  // no level below nt-2 to update
   STCHK(ensure_device());
-  const int ldz = device_ldz(nz);
+  // The caller's layout, unpadded: each field's history moves in one
+  // contiguous copy (a pitched copy of nx*nt rows from pageable memory runs
+  // at a small fraction of the link rate).
+  const int ldz = nz;
   const long long n2 = (long long)ldz * nx;
   size_t freeb = 0, totb = 0;
   HIPCHK(hipMemGetInfo(&freeb, &totb));
@@ -80,8 +83,8 @@ static int op3(int nt, int nz, int nx, float* const* hist, const float* const* c
   STCHK(fill_masked_coefs(c5.data(), nz, nx, ldz, cf.as<float>()));
   float* hp = h.as<float>();
   for (int f = 0; f < 5; ++f)
-    HIPCHK(hipMemcpy2DAsync(hp + (size_t)f * nt * n2, ldz * sizeof(float), hist[f], nz * sizeof(float),
-                            nz * sizeof(float), (size_t)nx * nt, hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(hp + (size_t)f * nt * n2, hist[f], (size_t)nt * n2 * sizeof(float), hipMemcpyHostToDevice,
+                          st));
   // one shot, history layout: field f of 0-based level L at hp + L*n2 + f*nt*n2
   ShotDev d{};
   d.nz = nz; d.nx = nx; d.ldz = ldz; d.nt = nt; d.n2 = n2;
@@ -103,8 +106,8 @@ static int op3(int nt, int nz, int nx, float* const* hist, const float* const* c
                      default_xcd(), st));
   HIPCHK(hipEventRecord(e1, st));
   for (int f = 0; f < 5; ++f)
-    HIPCHK(hipMemcpy2DAsync(hist[f], nz * sizeof(float), hp + (size_t)f * nt * n2, ldz * sizeof(float),
-                            nz * sizeof(float), (size_t)nx * nt, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipMemcpyAsync(hist[f], hp + (size_t)f * nt * n2, (size_t)nt * n2 * sizeof(float), hipMemcpyDeviceToHost,
+                          st));
   HIPCHK(hipStreamSynchronize(st));
   float ms = 0;
   HIPCHK(hipEventElapsedTime(&ms, e0, e1));
