@@ -267,3 +267,23 @@ def test_movie_frames_are_forward_levels(hip, tmp_path):
     cube, h = sep.read(paths["Vz0"])
     assert cube.shape == (wnz, wnx, 6) and np.array_equal(cube[:, :, 2], movie[2, 1])
     assert sep.axes(h)[2][2] == pytest.approx(1e-3 * dsnap)
+
+
+@pytest.mark.parametrize("tblock", ["0", "1"])
+def test_sponge_free_forward_and_migrate(hip, monkeypatch, tblock):
+    """abc = 0 (no sponge anywhere; the kernels get no sponge factors):
+    forward levels and images against the FP64 oracle, one and two levels per
+    launch."""
+    monkeypatch.setenv("RTM_TBLOCK", tblock)
+    p, w, coefs, m, geo = _shot(nt=180)
+    p.abc = 0
+    rec, last = hip.forward(_hip_shot(hip, p), coefs, w, want_last=True)
+    ref = O.forward(p, w, coefs, np.float64, want_hist5=True)
+    assert O.rel_l2(rec, ref["rec"]) <= WAVE_TOL
+    for f in range(5):
+        assert O.rel_l2(last[5 + f], ref["hist5"][f][:, :, p.nt - 1]) <= WAVE_TOL, f
+    d, cs = _observed(p, w, coefs, m, geo, 1e-3)
+    img = hip.migrate(_hip_shot(hip, p), cs, w, d)
+    refi = O.rtm(p, w, cs, d, np.float64)
+    for a, b in zip(img, refi):
+        assert O.rel_l2(a, b) <= IMG_TOL
