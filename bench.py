@@ -138,12 +138,15 @@ def cpu_baseline(lib, synth, seed, dx, dt, fp, abc, nt_sample, nthreads):
     return phys / dtm / 1e6, dtm, sp
 
 
-def load_traffic(variant):
+def load_traffic(variant, kernel):
+    """Measured HBM bytes per cell-update of a kernel (profiles/traffic_<variant>.json:
+    one entry per kernel form, from rocprofv3 FETCH_SIZE / WRITE_SIZE passes)."""
     path = os.path.join(ROOT, "profiles", f"traffic_{variant}.json")
-    if os.path.exists(path):
-        with open(path) as f:
-            return json.load(f)
-    return None
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        d = json.load(f)
+    return d.get(kernel)
 
 
 def acoustic3d(args):
@@ -270,9 +273,12 @@ def main():
     # HBM bytes per launch of the dominant kernel: measured bytes per
     # cell-update (rocprofv3 FETCH_SIZE / WRITE_SIZE passes, calibrated on a
     # copy kernel) times this run's cell-updates per launch
-    tr = load_traffic(args.variant)
+    # kernel form the batch runs (autotuned unless --txr / RTM_TXR / RTM_TBLOCK fix it)
+    form, txr = b.tblock, b.txr
+    kname = ("k_" + dom) + {0: "", 1: "2", 2: "m"}[form]
+    tr = load_traffic(args.variant, kname)
     traffic = None
-    if tr and tr.get("kernel") == ("k_adj" if dom == "adj" else "k_fwd"):
+    if tr:
         cells_per_launch = (info["adj_cells"] / info["adj_launches"]) if dom == "adj" else \
             (info["fwd_cells"] / info["fwd_launches"])
         traffic = round(tr["bytes_per_cell_update"] * cells_per_launch)
@@ -311,12 +317,15 @@ def main():
         "config": {"workload": f"batched 2D elastic RTM, {args.shots} shots, model {args.nz}x{args.nx}, dx=10 m, "
                                f"dt=0.8 ms, {args.nt} levels, 15 Hz Ricker, apertures 600-1200 columns, 40-pt sponge",
                    "variant": args.variant, "shots": args.shots, "nt": args.nt,
-                   "shots_per_group": int(info["group_size"])},
+                   "shots_per_group": int(info["group_size"]),
+                   "kernel_form": {0: "tiles, one level per launch", 1: "tiles, two levels per launch",
+                                   2: "one wave per strip marching along x"}[form],
+                   "tile_width": txr},
         "shots_per_hour": round(shots_per_hour, 1),
         "mcells_computational": round(comp / (ms_step * 1e-3) / 1e6, 1),
         "image_rel_l2_vs_fp64": max_err,
         "valid": (max_err is not None and max_err <= 1e-3),
-        "roofline": {"bound": "hbm", "kernel": "k_adj" if dom == "adj" else "k_fwd", "achieved": round(achieved, 1),
+        "roofline": {"bound": "hbm", "kernel": kname, "achieved": round(achieved, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": traffic,
                      "traffic_per_cell_update": tr["bytes_per_cell_update"] if traffic else None,

@@ -8,10 +8,11 @@
 // levels, the same options) once per width and keeps the fastest. Results
 // are cached for the process, keyed by the workload signature, and, when
 // RTM_TUNE_CACHE names a file, in that file (one "key code" line per entry,
-// code = txr + 16 * two_levels_per_launch). The two-level (temporally
-// blocked) kernels are candidates too: they halve the HBM traffic of the
-// fields but recompute a radius-5 ring of the first level and run at lower
-// occupancy, so which form wins is measured, not assumed.
+// code = txr + 16 * form). The two-level (temporally blocked) kernels are
+// candidates too: they halve the HBM traffic of the fields but recompute a
+// radius-5 ring of the first level and run at lower occupancy; so is the
+// marching form (one wave per strip walking along x, halo columns read once
+// per strip). Which form wins is measured, not assumed.
 #include "engine2d.hpp"
 
 #include <cstdio>
@@ -60,7 +61,8 @@ static void load_file_cache() {
   int txr = 0;
   while (std::fscanf(f, "%511s %d", key, &txr) == 2) {
     const int w = txr & 15, tb = txr >> 4;
-    if ((w == 1 || w == 2 || w == 4 || w == 8) && (tb == 0 || (tb == 1 && w <= 4))) g_tune_cache[key] = txr;
+    if ((w == 1 || w == 2 || w == 4 || w == 8) && (tb == 0 || (tb == 1 && w <= 4) || (tb == 2 && w == 2)))
+      g_tune_cache[key] = txr;
   }
   std::fclose(f);
 }
@@ -104,17 +106,16 @@ int autotune_txr(const EngineCfg& cfg, const std::vector<ShotSpec>& shots, const
   c.nt = std::min(c.nt, cfg.nt);
   c.stack_nz = c.stack_nx = 0;
   c.max_group = 0;
-  // candidates: widths 1, 2, 4, 8 with one level per launch (slots 0..3)
-  // and widths 1, 2, 4 with two levels per launch (slots 4..6)
-  const int cand[4] = {1, 2, 4, 8};
+  // candidates (width, form): widths 1, 2, 4, 8 with one level per launch
+  // (slots 0..3), widths 1, 2, 4 with two levels per launch (slots 4..6) and
+  // the marching form (slot 7)
+  const int cand[8][2] = {{1, 0}, {2, 0}, {4, 0}, {8, 0}, {1, 1}, {2, 1}, {4, 1}, {2, 2}};
   double t[8];
   int arg = -1;
   for (int i = 0; i < 8; ++i) {
     t[i] = -1.0;
-    const int tb = i >= 4, txr = cand[i & 3];
-    if (tb && txr > 4) continue;
-    c.txr = txr;
-    c.tblock = tb;
+    c.txr = cand[i][0];
+    c.tblock = cand[i][1];
     Engine e;
     STCHK(e.setup(c, sp, cf, wavelet));
     STCHK(e.migrate());  // warm-up (and graph instantiation)
@@ -122,8 +123,8 @@ int autotune_txr(const EngineCfg& cfg, const std::vector<ShotSpec>& shots, const
     t[i] = e.timing().total_ms;
     if (arg < 0 || t[i] < t[arg]) arg = i;
   }
-  *best = cand[arg & 3];
-  *best_tblock = arg >= 4;
+  *best = cand[arg][0];
+  *best_tblock = cand[arg][1];
   if (ms_out)
     for (int i = 0; i < 8; ++i) ms_out[i] = t[i];
   std::lock_guard<std::mutex> lk(g_tune_mu);

@@ -91,7 +91,11 @@ static int op3(int nt, int nz, int nx, float* const* hist, const float* const* c
   d.lev = hp; d.lev_stride = n2; d.fstride = (long long)nt * n2; d.lev_mod = nt; d.lev_off = 0;
   d.coef = cf.as<float>();
   const int txr = default_txr();
-  const int tx = tile_x(txr);
+  // kernel form 2 (RTM_TBLOCK=2): one wave per strip of RTM_MARCH_W columns
+  const bool march = default_tblock() == 2;
+  const int mw = default_march_w();
+  if (march && (mw < 2 || (mw & 1))) { set_error("RTM_MARCH_W=%d is not a positive even number", mw); return E_ARG; }
+  const int tx = march ? mw : tile_x(txr);
   const int nbz = (nz + TILE_Z - 1) / TILE_Z, nbx = (nx + tx - 1) / tx;
   std::vector<TileRef> tiles;
   for (int bx = 0; bx < nbx; ++bx)
@@ -101,9 +105,14 @@ static int op3(int nt, int nz, int nx, float* const* hist, const float* const* c
   HIPCHK(hipMemcpyAsync(dd.p, &d, sizeof(ShotDev), hipMemcpyHostToDevice, st));
   HIPCHK(hipMemcpyAsync(dt.p, tiles.data(), tiles.size() * sizeof(TileRef), hipMemcpyHostToDevice, st));
   HIPCHK(hipEventRecord(e0, st));
-  for (int it = nt - 3; it >= 0; --it)
-    STCHK(launch_adj(dd.as<ShotDev>(), dt.as<TileRef>(), (int)tiles.size(), it, A_XIN, txr, SNAP_HIST,
-                     default_xcd(), st));
+  for (int it = nt - 3; it >= 0; --it) {
+    if (march)
+      STCHK(launch_adjm(dd.as<ShotDev>(), dt.as<TileRef>(), (int)tiles.size(), it, A_XIN, mw, SNAP_HIST,
+                        default_xcd(), st));
+    else
+      STCHK(launch_adj(dd.as<ShotDev>(), dt.as<TileRef>(), (int)tiles.size(), it, A_XIN, txr, SNAP_HIST,
+                       default_xcd(), st));
+  }
   HIPCHK(hipEventRecord(e1, st));
   for (int f = 0; f < 5; ++f)
     HIPCHK(hipMemcpyAsync(hist[f], hp + (size_t)f * nt * n2, (size_t)nt * n2 * sizeof(float), hipMemcpyDeviceToHost,
@@ -133,6 +142,7 @@ static int single_engine(Engine& e, const rtm_shot_t* s, const float* coef5, con
   cfg.txr = default_txr();
   cfg.xcd = default_xcd();
   cfg.tblock = default_tblock();
+  cfg.march_w = default_march_w();
   cfg.use_graph = getenv("RTM_GRAPH") ? atoi(getenv("RTM_GRAPH")) : 0;
   ShotSpec p{};
   p.nz = s->nz; p.nx = s->nx; p.xs = s->xs; p.zs = s->zs; p.zr = s->zr; p.rx0 = s->rx0; p.nrx = s->nrx;
@@ -363,6 +373,7 @@ RTM_EXPORT int rtm_forward_movie_hip(const rtm_shot_t* s, const float* coef5, co
   cfg.txr = default_txr();
   cfg.xcd = default_xcd();
   cfg.tblock = default_tblock();
+  cfg.march_w = default_march_w();
   cfg.movie_every = every;
   cfg.movie_n = nsnap;
   cfg.movie_z0 = z0; cfg.movie_x0 = x0; cfg.movie_nz = wnz; cfg.movie_nx = wnx;
@@ -581,6 +592,7 @@ RTM_EXPORT int rtm_batch_create(rtm_batch** out, const rtm_batch_cfg_t* cfg, int
   const bool tune = cfg->txr == 0 && !getenv("RTM_TXR") && !getenv("RTM_TBLOCK");
   ec.xcd = default_xcd();
   ec.tblock = default_tblock();
+  ec.march_w = default_march_w();
   ec.use_graph = cfg->use_graph;
   ec.max_group = cfg->max_group;
   ec.stack_nz = cfg->stack_nz;
@@ -618,6 +630,7 @@ RTM_EXPORT int rtm_batch_autotune(const rtm_batch_cfg_t* cfg, int nshots, const 
   ec.comp = cfg->comp_image;
   ec.xcd = default_xcd();
   ec.tblock = default_tblock();
+  ec.march_w = default_march_w();
   ec.use_graph = cfg->use_graph;
   std::vector<ShotSpec> sp(nshots);
   std::vector<const float*> cf(coef5, coef5 + nshots);

@@ -113,6 +113,30 @@ __device__ __forceinline__ float* lev_ptr(float* lev, long long stride, int mod,
   return lev + (long long)slot * stride;
 }
 
+// Global-memory accesses through address-space-1 pointers. The descriptor's
+// pointers are generic; read through them the compiler emits FLAT
+// instructions, which also count against the LDS wait counter, so waiting
+// for an LDS read would wait for every global load in flight as well.
+template <class T>
+__device__ __forceinline__ T gld(const T* p, int i) {
+  return ((const __attribute__((address_space(1))) T*)p)[i];
+}
+template <class T>
+__device__ __forceinline__ void gst(T* p, int i, T v) {
+  ((__attribute__((address_space(1))) T*)p)[i] = v;
+}
+template <class T>
+__device__ __forceinline__ void gst_nt(T* p, int i, T v) {
+  __builtin_nontemporal_store(v, (__attribute__((address_space(1))) T*)p + i);
+}
+// Masked load without a branch: lanes outside read element 0 (a valid
+// address) and return zero.
+template <class T>
+__device__ __forceinline__ T gldm(const T* p, int i, bool in) {
+  const T v = gld(p, in ? i : 0);
+  return in ? v : T(0);
+}
+
 // ---------------------------------------------------------------------------
 // Forward step, W/rtm-mod-adj.f90:225-252 (Born propagation :323-348):
 //   Vx  = K(Vx(L-2)  + m1_x (S-x sxx + S+z sxz))            Ax, Bx
@@ -157,7 +181,7 @@ __global__ __launch_bounds__(256) void k_fwd(const ShotDev* __restrict__ shots,
     const bool in = c < T::N && gx >= 0 && gx < nx && gz >= 0 && gz < nz;
     const int gi = gz + ldz * gx;
 #pragma unroll
-    for (int f = 0; f < 5; ++f) st[k][f] = in ? y1[f * fs + gi] : 0.f;
+    for (int f = 0; f < 5; ++f) st[k][f] = gldm(y1 + f * fs, gi, in);
   }
   const int gz = z0 + tz;
   const bool zok = gz < nz;
@@ -169,8 +193,8 @@ __global__ __launch_bounds__(256) void k_fwd(const ShotDev* __restrict__ shots,
     const int gi = gz + ldz * gx;
 #pragma unroll
     for (int f = 0; f < 5; ++f) {
-      ox[j][f] = ok ? x2[f * fs + gi] : 0.f;
-      oc[j][f] = ok ? coef[f * n2 + gi] : 0.f;
+      ox[j][f] = gld(x2 + f * fs, ok ? gi : 0);
+      oc[j][f] = gld(coef + f * n2, ok ? gi : 0);
     }
   }
 #pragma unroll
@@ -183,7 +207,7 @@ __global__ __launch_bounds__(256) void k_fwd(const ShotDev* __restrict__ shots,
   }
   __syncthreads();
 
-  const float kz = ((flags & F_K) && zok) ? S.kz[gz] : 1.f;
+  const float kz = ((flags & F_K) && zok) ? gld(S.kz, gz) : 1.f;
   constexpr int SX = T::LZ;
   float sv[4][TXR];
 #pragma unroll
@@ -202,7 +226,7 @@ __global__ __launch_bounds__(256) void k_fwd(const ShotDev* __restrict__ shots,
     for (int f = 0; f < 5; ++f) v[f] = ox[j][f];
     if (flags & F_BORN) {
 #pragma unroll
-      for (int f = 0; f < 5; ++f) v[f] = S.bsrc[f * n2 + gi] + v[f];
+      for (int f = 0; f < 5; ++f) v[f] = gld(S.bsrc + f * n2, gi) + v[f];
     }
     const float dxx_xm = sm<SX>(sh[2], p);  // Ax
     const float dxz_zp = sp<1>(sh[4], p);   // Bx
@@ -217,7 +241,7 @@ __global__ __launch_bounds__(256) void k_fwd(const ShotDev* __restrict__ shots,
     v[1] = fmaf(m1z, dxz_xp, v[1]);
     v[1] = fmaf(m1z, dzz_zm, v[1]);
     if ((flags & F_SRC) && gz == S.zs && gx == S.xs) {
-      const float amp = S.src_amp[L - 1];
+      const float amp = gld(S.src_amp, L - 1);
       v[2] = v[2] + amp;
       v[3] = v[3] + amp;
     }
@@ -228,16 +252,16 @@ __global__ __launch_bounds__(256) void k_fwd(const ShotDev* __restrict__ shots,
     v[4] = fmaf(m3, dvx_zm, v[4]);
     v[4] = fmaf(m3, dvz_xm, v[4]);
     if (flags & F_K) {
-      const float kx = S.kx[gx];
+      const float kx = gld(S.kx, gx);
 #pragma unroll
       for (int f = 0; f < 5; ++f) v[f] = (v[f] * kz) * kx;
     }
     if (flags & F_NTS) {
 #pragma unroll
-      for (int f = 0; f < 5; ++f) __builtin_nontemporal_store(v[f], out + f * fs + gi);
+      for (int f = 0; f < 5; ++f) gst_nt(out + f * fs, gi, v[f]);
     } else {
 #pragma unroll
-      for (int f = 0; f < 5; ++f) out[f * fs + gi] = v[f];
+      for (int f = 0; f < 5; ++f) gst(out + f * fs, gi, v[f]);
     }
     if ((flags & (F_REC | F_RECSUB)) && gz == S.zr && gx >= S.rx0 && gx < S.rx0 + S.nrx) {
       float* r = S.rec + (long long)(L - 1) * S.nrx + (gx - S.rx0);
@@ -245,15 +269,15 @@ __global__ __launch_bounds__(256) void k_fwd(const ShotDev* __restrict__ shots,
 #pragma unroll
       for (int f = 0; f < 5; ++f) {
         if (flags & F_RECSUB)
-          r[f * rs] = r[f * rs] - v[f];
+          gst(r + f * rs, 0, gld(r + f * rs, 0) - v[f]);
         else
-          r[f * rs] = v[f];
+          gst(r + f * rs, 0, v[f]);
       }
     }
     if (SNAP == SNAP_HIST && (flags & F_SNAP)) {
       float* h = static_cast<float*>(S.snap) + (long long)(L - 1) * 2 * (long long)n2;
-      h[gi] = v[0];
-      h[n2 + gi] = v[1];
+      gst(h, gi, v[0]);
+      gst(h + n2, gi, v[1]);
     }
     if (SNAP != SNAP_HIST) {
       sv[0][j] = sh[0][p];
@@ -274,7 +298,7 @@ __global__ __launch_bounds__(256) void k_fwd(const ShotDev* __restrict__ shots,
         if (!zok || gx >= nx) continue;
         const int gi = gz + ldz * gx;
 #pragma unroll
-        for (int a = 0; a < 4; ++a) sp4[a * n2 + gi] = sv[a][j];
+        for (int a = 0; a < 4; ++a) gst(sp4 + a * n2, gi, sv[a][j]);
       }
     } else {
       unsigned short* q4 = static_cast<unsigned short*>(S.snap) + k0 * 4 * n2;
@@ -286,13 +310,13 @@ __global__ __launch_bounds__(256) void k_fwd(const ShotDev* __restrict__ shots,
         for (int j = 0; j < TXR; ++j) m = fmaxf(m, fabsf(sv[a][j]));
         m = wave_max(m);
         const float inv = m > kSnapFloor ? SnapQ<SNAP>::Q / m : 0.f;
-        if (tz == 0) S.snap_scale[(k0 * 4 + a) * S.snap_nstrip + strip] = m / SnapQ<SNAP>::Q;
+        if (tz == 0) gst(S.snap_scale + (k0 * 4 + a) * S.snap_nstrip, strip, m / SnapQ<SNAP>::Q);
 #pragma unroll
         for (int j = 0; j < TXR; ++j) {
           const int gx = x0 + ty * TXR + j;
           if (!zok || gx >= nx) continue;
           const int gi = gz + ldz * gx;
-          q4[a * n2 + gi] = SnapQ<SNAP>::enc(sv[a][j] * inv);
+          gst(q4 + a * n2, gi, SnapQ<SNAP>::enc(sv[a][j] * inv));
         }
       }
     }
@@ -663,8 +687,12 @@ struct SrcView {
     nbz = S.snap_nbz;
     sw = S.snap_sw;
     if (SNAP == SNAP_HIST) {
+      // arrays 2, 3 (level L+1) exist below the last level only; at the last
+      // level they alias arrays 0, 1 so that every pointer stays inside the
+      // history (element 0 of each array is read by masked loads)
       const float* h = static_cast<const float*>(S.snap) + (long long)(L - 1) * 2 * n2;
-      f[0] = h; f[1] = h + n2; f[2] = h + 2 * n2; f[3] = h + 3 * n2;
+      const bool nxt = L < S.nt;
+      f[0] = h; f[1] = h + n2; f[2] = nxt ? h + 2 * n2 : h; f[3] = nxt ? h + 3 * n2 : h + n2;
     } else {
       const long long k0 = L / S.snap_every - 1;
 #pragma unroll
@@ -683,6 +711,33 @@ struct SrcView {
     if (SNAP == SNAP_HIST || SNAP == SNAP_F32) return f[a][gi];
     const int strip = (gz >> 6) + nbz * (gx >> sw);
     return SnapQ<SNAP>::dec(q[a][gi]) * sc[a][strip];
+  }
+  // an element as loaded (clamped index), decoded and masked by value():
+  // loads and their use can be a column apart
+  struct Raw {
+    float v;
+    unsigned short h;
+    bool in;
+  };
+  __device__ __forceinline__ void load(int a, int gi, int gz, int gx, bool in, Raw& r) const {
+    r.in = in;
+    if (SNAP == SNAP_HIST || SNAP == SNAP_F32) {
+      r.v = gld(f[a], in ? gi : 0);
+    } else {
+      r.h = gld(q[a], in ? gi : 0);
+      r.v = gld(sc[a], in ? (gz >> 6) + nbz * (gx >> sw) : 0);
+    }
+  }
+  __device__ __forceinline__ float value(const Raw& r) const {
+    if (SNAP == SNAP_HIST || SNAP == SNAP_F32) return r.in ? r.v : 0.f;
+    return r.in ? SnapQ<SNAP>::dec(r.h) * r.v : 0.f;
+  }
+  // the same as a branch-free global load, zero where !in
+  __device__ __forceinline__ float getm(int a, int gi, int gz, int gx, bool in) const {
+    if (SNAP == SNAP_HIST || SNAP == SNAP_F32) return gldm(f[a], gi, in);
+    const int strip = in ? (gz >> 6) + nbz * (gx >> sw) : 0;
+    const float v = SnapQ<SNAP>::dec(gld(q[a], in ? gi : 0)) * gld(sc[a], strip);
+    return in ? v : 0.f;
   }
 };
 
@@ -726,18 +781,21 @@ __global__ __launch_bounds__(256) void k_adj(const ShotDev* __restrict__ shots,
     const int lx = c / T::LZ, lz = c - lx * T::LZ;
     const int gx = x0 - T::H + lx, gz = z0 - T::H + lz;
     const bool in = c < T::N && gx >= 0 && gx < nx && gz >= 0 && gz < nz;
+    // With the fp32 history out-of-range cells issue no loads; with
+    // decimated snapshots the loads are branch-free (clamped index, masked
+    // products). Each form measured faster for its format.
     float p0 = 0.f, p1 = 0.f, p2 = 0.f, p3 = 0.f, p4 = 0.f;
-    if (in) {
-      const int gi = gz + ldz * gx;
-      const float vx = y1[gi], vz = y1[fs + gi], xx = y1[2 * fs + gi], zz = y1[3 * fs + gi],
-                  xz = y1[4 * fs + gi];
-      const float m1x = coef[gi], m1z = coef[n2 + gi], m2 = coef[2 * n2 + gi],
-                  m3 = coef[3 * n2 + gi], m2m3 = coef[4 * n2 + gi];
-      p0 = fmaf(m2m3, xx, m2 * zz);
-      p1 = fmaf(m2, xx, m2m3 * zz);
-      p2 = m3 * xz;
-      p3 = m1x * vx;
-      p4 = m1z * vz;
+    if (SNAP != SNAP_HIST || in) {
+      const int gi = in ? gz + ldz * gx : 0;
+      const float vx = gld(y1, gi), vz = gld(y1 + fs, gi), xx = gld(y1 + 2 * fs, gi), zz = gld(y1 + 3 * fs, gi),
+                  xz = gld(y1 + 4 * fs, gi);
+      const float m1x = gld(coef, gi), m1z = gld(coef + n2, gi), m2 = gld(coef + 2 * n2, gi),
+                  m3 = gld(coef + 3 * n2, gi), m2m3 = gld(coef + 4 * n2, gi);
+      p0 = in ? fmaf(m2m3, xx, m2 * zz) : 0.f;
+      p1 = in ? fmaf(m2, xx, m2m3 * zz) : 0.f;
+      p2 = in ? m3 * xz : 0.f;
+      p3 = in ? m1x * vx : 0.f;
+      p4 = in ? m1z * vz : 0.f;
     }
     st[k][0] = p0;
     st[k][1] = p1;
@@ -755,15 +813,15 @@ __global__ __launch_bounds__(256) void k_adj(const ShotDev* __restrict__ shots,
     const bool ok = zok && gx < nx;
     const int gi = gz + ldz * gx;
 #pragma unroll
-    for (int f = 0; f < 5; ++f) b[j][f] = ok ? x2[f * fs + gi] : 0.f;
-    if (ok && (flags & A_XIN)) {
+    for (int f = 0; f < 5; ++f) b[j][f] = gld(x2 + f * fs, ok ? gi : 0);  // used where ok only
+    if (flags & A_XIN) {
 #pragma unroll
-      for (int f = 0; f < 5; ++f) b[j][f] = out[f * fs + gi] + b[j][f];
+      for (int f = 0; f < 5; ++f) b[j][f] = gld(out + f * fs, ok ? gi : 0) + b[j][f];
     } else if (ok && row && gx >= S.rx0 && gx < S.rx0 + S.nrx) {
       const float* r = S.rec + (long long)(L - 1) * S.nrx + (gx - S.rx0);
       const long long rs = (long long)S.nt * S.nrx;
 #pragma unroll
-      for (int f = 0; f < 5; ++f) b[j][f] = r[f * rs] + b[j][f];
+      for (int f = 0; f < 5; ++f) b[j][f] = gld(r + f * rs, 0) + b[j][f];
     }
   }
   // Imaging inputs: the source Vx, Vz of level L with their halo, the
@@ -784,8 +842,8 @@ __global__ __launch_bounds__(256) void k_adj(const ShotDev* __restrict__ shots,
       const int gx = x0 - T::H + lx, gz2 = z0 - T::H + lz;
       const bool in = c < T::N && gx >= 0 && gx < nx && gz2 >= 0 && gz2 < nz;
       const int gi = gz2 + ldz * gx;
-      sx[k][0] = in ? sv.get(0, gi, gz2, gx) : 0.f;
-      sx[k][1] = in ? sv.get(1, gi, gz2, gx) : 0.f;
+      sx[k][0] = sv.getm(0, gi, gz2, gx, in);
+      sx[k][1] = sv.getm(1, gi, gz2, gx, in);
     }
     const bool next = flags & A_NEXT;
 #pragma unroll
@@ -793,12 +851,12 @@ __global__ __launch_bounds__(256) void k_adj(const ShotDev* __restrict__ shots,
       const int gx = x0 + ty * TXR + j;
       const bool ok = zok && gx < nx;
       const int gi = gz + ldz * gx;
-      nt2[j][0] = ok && next ? sv.get(2, gi, gz, gx) : 0.f;
-      nt2[j][1] = ok && next ? sv.get(3, gi, gz, gx) : 0.f;
+      nt2[j][0] = next ? sv.getm(2, gi, gz, gx, ok) : 0.f;  // arrays 2, 3 exist below the last level only
+      nt2[j][1] = next ? sv.getm(3, gi, gz, gx, ok) : 0.f;
 #pragma unroll
-      for (int a = 0; a < 3; ++a) {
-        im[j][a] = ok ? S.img[a * n2 + gi] : 0.f;
-        imc[j][a] = ok && (flags & A_COMP) ? S.imgc[a * n2 + gi] : 0.f;
+      for (int a = 0; a < 3; ++a) {  // used where ok only
+        im[j][a] = gld(S.img + a * n2, ok ? gi : 0);
+        imc[j][a] = (flags & A_COMP) ? gld(S.imgc + a * n2, ok ? gi : 0) : 0.f;
       }
     }
   };
@@ -813,7 +871,7 @@ __global__ __launch_bounds__(256) void k_adj(const ShotDev* __restrict__ shots,
   }
   __syncthreads();
 
-  const float kz = ((flags & A_K) && zok) ? S.kz[gz] : 1.f;
+  const float kz = ((flags & A_K) && zok) ? gld(S.kz, gz) : 1.f;
   float o[TXR][5];
   constexpr int SX = T::LZ;
 #pragma unroll
@@ -836,16 +894,16 @@ __global__ __launch_bounds__(256) void k_adj(const ShotDev* __restrict__ shots,
     v4 -= sm<SX>(sh[4], p);
     float v[5] = {v0, v1, v2, v3, v4};
     if (flags & A_K) {
-      const float kx = S.kx[gx];
+      const float kx = gld(S.kx, gx);
 #pragma unroll
       for (int f = 0; f < 5; ++f) v[f] = (v[f] * kz) * kx;
     }
     if (flags & A_NTS) {
 #pragma unroll
-      for (int f = 0; f < 5; ++f) __builtin_nontemporal_store(v[f], out + f * fs + gi);
+      for (int f = 0; f < 5; ++f) gst_nt(out + f * fs, gi, v[f]);
     } else {
 #pragma unroll
-      for (int f = 0; f < 5; ++f) out[f * fs + gi] = v[f];
+      for (int f = 0; f < 5; ++f) gst(out + f * fs, gi, v[f]);
     }
 #pragma unroll
     for (int f = 0; f < 5; ++f) o[j][f] = v[f];
@@ -910,12 +968,12 @@ __global__ __launch_bounds__(256) void k_adj(const ShotDev* __restrict__ shots,
       for (int a = 0; a < 3; ++a) {
         float sa = im[j][a], ca = imc[j][a];
         neumaier(sa, ca, t[a]);
-        img[a * n2 + gi] = sa;
-        imgc[a * n2 + gi] = ca;
+        gst(img + a * n2, gi, sa);
+        gst(imgc + a * n2, gi, ca);
       }
     } else {
 #pragma unroll
-      for (int a = 0; a < 3; ++a) img[a * n2 + gi] = im[j][a] + t[a];
+      for (int a = 0; a < 3; ++a) gst(img + a * n2, gi, im[j][a] + t[a]);
     }
   }
 }
@@ -1076,11 +1134,11 @@ __global__ __launch_bounds__(256) void k_adj2(const ShotDev* __restrict__ shots,
                   xz = y1[4 * fs + gi];
       const float m1x = coef[gi], m1z = coef[n2 + gi], m2 = coef[2 * n2 + gi],
                   m3 = coef[3 * n2 + gi], m2m3 = coef[4 * n2 + gi];
-      p0 = fmaf(m2m3, xx, m2 * zz);
-      p1 = fmaf(m2, xx, m2m3 * zz);
-      p2 = m3 * xz;
-      p3 = m1x * vx;
-      p4 = m1z * vz;
+      p0 = in ? fmaf(m2m3, xx, m2 * zz) : 0.f;
+      p1 = in ? fmaf(m2, xx, m2m3 * zz) : 0.f;
+      p2 = in ? m3 * xz : 0.f;
+      p3 = in ? m1x * vx : 0.f;
+      p4 = in ? m1z * vz : 0.f;
     }
     st[k][0] = p0;
     st[k][1] = p1;
@@ -1262,9 +1320,10 @@ __global__ __launch_bounds__(256) void k_adj2(const ShotDev* __restrict__ shots,
 // ---------------------------------------------------------------------------
 // Born scattering sources of level L, image_op2 forward branch
 // (W/image-mod-rtm.f90:171-179) from the background S1..S6 (fp32 history).
+template <int TXR>
 __global__ __launch_bounds__(256) void k_born_sources(const ShotDev* __restrict__ shots,
                                                       const TileRef* __restrict__ tiles, int L) {
-  using T = Tile<4>;
+  using T = Tile<TXR>;
   __shared__ float sh[2][T::N];
   const TileRef tr = tiles[blockIdx.x];
   const ShotDev& S = shots[tr.shot];
@@ -1289,9 +1348,9 @@ __global__ __launch_bounds__(256) void k_born_sources(const ShotDev* __restrict_
   const int gz = z0 + tz;
   if (gz >= nz) return;
   constexpr int SX = T::LZ;
-  for (int j = 0; j < 4; ++j) {
+  for (int j = 0; j < TXR; ++j) {
 #pragma clang fp contract(off)
-    const int lxo = ty * 4 + j;
+    const int lxo = ty * TXR + j;
     const int gx = x0 + lxo;
     if (gx >= nx) break;
     const int p = (lxo + T::H) * T::LZ + (tz + T::H);
@@ -1317,6 +1376,489 @@ __global__ __launch_bounds__(256) void k_born_sources(const ShotDev* __restrict_
     src[2 * n2 + gi] = fmaf(-(d3 * 2.f), s4, -(d2 * s3));
     src[3 * n2 + gi] = fmaf(-(d3 * 2.f), s5, -(d2 * s3));
     src[4 * n2 + gi] = -(d3 * s6);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Marching step kernels: one wave (64 lanes = 64 z-rows, the fast axis) walks
+// a strip of columns x0 .. x0+xw-1 along x. The x-differences come from
+// 11-deep register queues (columns x-5 .. x+5 of each field the x-stencils
+// read), so a column is loaded once per strip instead of once per tile it
+// is a halo of; only the z-differences go through LDS: the centre column of
+// the fields they read plus 5 halo rows on each side, loaded by lanes 0..9.
+// The next column's operands are issued before the current column is
+// computed. Same arithmetic, in the same order, as k_fwd / k_adj, so the
+// results are bitwise the same.
+
+constexpr int MZ = TILE_Z + 10;  // LDS column: rows z0-5 .. z0+68
+
+// x-differences of a queue q[k] = column x-5+k (same operations and order
+// as sp / sm)
+__device__ __forceinline__ float spq(const float (&q)[11]) {
+  float r = C1 * (q[10] - q[1]);
+  r = fmaf(C2, q[9] - q[2], r);
+  r = fmaf(C3, q[8] - q[3], r);
+  r = fmaf(C4, q[7] - q[4], r);
+  r = fmaf(C5, q[6] - q[5], r);
+  return r;
+}
+__device__ __forceinline__ float smq(const float (&q)[11]) {
+  float r = C1 * (q[9] - q[0]);
+  r = fmaf(C2, q[8] - q[1], r);
+  r = fmaf(C3, q[7] - q[2], r);
+  r = fmaf(C4, q[6] - q[3], r);
+  r = fmaf(C5, q[5] - q[4], r);
+  return r;
+}
+__device__ __forceinline__ void push(float (&q)[11], float v) {
+#pragma unroll
+  for (int k = 0; k < 10; ++k) q[k] = q[k + 1];
+  q[10] = v;
+}
+
+// LDS exchange between the lanes of one wave (the marching kernels run one
+// wave per workgroup). LDS operations of a wave are performed in order, so
+// wavefront-scope fences suffice: they keep the compiler from moving LDS
+// accesses across the exchange and emit no wait, unlike a workgroup
+// barrier, whose fence would also wait for the global loads of the next
+// column that are meant to stay in flight.
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Lane geometry of a marching wave: own row gz = z0 + lane; lanes 0..9 also
+// carry one halo row hz (z0-5 .. z0-1, z0+64 .. z0+68) at LDS slot hs.
+struct MarchLane {
+  int lane, gz, hz, hs;
+  bool zok, hok;
+  __device__ MarchLane(int z0, int nz) {
+    lane = threadIdx.x;
+    gz = z0 + lane;
+    zok = gz < nz;
+    hz = lane < 5 ? z0 - 5 + lane : z0 + 59 + lane;
+    hs = lane < 5 ? lane : TILE_Z + lane;
+    hok = lane < 10 && hz >= 0 && hz < nz;
+  }
+};
+
+template <int SNAP, bool BORN>
+__global__ __launch_bounds__(64) void k_fwdm(const ShotDev* __restrict__ shots, const TileRef* __restrict__ strips,
+                                            int nstrips, int L, int flags, int xw, int xcd) {
+  __shared__ float col[2][4][MZ];  // sigma_zz, sigma_xz, Vx, Vz of the centre column
+  const TileRef tr = strips[xcd ? xcd_tile(blockIdx.x, nstrips) : (int)blockIdx.x];
+  const ShotDev& S = shots[tr.shot];
+  const int nz = S.nz, nx = S.nx, ldz = S.ldz;
+  const int fs = (int)S.fstride, n2 = (int)S.n2;
+  float* const out = lev_ptr(S.lev, S.lev_stride, S.lev_mod, S.lev_off, L);
+  const float* const x2 = lev_ptr(S.lev, S.lev_stride, S.lev_mod, S.lev_off, L - 2);
+  const float* const y1 = lev_ptr(S.lev, S.lev_stride, S.lev_mod, S.lev_off, L - 1);
+  const float* const coef = S.coef;
+  const float* const bsrc = S.bsrc;
+  const int z0 = tr.z0, xb = tr.x0, xe = min(tr.x0 + xw, nx);
+  const MarchLane m(z0, nz);
+  const int gz = m.gz;
+  const bool born = BORN;  // Born sources (F_BORN) in their own instance: 10 fewer live registers
+
+  // queues: sigma_xx (S-x), sigma_xz (S+x), Vx (S+x), Vz (S-x)
+  float qa[11], qb[11], qc[11], qd[11];
+#pragma unroll
+  for (int k = 0; k < 11; ++k) {
+    const int gx = xb - 5 + k;
+    const bool in = m.zok && gx >= 0 && gx < nx;
+    qa[k] = gldm(y1 + 2 * fs, gz + ldz * gx, in);
+    qb[k] = gldm(y1 + 4 * fs, gz + ldz * gx, in);
+    qc[k] = gldm(y1, gz + ldz * gx, in);
+    qd[k] = gldm(y1 + fs, gz + ldz * gx, in);
+  }
+  // Operands of a column: centre sigma_zz, halo rows of the LDS fields, own
+  // cell level L-2 (+ Born sources) and coefficients. Loads are issued with
+  // clamped indices and masked where the values are taken, so that nothing
+  // waits for them before the next column.
+  struct Col {
+    float zz, h[4], o[5], c[5], bs[5];
+  };
+  auto load_col = [&](int gx, bool more, Col& v) {
+    const int ic = m.zok && more ? gz + ldz * gx : 0;
+    const int ih = m.hok && more ? m.hz + ldz * gx : 0;
+    v.zz = gld(y1 + 3 * fs, ic);
+    v.h[0] = gld(y1 + 3 * fs, ih);
+    v.h[1] = gld(y1 + 4 * fs, ih);
+    v.h[2] = gld(y1, ih);
+    v.h[3] = gld(y1 + fs, ih);
+#pragma unroll
+    for (int f = 0; f < 5; ++f) {
+      v.o[f] = gld(x2 + f * fs, ic);
+      v.c[f] = gld(coef + f * n2, ic);
+      v.bs[f] = born ? gld(bsrc + f * n2, ic) : 0.f;
+    }
+  };
+  Col cur;
+  load_col(xb, true, cur);
+  bool cur_h = m.hok;  // halo values of the current column are valid
+  const float kz = ((flags & F_K) && m.zok) ? gld(S.kz, gz) : 1.f;
+  float svp[4] = {0.f, 0.f, 0.f, 0.f};  // previous column's snapshot values (16-bit formats)
+
+  for (int x = xb; x < xe; ++x) {
+    const bool more = x + 1 < xe;
+    const bool qok = m.zok && more && x + 6 < nx;
+    const int iq = qok ? gz + ldz * (x + 6) : 0;
+    const float na = gld(y1 + 2 * fs, iq), nb = gld(y1 + 4 * fs, iq), nc = gld(y1, iq), nd = gld(y1 + fs, iq);
+    Col nxt;
+    load_col(x + 1, more, nxt);
+
+    const int b = (x - xb) & 1;
+    col[b][0][m.lane + 5] = m.zok ? cur.zz : 0.f;
+    col[b][1][m.lane + 5] = qb[5];
+    col[b][2][m.lane + 5] = qc[5];
+    col[b][3][m.lane + 5] = qd[5];
+    if (m.lane < 10) {
+#pragma unroll
+      for (int f = 0; f < 4; ++f) col[b][f][m.hs] = cur_h ? cur.h[f] : 0.f;
+    }
+    wave_lds_sync();
+
+    float sv[4] = {0.f, 0.f, 0.f, 0.f};
+    if (m.zok) {
+      const int p = m.lane + 5;
+      const int gx = x;
+      const int gi = gz + ldz * gx;
+      const float m1x = cur.c[0], m1z = cur.c[1], m2 = cur.c[2], m3 = cur.c[3], m2m3 = cur.c[4];
+      float v[5];
+#pragma unroll
+      for (int f = 0; f < 5; ++f) v[f] = born ? cur.bs[f] + cur.o[f] : cur.o[f];
+      const float dxx_xm = smq(qa);             // Ax
+      const float dxz_zp = sp<1>(col[b][1], p);  // Bx
+      const float dxz_xp = spq(qb);             // Az
+      const float dzz_zm = sm<1>(col[b][0], p);  // Bz
+      const float dvx_xp = spq(qc);             // D, F
+      const float dvz_zp = sp<1>(col[b][3], p);  // E, G
+      const float dvx_zm = sm<1>(col[b][2], p);  // H
+      const float dvz_xm = smq(qd);             // J
+      v[0] = fmaf(m1x, dxx_xm, v[0]);
+      v[0] = fmaf(m1x, dxz_zp, v[0]);
+      v[1] = fmaf(m1z, dxz_xp, v[1]);
+      v[1] = fmaf(m1z, dzz_zm, v[1]);
+      if ((flags & F_SRC) && gz == S.zs && gx == S.xs) {
+        const float amp = gld(S.src_amp, L - 1);
+        v[2] = v[2] + amp;
+        v[3] = v[3] + amp;
+      }
+      v[2] = fmaf(m2m3, dvx_xp, v[2]);
+      v[2] = fmaf(m2, dvz_zp, v[2]);
+      v[3] = fmaf(m2, dvx_xp, v[3]);
+      v[3] = fmaf(m2m3, dvz_zp, v[3]);
+      v[4] = fmaf(m3, dvx_zm, v[4]);
+      v[4] = fmaf(m3, dvz_xm, v[4]);
+      if (flags & F_K) {
+        const float kx = gld(S.kx, gx);
+#pragma unroll
+        for (int f = 0; f < 5; ++f) v[f] = (v[f] * kz) * kx;
+      }
+      if (flags & F_NTS) {
+#pragma unroll
+        for (int f = 0; f < 5; ++f) gst_nt(out + f * fs, gi, v[f]);
+      } else {
+#pragma unroll
+        for (int f = 0; f < 5; ++f) gst(out + f * fs, gi, v[f]);
+      }
+      if ((flags & (F_REC | F_RECSUB)) && gz == S.zr && gx >= S.rx0 && gx < S.rx0 + S.nrx) {
+        float* r = S.rec + (long long)(L - 1) * S.nrx + (gx - S.rx0);
+        const long long rs = (long long)S.nt * S.nrx;
+#pragma unroll
+        for (int f = 0; f < 5; ++f) {
+          if (flags & F_RECSUB)
+            gst(r + f * rs, 0, gld(r + f * rs, 0) - v[f]);
+          else
+            gst(r + f * rs, 0, v[f]);
+        }
+      }
+      if (SNAP == SNAP_HIST && (flags & F_SNAP)) {
+        float* h = static_cast<float*>(S.snap) + (long long)(L - 1) * 2 * (long long)n2;
+        gst(h, gi, v[0]);
+        gst(h + n2, gi, v[1]);
+      }
+      if (SNAP != SNAP_HIST) {
+        sv[0] = qc[5];
+        sv[1] = qd[5];
+        sv[2] = v[0] - qc[5];
+        sv[3] = v[1] - qd[5];
+      }
+    }
+    if (SNAP != SNAP_HIST && (flags & F_SNAP)) {
+      const long long k0 = (L - 1) / S.snap_every - 1;
+      if (SNAP == SNAP_F32) {
+        float* sp4 = static_cast<float*>(S.snap) + k0 * 4 * n2;
+        if (m.zok) {
+#pragma unroll
+          for (int a = 0; a < 4; ++a) gst(sp4 + a * n2, gz + ldz * x, sv[a]);
+        }
+      } else if ((x & 1) || x + 1 == nx) {
+        // 16-bit formats: one max-abs scale per 64 x 2 strip (columns 2i,
+        // 2i+1; strips start on even columns, and so do strips of a march)
+        unsigned short* q4 = static_cast<unsigned short*>(S.snap) + k0 * 4 * n2;
+        const int strip = (z0 >> 6) + S.snap_nbz * (x >> 1);
+        const bool pair = x & 1;
+#pragma unroll
+        for (int a = 0; a < 4; ++a) {
+          float mx = 0.f;
+          if (pair) mx = fmaxf(mx, fabsf(svp[a]));
+          mx = fmaxf(mx, fabsf(sv[a]));
+          mx = wave_max(mx);
+          const float inv = mx > kSnapFloor ? SnapQ<SNAP>::Q / mx : 0.f;
+          if (m.lane == 0) gst(S.snap_scale + (k0 * 4 + a) * S.snap_nstrip, strip, mx / SnapQ<SNAP>::Q);
+          if (m.zok) {
+            if (pair) gst(q4 + a * n2, gz + ldz * (x - 1), SnapQ<SNAP>::enc(svp[a] * inv));
+            gst(q4 + a * n2, gz + ldz * x, SnapQ<SNAP>::enc(sv[a] * inv));
+          }
+        }
+      }
+#pragma unroll
+      for (int a = 0; a < 4; ++a) svp[a] = sv[a];
+    }
+    push(qa, qok ? na : 0.f);
+    push(qb, qok ? nb : 0.f);
+    push(qc, qok ? nc : 0.f);
+    push(qd, qok ? nd : 0.f);
+    cur = nxt;
+    cur_h = m.hok && more;
+  }
+}
+
+// Adjoint step (gather form, as k_adj) marching along x: the products
+// P0 = m2m3 sxx + m2 szz, P1 = m2 sxx + m2m3 szz, P2 = m3 sxz, P3 = m1x Vx,
+// P4 = m1z Vz of level L+1 are formed once per column as it enters the
+// queues; imaging reads the source Vx, Vz the same way.
+template <int SNAP>
+__global__ __launch_bounds__(64) void k_adjm(const ShotDev* __restrict__ shots, const TileRef* __restrict__ strips,
+                                            int nstrips, int L, int flags, int xw, int xcd) {
+  __shared__ float col[2][6][MZ];  // P1, P2, P3, P4, source Vx, source Vz of the centre column
+  const TileRef tr = strips[xcd ? xcd_tile(blockIdx.x, nstrips) : (int)blockIdx.x];
+  const ShotDev& S = shots[tr.shot];
+  const int nz = S.nz, nx = S.nx, ldz = S.ldz;
+  const int fs = (int)S.fstride, n2 = (int)S.n2;
+  float* const out = lev_ptr(S.lev, S.lev_stride, S.lev_mod, S.lev_off, L);
+  const float* const x2 = lev_ptr(S.lev, S.lev_stride, S.lev_mod, S.lev_off, L + 2);
+  const float* const y1 = lev_ptr(S.lev, S.lev_stride, S.lev_mod, S.lev_off, L + 1);
+  const float* const coef = S.coef;
+  const int z0 = tr.z0, xb = tr.x0, xe = min(tr.x0 + xw, nx);
+  const MarchLane m(z0, nz);
+  const int gz = m.gz;
+  const bool imaging = flags & A_IMG;
+  const bool next = flags & A_NEXT;
+  const bool comp = flags & A_COMP;
+  const bool xin = flags & A_XIN;
+  const SrcView<SNAP> sv(S, L);
+
+  // the ten level-(L+1) operands at element i (clamped index), as loaded
+  struct Raw {
+    float vx, vz, xx, zz, xz, m1x, m1z, m2, m3, m2m3;
+  };
+  auto raw = [&](int i, Raw& w) {
+    w.vx = gld(y1, i);
+    w.vz = gld(y1 + fs, i);
+    w.xx = gld(y1 + 2 * fs, i);
+    w.zz = gld(y1 + 3 * fs, i);
+    w.xz = gld(y1 + 4 * fs, i);
+    w.m1x = gld(coef, i);
+    w.m1z = gld(coef + n2, i);
+    w.m2 = gld(coef + 2 * n2, i);
+    w.m3 = gld(coef + 3 * n2, i);
+    w.m2m3 = gld(coef + 4 * n2, i);
+  };
+
+  // queues: P0 (S-x), P2 (S+x), P3 (S+x), P4 (S-x); P1 at columns x .. x+5
+  float q0[11], q2[11], q3[11], q4[11], d1[6];
+  float s0q[11], s1q[11];  // source Vx (S+x), Vz (S-x)
+#pragma unroll
+  for (int k = 0; k < 11; ++k) {
+    const int gx = xb - 5 + k;
+    const bool in = m.zok && gx >= 0 && gx < nx;
+    Raw w;
+    raw(in ? gz + ldz * gx : 0, w);
+    q0[k] = in ? fmaf(w.m2m3, w.xx, w.m2 * w.zz) : 0.f;
+    q2[k] = in ? w.m3 * w.xz : 0.f;
+    q3[k] = in ? w.m1x * w.vx : 0.f;
+    q4[k] = in ? w.m1z * w.vz : 0.f;
+    if (k >= 5) d1[k - 5] = in ? fmaf(w.m2, w.xx, w.m2m3 * w.zz) : 0.f;
+    if (imaging) {
+      s0q[k] = sv.getm(0, gz + ldz * gx, gz, gx, in);
+      s1q[k] = sv.getm(1, gz + ldz * gx, gz, gx, in);
+    } else {
+      s0q[k] = s1q[k] = 0.f;
+    }
+  }
+  const bool row = (flags & A_INJ) && gz == S.zr;
+  // Operands of a column, loaded with clamped indices and masked where they
+  // are taken: own cell (level L+2, the level's own content for the drop-in
+  // form, the injected receivers), halo rows, imaging operands.
+  struct Col {
+    float b[5], add[5];  // level L+2; the level's own content (drop-in form) or the injected receivers
+    Raw h;
+    typename SrcView<SNAP>::Raw hs[2], nt2[2];
+    float im[3], imc[3];
+  };
+  auto load_col = [&](int gx, bool more, Col& c) {
+    const int ic = m.zok && more ? gz + ldz * gx : 0;
+    const int ih = m.hok && more ? m.hz + ldz * gx : 0;
+    const bool inj = row && more && gx >= S.rx0 && gx < S.rx0 + S.nrx;
+    const float* r = S.rec + (long long)(L - 1) * S.nrx + (inj ? gx - S.rx0 : 0);
+    const long long rs = (long long)S.nt * S.nrx;
+#pragma unroll
+    for (int f = 0; f < 5; ++f) {
+      c.b[f] = gld(x2 + f * fs, ic);
+      if (xin)
+        c.add[f] = gld(out + f * fs, ic);
+      else
+        c.add[f] = row ? gld(r + f * rs, 0) : 0.f;
+    }
+    raw(ih, c.h);
+    if (imaging) {
+      sv.load(0, ih, m.hz, gx, m.hok && more, c.hs[0]);
+      sv.load(1, ih, m.hz, gx, m.hok && more, c.hs[1]);
+      if (next) {  // arrays 2, 3 exist only below the last level
+        sv.load(2, ic, gz, gx, m.zok && more, c.nt2[0]);
+        sv.load(3, ic, gz, gx, m.zok && more, c.nt2[1]);
+      } else {
+        c.nt2[0].in = c.nt2[1].in = false;
+      }
+#pragma unroll
+      for (int a = 0; a < 3; ++a) {
+        c.im[a] = gld(S.img + a * n2, ic);
+        c.imc[a] = comp ? gld(S.imgc + a * n2, ic) : 0.f;
+      }
+    }
+  };
+  Col cur;
+  load_col(xb, true, cur);
+  bool cur_inj = row && xb >= S.rx0 && xb < S.rx0 + S.nrx;
+  bool cur_h = m.hok;
+  const float kz = ((flags & A_K) && m.zok) ? gld(S.kz, gz) : 1.f;
+  const bool zin4 = gz >= 4 && gz <= nz - 6, zin5 = gz >= 5 && gz <= nz - 6;
+  const float w = S.img_w, inv_dx = S.inv_dx, inv_dt = S.inv_dt;
+
+  for (int x = xb; x < xe; ++x) {
+    const bool more = x + 1 < xe;
+    const bool qok = m.zok && more && x + 6 < nx;
+    const int iq = qok ? gz + ldz * (x + 6) : 0;
+    Raw nr;
+    raw(iq, nr);
+    typename SrcView<SNAP>::Raw ns0, ns1;
+    if (imaging) {
+      sv.load(0, iq, gz, x + 6, qok, ns0);
+      sv.load(1, iq, gz, x + 6, qok, ns1);
+    }
+    Col nxt;
+    load_col(x + 1, more, nxt);
+
+    const int b = (x - xb) & 1;
+    col[b][0][m.lane + 5] = d1[0];
+    col[b][1][m.lane + 5] = q2[5];
+    col[b][2][m.lane + 5] = q3[5];
+    col[b][3][m.lane + 5] = q4[5];
+    if (imaging) {
+      col[b][4][m.lane + 5] = s0q[5];
+      col[b][5][m.lane + 5] = s1q[5];
+    }
+    if (m.lane < 10) {
+      const Raw& h = cur.h;
+      col[b][0][m.hs] = cur_h ? fmaf(h.m2, h.xx, h.m2m3 * h.zz) : 0.f;
+      col[b][1][m.hs] = cur_h ? h.m3 * h.xz : 0.f;
+      col[b][2][m.hs] = cur_h ? h.m1x * h.vx : 0.f;
+      col[b][3][m.hs] = cur_h ? h.m1z * h.vz : 0.f;
+      if (imaging) {
+        col[b][4][m.hs] = sv.value(cur.hs[0]);
+        col[b][5][m.hs] = sv.value(cur.hs[1]);
+      }
+    }
+    wave_lds_sync();
+
+    if (m.zok) {
+      const int p = m.lane + 5;
+      const int gx = x;
+      const int gi = gz + ldz * gx;
+      float bb[5];
+#pragma unroll
+      for (int f = 0; f < 5; ++f) {
+        bb[f] = cur.b[f];
+        if (xin || cur_inj) bb[f] = cur.add[f] + bb[f];
+      }
+      float v0 = bb[0] - smq(q0);
+      v0 -= sp<1>(col[b][1], p);
+      float v1 = bb[1] - sm<1>(col[b][0], p);
+      v1 -= spq(q2);
+      const float v2 = bb[2] - spq(q3);
+      const float v3 = bb[3] - sp<1>(col[b][3], p);
+      float v4 = bb[4] - sm<1>(col[b][2], p);
+      v4 -= smq(q4);
+      float v[5] = {v0, v1, v2, v3, v4};
+      if (flags & A_K) {
+        const float kx = gld(S.kx, gx);
+#pragma unroll
+        for (int f = 0; f < 5; ++f) v[f] = (v[f] * kz) * kx;
+      }
+      if (flags & A_NTS) {
+#pragma unroll
+        for (int f = 0; f < 5; ++f) gst_nt(out + f * fs, gi, v[f]);
+      } else {
+#pragma unroll
+        for (int f = 0; f < 5; ++f) gst(out + f * fs, gi, v[f]);
+      }
+      if (imaging) {
+        // the imaging arithmetic is fused only where written (fmaf), so
+        // every kernel variant rounds it the same way
+#pragma clang fp contract(off)
+        float s1 = 0.f, s2 = 0.f;
+        if (next) {
+          if (SNAP == SNAP_HIST) {  // arrays 2, 3 are Vx, Vz of level L+1
+            s1 = inv_dt * (sv.value(cur.nt2[0]) - s0q[5]);
+            s2 = inv_dt * (sv.value(cur.nt2[1]) - s1q[5]);
+          } else {
+            s1 = inv_dt * sv.value(cur.nt2[0]);
+            s2 = inv_dt * sv.value(cur.nt2[1]);
+          }
+        }
+        float s4 = 0.f, s5 = 0.f, s6 = 0.f;
+        if (zin4 && gx >= 4 && gx <= nx - 6) {
+          s4 = spq(s0q) * inv_dx;
+          s5 = sp<1>(col[b][5], p) * inv_dx;
+        }
+        if (zin5 && gx >= 5 && gx <= nx - 6) {
+          s6 = sm<1>(col[b][4], p) * inv_dx;
+          s6 = fmaf(smq(s1q), inv_dx, s6);
+        }
+        const float s3 = s4 + s5;
+        const float t[3] = {w * fmaf(v[0], s1, v[1] * s2), -w * fmaf(v[2], s3, v[3] * s3),
+                            -w * fmaf(v[4], s6, fmaf(v[3] * 2.f, s5, v[2] * 2.f * s4))};
+        if (comp) {
+#pragma unroll
+          for (int a = 0; a < 3; ++a) {
+            float sa = cur.im[a], ca = cur.imc[a];
+            neumaier(sa, ca, t[a]);
+            gst(S.img + a * n2, gi, sa);
+            gst(S.imgc + a * n2, gi, ca);
+          }
+        } else {
+#pragma unroll
+          for (int a = 0; a < 3; ++a) gst(S.img + a * n2, gi, cur.im[a] + t[a]);
+        }
+      }
+    }
+    push(q0, qok ? fmaf(nr.m2m3, nr.xx, nr.m2 * nr.zz) : 0.f);
+    push(q2, qok ? nr.m3 * nr.xz : 0.f);
+    push(q3, qok ? nr.m1x * nr.vx : 0.f);
+    push(q4, qok ? nr.m1z * nr.vz : 0.f);
+#pragma unroll
+    for (int k = 0; k < 5; ++k) d1[k] = d1[k + 1];
+    d1[5] = qok ? fmaf(nr.m2, nr.xx, nr.m2m3 * nr.zz) : 0.f;
+    if (imaging) {
+      push(s0q, sv.value(ns0));
+      push(s1q, sv.value(ns1));
+    }
+    cur = nxt;
+    cur_h = m.hok && more;
+    cur_inj = row && more && x + 1 >= S.rx0 && x + 1 < S.rx0 + S.nrx;
   }
 }
 
@@ -1484,10 +2026,55 @@ int launch_snap_last(const ShotDev* shots, const TileRef* tiles, int ntiles, int
   return OK;
 }
 
-int launch_born_sources(const ShotDev* shots, const TileRef* tiles, int ntiles, int L,
+int launch_born_sources(const ShotDev* shots, const TileRef* tiles, int ntiles, int L, int txr,
                         hipStream_t st) {
   if (ntiles <= 0) return OK;
-  hipLaunchKernelGGL(k_born_sources, dim3(ntiles), dim3(TILE_Z, TILE_WY), 0, st, shots, tiles, L);
+  const dim3 g(ntiles), b(TILE_Z, TILE_WY);
+  switch (txr) {
+    case 1: hipLaunchKernelGGL(k_born_sources<1>, g, b, 0, st, shots, tiles, L); break;
+    case 2: hipLaunchKernelGGL(k_born_sources<2>, g, b, 0, st, shots, tiles, L); break;
+    case 4: hipLaunchKernelGGL(k_born_sources<4>, g, b, 0, st, shots, tiles, L); break;
+    case 8: hipLaunchKernelGGL(k_born_sources<8>, g, b, 0, st, shots, tiles, L); break;
+    default: set_error("Born sources: unsupported tile width %d", txr); return E_ARG;
+  }
+  HIPCHK(hipGetLastError());
+  return OK;
+}
+
+int launch_fwdm(const ShotDev* shots, const TileRef* strips, int nstrips, int L, int flags, int xw, int snap,
+                int xcd, hipStream_t st) {
+  if (nstrips <= 0) return OK;
+  if (xw < 2 || (xw & 1)) { set_error("marching strip width %d is not a positive even number", xw); return E_ARG; }
+  if ((flags & F_BORN) && snap != SNAP_HIST) { set_error("Born sources need the fp32 history format"); return E_ARG; }
+  const dim3 g(nstrips), b(TILE_Z);
+  switch (snap) {
+    case SNAP_HIST:
+      if (flags & F_BORN)
+        hipLaunchKernelGGL((k_fwdm<SNAP_HIST, true>), g, b, 0, st, shots, strips, nstrips, L, flags, xw, xcd);
+      else
+        hipLaunchKernelGGL((k_fwdm<SNAP_HIST, false>), g, b, 0, st, shots, strips, nstrips, L, flags, xw, xcd);
+      break;
+    case SNAP_F32: hipLaunchKernelGGL((k_fwdm<SNAP_F32, false>), g, b, 0, st, shots, strips, nstrips, L, flags, xw, xcd); break;
+    case SNAP_BF16: hipLaunchKernelGGL((k_fwdm<SNAP_BF16, false>), g, b, 0, st, shots, strips, nstrips, L, flags, xw, xcd); break;
+    case SNAP_F16: hipLaunchKernelGGL((k_fwdm<SNAP_F16, false>), g, b, 0, st, shots, strips, nstrips, L, flags, xw, xcd); break;
+    default: set_error("unknown snapshot format %d", snap); return E_ARG;
+  }
+  HIPCHK(hipGetLastError());
+  return OK;
+}
+
+int launch_adjm(const ShotDev* shots, const TileRef* strips, int nstrips, int L, int flags, int xw, int snap,
+                int xcd, hipStream_t st) {
+  if (nstrips <= 0) return OK;
+  if (xw < 2 || (xw & 1)) { set_error("marching strip width %d is not a positive even number", xw); return E_ARG; }
+  const dim3 g(nstrips), b(TILE_Z);
+  switch (snap) {
+    case SNAP_HIST: hipLaunchKernelGGL(k_adjm<SNAP_HIST>, g, b, 0, st, shots, strips, nstrips, L, flags, xw, xcd); break;
+    case SNAP_F32: hipLaunchKernelGGL(k_adjm<SNAP_F32>, g, b, 0, st, shots, strips, nstrips, L, flags, xw, xcd); break;
+    case SNAP_BF16: hipLaunchKernelGGL(k_adjm<SNAP_BF16>, g, b, 0, st, shots, strips, nstrips, L, flags, xw, xcd); break;
+    case SNAP_F16: hipLaunchKernelGGL(k_adjm<SNAP_F16>, g, b, 0, st, shots, strips, nstrips, L, flags, xw, xcd); break;
+    default: set_error("unknown snapshot format %d", snap); return E_ARG;
+  }
   HIPCHK(hipGetLastError());
   return OK;
 }

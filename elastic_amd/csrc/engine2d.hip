@@ -31,9 +31,11 @@ int default_txr() { return env_int("RTM_TXR", 2); }
 int default_xcd() { return env_int("RTM_XCD_REMAP", 1); }
 // Non-temporal stores of the new level (RTM_NT=1).
 static int nt_stores() { return env_int("RTM_NT", 0); }
-// Two levels per launch (temporal blocking) when RTM_TBLOCK=1; the batch
-// autotuner chooses it per workload.
+// Kernel form (RTM_TBLOCK): 0 one level per launch on tiles, 1 two levels
+// per launch, 2 one level per launch marching along x in strips of
+// RTM_MARCH_W columns; the batch autotuner chooses the form per workload.
 int default_tblock() { return env_int("RTM_TBLOCK", 0); }
+int default_march_w() { return env_int("RTM_MARCH_W", 16); }
 
 // Sponge taper, W/abc-mod.f90:35-50, and per-row/column factors of K
 // (:59-78): rows first then columns, so corners are damped twice.
@@ -118,6 +120,13 @@ int Engine::setup(const EngineCfg& cfg, const std::vector<ShotSpec>& shots,
   }
   if (cfg_.txr != 1 && cfg_.txr != 2 && cfg_.txr != 4 && cfg_.txr != 8) {
     set_error("tile width %d not in {1,2,4,8}", cfg_.txr);
+    return E_ARG;
+  }
+  if (cfg_.tblock < 0 || cfg_.tblock > 2 ||
+      (cfg_.tblock == 2 && (cfg_.txr != 2 || cfg_.march_w < 2 || (cfg_.march_w & 1)))) {
+    set_error("kernel form %d with tile width %d / strip width %d: two-level steps need width <= 4, marching "
+              "steps width 2 and an even strip width",
+              cfg_.tblock, cfg_.txr, cfg_.march_w);
     return E_ARG;
   }
   if (cfg_.movie) {
@@ -222,14 +231,16 @@ int Engine::alloc_working() {
 int Engine::build_groups() {
   groups_.clear();
   const int ns = (int)specs_.size();
-  std::vector<TileRef> tiles;
+  std::vector<TileRef> tiles, strips;
   const int tx = tile_x(cfg_.txr);
+  const bool march = cfg_.tblock == 2;
   int s = 0;
   G_ = 0;
   while (s < ns) {
     Group g;
     g.desc0 = s;
     g.tile0 = (int)tiles.size();
+    g.strip0 = (int)strips.size();
     size_t used = 0;
     while (s < ns && used + res_[s].work_bytes <= arena_bytes_ &&
            (cfg_.max_group <= 0 || (int)g.shots.size() < cfg_.max_group)) {
@@ -240,10 +251,16 @@ int Engine::build_groups() {
       const int nbz = (p.nz + TILE_Z - 1) / TILE_Z, nbx = (p.nx + tx - 1) / tx;
       for (int bx = 0; bx < nbx; ++bx)
         for (int bz = 0; bz < nbz; ++bz) tiles.push_back({slot, bz * TILE_Z, bx * tx, bz + nbz * bx});
+      if (march) {
+        const int nsx = (p.nx + cfg_.march_w - 1) / cfg_.march_w;
+        for (int bx = 0; bx < nsx; ++bx)
+          for (int bz = 0; bz < nbz; ++bz) strips.push_back({slot, bz * TILE_Z, bx * cfg_.march_w, bz + nbz * bx});
+      }
       g.shots.push_back(s);
       ++s;
     }
     g.ntiles = (int)tiles.size() - g.tile0;
+    g.nstrips = (int)strips.size() - g.strip0;
     G_ = std::max(G_, (int)g.shots.size());
     groups_.push_back(g);
   }
@@ -253,6 +270,10 @@ int Engine::build_groups() {
   STCHK(d_desc_.alloc(h_desc_.size() * sizeof(ShotDev), "shot descriptors"));
   STCHK(d_tiles_.alloc(tiles.size() * sizeof(TileRef), "tile table"));
   HIPCHK(hipMemcpy(d_tiles_.p, tiles.data(), tiles.size() * sizeof(TileRef), hipMemcpyHostToDevice));
+  if (march) {
+    STCHK(d_strips_.alloc(strips.size() * sizeof(TileRef), "strip table"));
+    HIPCHK(hipMemcpy(d_strips_.p, strips.data(), strips.size() * sizeof(TileRef), hipMemcpyHostToDevice));
+  }
   for (int g = 0; g < (int)groups_.size(); ++g) STCHK(fill_desc(g, false, false));
   return OK;
 }
@@ -313,6 +334,8 @@ int Engine::enqueue_forward(int g, int mode, hipStream_t st) {
   const int variant = (mode == 1) ? 1 : (mode >= 4 ? 2 : 0);
   const ShotDev* desc = d_desc_.as<ShotDev>() + (size_t)variant * specs_.size() + gr.desc0;
   const TileRef* tiles = d_tiles_.as<TileRef>() + gr.tile0;
+  const bool march = cfg_.tblock == 2;
+  const TileRef* strips = march ? d_strips_.as<TileRef>() + gr.strip0 : nullptr;
   const int snap = (mode == 0) ? cfg_.snap_fmt : SNAP_HIST;
   for (int s : gr.shots) {
     STCHK(launch_fill(ring_of(s), (size_t)5 * kRing * res_[s].n2, 0.f, st));
@@ -323,7 +346,7 @@ int Engine::enqueue_forward(int g, int mode, hipStream_t st) {
   const int first = born ? 1 : 2;
   // levels L >= 3 go two per launch (temporal blocking) where the kernel
   // supports the tile width; Born scattering sources are per level
-  const bool pairs = cfg_.tblock && !born && cfg_.txr <= 4;
+  const bool pairs = cfg_.tblock == 1 && !born && cfg_.txr <= 4;
   for (int L = first; L <= cfg_.nt;) {
     const bool two = pairs && L >= 3 && L + 1 <= cfg_.nt;
     int fl = 0;
@@ -331,7 +354,7 @@ int Engine::enqueue_forward(int g, int mode, hipStream_t st) {
     if (!born) fl |= F_SRC;
     if (born) {
       fl |= F_BORN | F_REC;
-      STCHK(launch_born_sources(desc, tiles, gr.ntiles, L, st));
+      STCHK(launch_born_sources(desc, tiles, gr.ntiles, L, cfg_.txr, st));
     }
     if (mode == 1 || mode == 3) fl |= F_REC;
     if (mode == 2) fl |= F_RECSUB;
@@ -347,6 +370,8 @@ int Engine::enqueue_forward(int g, int mode, hipStream_t st) {
     }
     if (two)
       STCHK(launch_fwd2(desc, tiles, gr.ntiles, L, fl, cfg_.txr, snap, cfg_.xcd, st));
+    else if (march)
+      STCHK(launch_fwdm(desc, strips, gr.nstrips, L, fl, cfg_.march_w, snap, cfg_.xcd, st));
     else
       STCHK(launch_fwd(desc, tiles, gr.ntiles, L, fl, cfg_.txr, snap, cfg_.xcd, st));
     for (int Lm = L; Lm <= L + (two ? 1 : 0); ++Lm) {
@@ -374,13 +399,15 @@ int Engine::enqueue_adjoint(int g, hipStream_t st) {
   const Group& gr = groups_[g];
   const ShotDev* desc = d_desc_.as<ShotDev>() + gr.desc0;
   const TileRef* tiles = d_tiles_.as<TileRef>() + gr.tile0;
+  const bool march = cfg_.tblock == 2;
+  const TileRef* strips = march ? d_strips_.as<TileRef>() + gr.strip0 : nullptr;
   for (size_t k = 0; k < gr.shots.size(); ++k) {
     Res& r = res_[gr.shots[k]];
     STCHK(launch_fill(ring_of(gr.shots[k]), (size_t)5 * kRing * r.n2, 0.f, st));
     STCHK(launch_fill(r.img.as<float>(), 3 * r.n2, 0.f, st));
     if (cfg_.comp) STCHK(launch_fill(r.imgc.as<float>(), 3 * r.n2, 0.f, st));
   }
-  const bool pairs = cfg_.tblock && cfg_.txr <= 4;
+  const bool pairs = cfg_.tblock == 1 && cfg_.txr <= 4;
   for (int L = cfg_.nt; L >= 1;) {
     const bool two = pairs && L - 1 >= 1;
     int fl = A_INJ;
@@ -392,6 +419,8 @@ int Engine::enqueue_adjoint(int g, hipStream_t st) {
     if (nt_stores()) fl |= A_NTS;
     if (two)
       STCHK(launch_adj2(desc, tiles, gr.ntiles, L, fl, cfg_.txr, cfg_.snap_fmt, cfg_.xcd, st));
+    else if (march)
+      STCHK(launch_adjm(desc, strips, gr.nstrips, L, fl, cfg_.march_w, cfg_.snap_fmt, cfg_.xcd, st));
     else
       STCHK(launch_adj(desc, tiles, gr.ntiles, L, fl, cfg_.txr, cfg_.snap_fmt, cfg_.xcd, st));
     L -= two ? 2 : 1;
@@ -408,13 +437,13 @@ int Engine::enqueue_adjoint(int g, hipStream_t st) {
 // loops of enqueue_forward / enqueue_adjoint).
 long long Engine::fwd_launch_count(int mode) const {
   const bool born = mode == 5;
-  const bool pairs = cfg_.tblock && !born && cfg_.txr <= 4;
+  const bool pairs = cfg_.tblock == 1 && !born && cfg_.txr <= 4;
   long long n = 0;
   for (int L = born ? 1 : 2; L <= cfg_.nt; ++n) L += (pairs && L >= 3 && L + 1 <= cfg_.nt) ? 2 : 1;
   return n;
 }
 long long Engine::adj_launch_count() const {
-  const bool pairs = cfg_.tblock && cfg_.txr <= 4;
+  const bool pairs = cfg_.tblock == 1 && cfg_.txr <= 4;
   long long n = 0;
   for (int L = cfg_.nt; L >= 1; ++n) L -= (pairs && L - 1 >= 1) ? 2 : 1;
   return n;

@@ -31,7 +31,10 @@ struct EngineCfg {
   int txr = 4;         // tile width (x cells per thread)
   int xcd = 1;         // XCD-aware tile numbering
   int use_graph = 0;   // capture the time loops in HIP graphs
-  int tblock = 0;      // two levels per launch (temporal blocking)
+  int tblock = 0;      // kernel form: 0 one level per launch on tiles, 1 two
+                       // levels per launch (temporal blocking), 2 one level
+                       // per launch marching along x (tile width 2 only)
+  int march_w = 16;    // strip width (columns, even) of the marching form
   int max_group = 0;   // cap on shots per group (0 = memory bound only)
   int stack_nz = 0, stack_nx = 0;  // stacked image (0 = none)
   double mem_fraction = 0.85;      // share of free memory the working set may use
@@ -93,6 +96,7 @@ class Engine {
   struct Group {
     std::vector<int> shots;
     int tile0 = 0, ntiles = 0;  // slice of the tile table
+    int strip0 = 0, nstrips = 0;  // slice of the strip table (marching form)
     int desc0 = 0;              // first descriptor
     hipGraphExec_t gfwd = nullptr, gadj = nullptr, gsyn = nullptr;
   };
@@ -113,7 +117,7 @@ class Engine {
   int G_ = 0;
   long long n2max_ = 0;
   size_t arena_bytes_ = 0;
-  DevBuf src_amp_, work_, d_desc_, d_tiles_, stack_, born_buf_;
+  DevBuf src_amp_, work_, d_desc_, d_tiles_, d_strips_, stack_, born_buf_;
   std::vector<ShotDev> h_desc_;
   hipStream_t st_ = nullptr;
   hipEvent_t ev_[4] = {};
@@ -127,6 +131,7 @@ int ensure_device();
 int default_txr();
 int default_xcd();
 int default_tblock();
+int default_march_w();
 // Tile autotuner (autotune.hip): times the engine on a short copy of the
 // workload for TXR 1, 2, 4, 8 with one level per launch and TXR 1, 2, 4 with
 // two, and returns the fastest in *best / *best_tblock (ms_out[tblock*4 +

@@ -315,7 +315,8 @@ class Batch:
 
     @property
     def tblock(self):
-        """1 when the batch runs two levels per launch (temporal blocking)."""
+        """Kernel form: 0 one level per launch on tiles, 1 two levels per
+        launch (temporal blocking), 2 one level per launch marching along x."""
         return int(load().rtm_batch_tblock(self._h))
 
     def info(self):
@@ -339,7 +340,8 @@ class Batch:
 def autotune(nt, dx, dt, shots, coefs, wavelet, abc=0, adj_abc=0, snap_every=1, snap_prec="fp32", comp_image=0,
              use_graph=0):
     """Run the tile autotuner on a workload (rtm_batch_autotune): returns
-    ((txr, levels per launch - 1), {(txr, tblock): ms})."""
+    ((txr, form), {(txr, form): ms}); form 0 one level per launch on tiles,
+    1 two levels per launch, 2 marching along x."""
     lib = load()
     n = len(shots)
     arr = (RtmBatchShot * n)()
@@ -354,7 +356,8 @@ def autotune(nt, dx, dt, shots, coefs, wavelet, abc=0, adj_abc=0, snap_every=1, 
     best, tb = C.c_int(), C.c_int()
     ms = np.zeros(8, np.float64)
     _chk(lib.rtm_batch_autotune(C.byref(cfg), C.c_int(n), arr, ptrs, _p(w), C.byref(best), C.byref(tb), _p(ms)))
-    times = {(t, k): ms[4 * k + i] for k in (0, 1) for i, t in enumerate((1, 2, 4, 8)) if ms[4 * k + i] >= 0}
+    cand = [(1, 0), (2, 0), (4, 0), (8, 0), (1, 1), (2, 1), (4, 1), (2, 2)]
+    times = {c: ms[i] for i, c in enumerate(cand) if ms[i] >= 0}
     return (best.value, tb.value), times
 
 

@@ -139,11 +139,44 @@ def test_tile_widths_agree_and_autotuner(hip, j):
             for x, y in zip(a, c):
                 assert np.array_equal(x, y), txr
     best, ms = hip.autotune(NT, DX, DT, specs, cmig, wav, abc=ABC, adj_abc=1)
-    assert best[0] in (1, 2, 4, 8) and best[1] in (0, 1) and len(ms) == 7
+    assert best[0] in (1, 2, 4, 8) and best[1] in (0, 1, 2) and len(ms) == 8
     assert all(v > 0 for v in ms.values()) and ms[best] == min(ms.values())
     b = _batch(hip, specs, ctrue, cmig, wav, txr=0)
-    assert b.txr in (1, 2, 4, 8) and b.tblock in (0, 1)
+    assert b.txr in (1, 2, 4, 8) and b.tblock in (0, 1, 2)
     b.close()
+
+
+@pytest.mark.parametrize("prec,j,mw", [("fp32", 1, 32), ("fp32", 3, 6), ("bf16", 4, 2), ("fp16", 2, 10)])
+def test_marching_form_equals_tiles(hip, monkeypatch, prec, j, mw):
+    """The marching step kernels (one wave per strip of mw columns walking
+    along x, register queues for the x-differences) give bitwise the same
+    observed data and images as the tile kernels of width 2; 16-bit snapshots
+    use the same 64 x 2 scale strips."""
+    specs, ctrue, cmig, wav = _survey(hip)
+    out = {}
+    monkeypatch.setenv("RTM_MARCH_W", str(mw))
+    for tb in ("0", "2"):
+        monkeypatch.setenv("RTM_TBLOCK", tb)
+        b = _batch(hip, specs, ctrue, cmig, wav, txr=2, snap_every=j, snap_prec=prec, comp_image=int(prec != "fp32"))
+        assert b.tblock == int(tb)
+        b.migrate()
+        out[tb] = ([b.get_data(i) for i in range(len(specs))], [b.image(i) for i in range(len(specs))])
+        b.close()
+    for a, c in zip(out["0"][0], out["2"][0]):
+        assert np.array_equal(a, c)
+    for a, c in zip(out["0"][1], out["2"][1]):
+        for x, y in zip(a, c):
+            assert np.array_equal(x, y)
+
+
+def test_marching_form_needs_width_two(hip, monkeypatch):
+    specs, ctrue, cmig, wav = _survey(hip, nshots=1)
+    monkeypatch.setenv("RTM_TBLOCK", "2")
+    with pytest.raises(hip.RtmError):
+        hip.Batch(NT, DX, DT, specs, cmig, wav, abc=ABC, txr=4)
+    monkeypatch.setenv("RTM_MARCH_W", "7")
+    with pytest.raises(hip.RtmError):
+        hip.Batch(NT, DX, DT, specs, cmig, wav, abc=ABC, txr=2)
 
 
 @pytest.mark.parametrize("prec,j", [("fp32", 1), ("bf16", 4), ("fp32", 3)])

@@ -269,11 +269,11 @@ def test_movie_frames_are_forward_levels(hip, tmp_path):
     assert sep.axes(h)[2][2] == pytest.approx(1e-3 * dsnap)
 
 
-@pytest.mark.parametrize("tblock", ["0", "1"])
+@pytest.mark.parametrize("tblock", ["0", "1", "2"])
 def test_sponge_free_forward_and_migrate(hip, monkeypatch, tblock):
     """abc = 0 (no sponge anywhere; the kernels get no sponge factors):
     forward levels and images against the FP64 oracle, one and two levels per
-    launch."""
+    launch and the marching form."""
     monkeypatch.setenv("RTM_TBLOCK", tblock)
     p, w, coefs, m, geo = _shot(nt=180)
     p.abc = 0
@@ -287,3 +287,27 @@ def test_sponge_free_forward_and_migrate(hip, monkeypatch, tblock):
     refi = O.rtm(p, w, cs, d, np.float64)
     for a, b in zip(img, refi):
         assert O.rel_l2(a, b) <= IMG_TOL
+
+
+@pytest.mark.parametrize("mw", ["2", "6", "32"])
+def test_marching_form_single_shot_paths_bitwise(hip, monkeypatch, mw):
+    """The marching step kernels (RTM_TBLOCK=2, strips of mw columns) give
+    bitwise the tile kernels' results on the drop-in adjoint (ragged grid),
+    the forward with receivers, Born modelling and migration with the exact
+    sponge adjoint and compensated images."""
+    hist, coefs5 = _adjoint_inputs(7, 70, 130, 25)
+    p, w, coefs, m, geo = _shot(nt=160)
+    rng = np.random.default_rng(2)
+    dm = [np.asfortranarray(rng.uniform(-1, 1, (p.nz, p.nx)).astype(np.float32) * 1e-3) for _ in range(3)]
+    d, cs = _observed(p, w, coefs, m, geo, 1e-3)
+    out = {}
+    monkeypatch.setenv("RTM_MARCH_W", mw)
+    for tb in ("0", "2"):
+        monkeypatch.setenv("RTM_TBLOCK", tb)
+        op3 = hip.rtm_op3(hist, coefs5)
+        rec, last = hip.forward(_hip_shot(hip, p), coefs, w, want_last=True)
+        born = hip.born(_hip_shot(hip, p), coefs, w, dm)
+        img = hip.migrate(_hip_shot(hip, p, adj_abc=1, comp_image=1), cs, w, d)
+        out[tb] = [*op3, rec, *last, born, *img]
+    for k, (a, b) in enumerate(zip(out["0"], out["2"])):
+        assert np.array_equal(a, b), k
