@@ -276,12 +276,10 @@ def main():
     # kernel form the batch runs (autotuned unless --txr / RTM_TXR / RTM_TBLOCK fix it)
     form, txr = b.tblock, b.txr
     kname = ("k_" + dom) + {0: "", 1: "2", 2: "m"}[form]
+    cells_per_launch = (info["adj_cells"] / info["adj_launches"]) if dom == "adj" else \
+        (info["fwd_cells"] / info["fwd_launches"])
     tr = load_traffic(args.variant, kname)
-    traffic = None
-    if tr:
-        cells_per_launch = (info["adj_cells"] / info["adj_launches"]) if dom == "adj" else \
-            (info["fwd_cells"] / info["fwd_launches"])
-        traffic = round(tr["bytes_per_cell_update"] * cells_per_launch)
+    traffic = round(tr["bytes_per_cell_update"] * cells_per_launch) if tr else None
     b.close()
 
     img_errs = None
@@ -330,6 +328,7 @@ def main():
                      "traffic": traffic,
                      "traffic_per_cell_update": tr["bytes_per_cell_update"] if traffic else None,
                      "ms_per_launch": round(ms_launch, 5),
+                     "cells_per_launch": round(cells_per_launch),
                      "algo_bytes_per_cell": fb if dom == "fwd" else abytes,
                      "other": {"k_fwd_GBs": round(fwd_gbs, 1), "k_adj_GBs": round(adj_gbs, 1),
                                "fwd_ms": round(info["fwd_ms"], 2), "adj_ms": round(info["adj_ms"], 2)}},

@@ -123,6 +123,17 @@ int autotune_txr(const EngineCfg& cfg, const std::vector<ShotSpec>& shots, const
     t[i] = e.timing().total_ms;
     if (arg < 0 || t[i] < t[arg]) arg = i;
   }
+  // The marching form reads 1.3x (forward) to 1.8x (adjoint) the HBM bytes
+  // of the tile form (its priming columns and halo rows are re-read from
+  // HBM by neighbouring strips); it is taken only when clearly faster on
+  // the short run, not on a near-tie that a longer, bandwidth-bound run
+  // would lose.
+  if (arg == 7) {
+    int alt = -1;
+    for (int i = 0; i < 7; ++i)
+      if (t[i] > 0 && (alt < 0 || t[i] < t[alt])) alt = i;
+    if (alt >= 0 && t[7] > 0.97 * t[alt]) arg = alt;
+  }
   *best = cand[arg][0];
   *best_tblock = cand[arg][1];
   if (ms_out)
