@@ -1392,22 +1392,31 @@ __global__ __launch_bounds__(256) void k_born_sources(const ShotDev* __restrict_
 
 constexpr int MZ = TILE_Z + 10;  // LDS column: rows z0-5 .. z0+68
 
-// x-differences of a queue q[k] = column x-5+k (same operations and order
-// as sp / sm)
+// x-differences of a queue. The queue holds columns in the order they were
+// loaded, oldest first: q[k] is column x-5+k on a march towards +x (DIR 1)
+// and column x+5-k on a march towards -x (DIR -1). qv<DIR>(q, j) is column
+// x-5+j either way, so both directions perform the same operations in the
+// same order as sp / sm.
+template <int DIR>
+__device__ __forceinline__ float qv(const float (&q)[11], int j) {
+  return q[DIR > 0 ? j : 10 - j];
+}
+template <int DIR>
 __device__ __forceinline__ float spq(const float (&q)[11]) {
-  float r = C1 * (q[10] - q[1]);
-  r = fmaf(C2, q[9] - q[2], r);
-  r = fmaf(C3, q[8] - q[3], r);
-  r = fmaf(C4, q[7] - q[4], r);
-  r = fmaf(C5, q[6] - q[5], r);
+  float r = C1 * (qv<DIR>(q, 10) - qv<DIR>(q, 1));
+  r = fmaf(C2, qv<DIR>(q, 9) - qv<DIR>(q, 2), r);
+  r = fmaf(C3, qv<DIR>(q, 8) - qv<DIR>(q, 3), r);
+  r = fmaf(C4, qv<DIR>(q, 7) - qv<DIR>(q, 4), r);
+  r = fmaf(C5, qv<DIR>(q, 6) - qv<DIR>(q, 5), r);
   return r;
 }
+template <int DIR>
 __device__ __forceinline__ float smq(const float (&q)[11]) {
-  float r = C1 * (q[9] - q[0]);
-  r = fmaf(C2, q[8] - q[1], r);
-  r = fmaf(C3, q[7] - q[2], r);
-  r = fmaf(C4, q[6] - q[3], r);
-  r = fmaf(C5, q[5] - q[4], r);
+  float r = C1 * (qv<DIR>(q, 9) - qv<DIR>(q, 0));
+  r = fmaf(C2, qv<DIR>(q, 8) - qv<DIR>(q, 1), r);
+  r = fmaf(C3, qv<DIR>(q, 7) - qv<DIR>(q, 2), r);
+  r = fmaf(C4, qv<DIR>(q, 6) - qv<DIR>(q, 3), r);
+  r = fmaf(C5, qv<DIR>(q, 5) - qv<DIR>(q, 4), r);
   return r;
 }
 __device__ __forceinline__ void push(float (&q)[11], float v) {
@@ -1443,12 +1452,10 @@ struct MarchLane {
   }
 };
 
-template <int SNAP, bool BORN>
-__global__ __launch_bounds__(64) void k_fwdm(const ShotDev* __restrict__ shots, const TileRef* __restrict__ strips,
-                                            int nstrips, int L, int flags, int xw, int xcd) {
-  __shared__ float col[2][4][MZ];  // sigma_zz, sigma_xz, Vx, Vz of the centre column
-  const TileRef tr = strips[xcd ? xcd_tile(blockIdx.x, nstrips) : (int)blockIdx.x];
-  const ShotDev& S = shots[tr.shot];
+// One strip of the forward step, marching towards +x (DIR 1) or -x (DIR -1).
+template <int SNAP, bool BORN, int DIR>
+__device__ __forceinline__ void fwdm_strip(const ShotDev& S, const TileRef& tr, int L, int flags, int xw,
+                                           float (*col)[4][MZ]) {
   const int nz = S.nz, nx = S.nx, ldz = S.ldz;
   const int fs = (int)S.fstride, n2 = (int)S.n2;
   float* const out = lev_ptr(S.lev, S.lev_stride, S.lev_mod, S.lev_off, L);
@@ -1457,6 +1464,7 @@ __global__ __launch_bounds__(64) void k_fwdm(const ShotDev* __restrict__ shots, 
   const float* const coef = S.coef;
   const float* const bsrc = S.bsrc;
   const int z0 = tr.z0, xb = tr.x0, xe = min(tr.x0 + xw, nx);
+  const int ncol = xe - xb, xs = DIR > 0 ? xb : xe - 1;  // columns, first column of the march
   const MarchLane m(z0, nz);
   const int gz = m.gz;
   const bool born = BORN;  // Born sources (F_BORN) in their own instance: 10 fewer live registers
@@ -1465,7 +1473,7 @@ __global__ __launch_bounds__(64) void k_fwdm(const ShotDev* __restrict__ shots, 
   float qa[11], qb[11], qc[11], qd[11];
 #pragma unroll
   for (int k = 0; k < 11; ++k) {
-    const int gx = xb - 5 + k;
+    const int gx = xs + DIR * (k - 5);
     const bool in = m.zok && gx >= 0 && gx < nx;
     qa[k] = gldm(y1 + 2 * fs, gz + ldz * gx, in);
     qb[k] = gldm(y1 + 4 * fs, gz + ldz * gx, in);
@@ -1495,20 +1503,22 @@ __global__ __launch_bounds__(64) void k_fwdm(const ShotDev* __restrict__ shots, 
     }
   };
   Col cur;
-  load_col(xb, true, cur);
+  load_col(xs, true, cur);
   bool cur_h = m.hok;  // halo values of the current column are valid
   const float kz = ((flags & F_K) && m.zok) ? gld(S.kz, gz) : 1.f;
   float svp[4] = {0.f, 0.f, 0.f, 0.f};  // previous column's snapshot values (16-bit formats)
 
-  for (int x = xb; x < xe; ++x) {
-    const bool more = x + 1 < xe;
-    const bool qok = m.zok && more && x + 6 < nx;
-    const int iq = qok ? gz + ldz * (x + 6) : 0;
+  for (int st = 0; st < ncol; ++st) {
+    const int x = xs + DIR * st;
+    const bool more = st + 1 < ncol;
+    const int xq = x + 6 * DIR;  // column entering the queues
+    const bool qok = m.zok && more && xq >= 0 && xq < nx;
+    const int iq = qok ? gz + ldz * xq : 0;
     const float na = gld(y1 + 2 * fs, iq), nb = gld(y1 + 4 * fs, iq), nc = gld(y1, iq), nd = gld(y1 + fs, iq);
     Col nxt;
-    load_col(x + 1, more, nxt);
+    load_col(x + DIR, more, nxt);
 
-    const int b = (x - xb) & 1;
+    const int b = st & 1;
     col[b][0][m.lane + 5] = m.zok ? cur.zz : 0.f;
     col[b][1][m.lane + 5] = qb[5];
     col[b][2][m.lane + 5] = qc[5];
@@ -1528,14 +1538,14 @@ __global__ __launch_bounds__(64) void k_fwdm(const ShotDev* __restrict__ shots, 
       float v[5];
 #pragma unroll
       for (int f = 0; f < 5; ++f) v[f] = born ? cur.bs[f] + cur.o[f] : cur.o[f];
-      const float dxx_xm = smq(qa);             // Ax
+      const float dxx_xm = smq<DIR>(qa);        // Ax
       const float dxz_zp = sp<1>(col[b][1], p);  // Bx
-      const float dxz_xp = spq(qb);             // Az
+      const float dxz_xp = spq<DIR>(qb);        // Az
       const float dzz_zm = sm<1>(col[b][0], p);  // Bz
-      const float dvx_xp = spq(qc);             // D, F
+      const float dvx_xp = spq<DIR>(qc);        // D, F
       const float dvz_zp = sp<1>(col[b][3], p);  // E, G
       const float dvx_zm = sm<1>(col[b][2], p);  // H
-      const float dvz_xm = smq(qd);             // J
+      const float dvz_xm = smq<DIR>(qd);        // J
       v[0] = fmaf(m1x, dxx_xm, v[0]);
       v[0] = fmaf(m1x, dxz_zp, v[0]);
       v[1] = fmaf(m1z, dxz_xp, v[1]);
@@ -1594,12 +1604,14 @@ __global__ __launch_bounds__(64) void k_fwdm(const ShotDev* __restrict__ shots, 
 #pragma unroll
           for (int a = 0; a < 4; ++a) gst(sp4 + a * n2, gz + ldz * x, sv[a]);
         }
-      } else if ((x & 1) || x + 1 == nx) {
+      } else if (DIR > 0 ? ((x & 1) || x + 1 == nx) : !(x & 1)) {
         // 16-bit formats: one max-abs scale per 64 x 2 strip (columns 2i,
-        // 2i+1; strips start on even columns, and so do strips of a march)
+        // 2i+1; strips start on even columns, and so do strips of a march).
+        // A pair is stored at its second column in march order; its first
+        // (column x - DIR) is held in svp.
         unsigned short* q4 = static_cast<unsigned short*>(S.snap) + k0 * 4 * n2;
         const int strip = (z0 >> 6) + S.snap_nbz * (x >> 1);
-        const bool pair = x & 1;
+        const bool pair = DIR > 0 ? (x & 1) : x + 1 < xe;
 #pragma unroll
         for (int a = 0; a < 4; ++a) {
           float mx = 0.f;
@@ -1609,7 +1621,7 @@ __global__ __launch_bounds__(64) void k_fwdm(const ShotDev* __restrict__ shots, 
           const float inv = mx > kSnapFloor ? SnapQ<SNAP>::Q / mx : 0.f;
           if (m.lane == 0) gst(S.snap_scale + (k0 * 4 + a) * S.snap_nstrip, strip, mx / SnapQ<SNAP>::Q);
           if (m.zok) {
-            if (pair) gst(q4 + a * n2, gz + ldz * (x - 1), SnapQ<SNAP>::enc(svp[a] * inv));
+            if (pair) gst(q4 + a * n2, gz + ldz * (x - DIR), SnapQ<SNAP>::enc(svp[a] * inv));
             gst(q4 + a * n2, gz + ldz * x, SnapQ<SNAP>::enc(sv[a] * inv));
           }
         }
@@ -1626,16 +1638,27 @@ __global__ __launch_bounds__(64) void k_fwdm(const ShotDev* __restrict__ shots, 
   }
 }
 
+template <int SNAP, bool BORN>
+__global__ __launch_bounds__(64) void k_fwdm(const ShotDev* __restrict__ shots, const TileRef* __restrict__ strips,
+                                            int nstrips, int L, int flags, int xw, int xcd) {
+  __shared__ float col[2][4][MZ];  // sigma_zz, sigma_xz, Vx, Vz of the centre column
+  const TileRef tr = strips[xcd ? xcd_tile(blockIdx.x, nstrips) : (int)blockIdx.x];
+  // neighbouring strips march in opposite directions, so the columns they
+  // share (a strip's priming columns) are read by both at the same end of
+  // their marches, close in time, and come from L2 for the second
+  if (((tr.x0 / xw) & 1) == 0)
+    fwdm_strip<SNAP, BORN, 1>(shots[tr.shot], tr, L, flags, xw, col);
+  else
+    fwdm_strip<SNAP, BORN, -1>(shots[tr.shot], tr, L, flags, xw, col);
+}
+
 // Adjoint step (gather form, as k_adj) marching along x: the products
 // P0 = m2m3 sxx + m2 szz, P1 = m2 sxx + m2m3 szz, P2 = m3 sxz, P3 = m1x Vx,
 // P4 = m1z Vz of level L+1 are formed once per column as it enters the
 // queues; imaging reads the source Vx, Vz the same way.
-template <int SNAP>
-__global__ __launch_bounds__(64) void k_adjm(const ShotDev* __restrict__ shots, const TileRef* __restrict__ strips,
-                                            int nstrips, int L, int flags, int xw, int xcd) {
-  __shared__ float col[2][6][MZ];  // P1, P2, P3, P4, source Vx, source Vz of the centre column
-  const TileRef tr = strips[xcd ? xcd_tile(blockIdx.x, nstrips) : (int)blockIdx.x];
-  const ShotDev& S = shots[tr.shot];
+template <int SNAP, int DIR>
+__device__ __forceinline__ void adjm_strip(const ShotDev& S, const TileRef& tr, int L, int flags, int xw,
+                                           float (*col)[6][MZ]) {
   const int nz = S.nz, nx = S.nx, ldz = S.ldz;
   const int fs = (int)S.fstride, n2 = (int)S.n2;
   float* const out = lev_ptr(S.lev, S.lev_stride, S.lev_mod, S.lev_off, L);
@@ -1643,6 +1666,7 @@ __global__ __launch_bounds__(64) void k_adjm(const ShotDev* __restrict__ shots, 
   const float* const y1 = lev_ptr(S.lev, S.lev_stride, S.lev_mod, S.lev_off, L + 1);
   const float* const coef = S.coef;
   const int z0 = tr.z0, xb = tr.x0, xe = min(tr.x0 + xw, nx);
+  const int ncol = xe - xb, xs = DIR > 0 ? xb : xe - 1;  // columns, first column of the march
   const MarchLane m(z0, nz);
   const int gz = m.gz;
   const bool imaging = flags & A_IMG;
@@ -1673,7 +1697,7 @@ __global__ __launch_bounds__(64) void k_adjm(const ShotDev* __restrict__ shots, 
   float s0q[11], s1q[11];  // source Vx (S+x), Vz (S-x)
 #pragma unroll
   for (int k = 0; k < 11; ++k) {
-    const int gx = xb - 5 + k;
+    const int gx = xs + DIR * (k - 5);
     const bool in = m.zok && gx >= 0 && gx < nx;
     Raw w;
     raw(in ? gz + ldz * gx : 0, w);
@@ -1731,28 +1755,30 @@ __global__ __launch_bounds__(64) void k_adjm(const ShotDev* __restrict__ shots, 
     }
   };
   Col cur;
-  load_col(xb, true, cur);
-  bool cur_inj = row && xb >= S.rx0 && xb < S.rx0 + S.nrx;
+  load_col(xs, true, cur);
+  bool cur_inj = row && xs >= S.rx0 && xs < S.rx0 + S.nrx;
   bool cur_h = m.hok;
   const float kz = ((flags & A_K) && m.zok) ? gld(S.kz, gz) : 1.f;
   const bool zin4 = gz >= 4 && gz <= nz - 6, zin5 = gz >= 5 && gz <= nz - 6;
   const float w = S.img_w, inv_dx = S.inv_dx, inv_dt = S.inv_dt;
 
-  for (int x = xb; x < xe; ++x) {
-    const bool more = x + 1 < xe;
-    const bool qok = m.zok && more && x + 6 < nx;
-    const int iq = qok ? gz + ldz * (x + 6) : 0;
+  for (int st = 0; st < ncol; ++st) {
+    const int x = xs + DIR * st;
+    const bool more = st + 1 < ncol;
+    const int xq = x + 6 * DIR;  // column entering the queues
+    const bool qok = m.zok && more && xq >= 0 && xq < nx;
+    const int iq = qok ? gz + ldz * xq : 0;
     Raw nr;
     raw(iq, nr);
     typename SrcView<SNAP>::Raw ns0, ns1;
     if (imaging) {
-      sv.load(0, iq, gz, x + 6, qok, ns0);
-      sv.load(1, iq, gz, x + 6, qok, ns1);
+      sv.load(0, iq, gz, xq, qok, ns0);
+      sv.load(1, iq, gz, xq, qok, ns1);
     }
     Col nxt;
-    load_col(x + 1, more, nxt);
+    load_col(x + DIR, more, nxt);
 
-    const int b = (x - xb) & 1;
+    const int b = st & 1;
     col[b][0][m.lane + 5] = d1[0];
     col[b][1][m.lane + 5] = q2[5];
     col[b][2][m.lane + 5] = q3[5];
@@ -1784,14 +1810,14 @@ __global__ __launch_bounds__(64) void k_adjm(const ShotDev* __restrict__ shots, 
         bb[f] = cur.b[f];
         if (xin || cur_inj) bb[f] = cur.add[f] + bb[f];
       }
-      float v0 = bb[0] - smq(q0);
+      float v0 = bb[0] - smq<DIR>(q0);
       v0 -= sp<1>(col[b][1], p);
       float v1 = bb[1] - sm<1>(col[b][0], p);
-      v1 -= spq(q2);
-      const float v2 = bb[2] - spq(q3);
+      v1 -= spq<DIR>(q2);
+      const float v2 = bb[2] - spq<DIR>(q3);
       const float v3 = bb[3] - sp<1>(col[b][3], p);
       float v4 = bb[4] - sm<1>(col[b][2], p);
-      v4 -= smq(q4);
+      v4 -= smq<DIR>(q4);
       float v[5] = {v0, v1, v2, v3, v4};
       if (flags & A_K) {
         const float kx = gld(S.kx, gx);
@@ -1821,12 +1847,12 @@ __global__ __launch_bounds__(64) void k_adjm(const ShotDev* __restrict__ shots, 
         }
         float s4 = 0.f, s5 = 0.f, s6 = 0.f;
         if (zin4 && gx >= 4 && gx <= nx - 6) {
-          s4 = spq(s0q) * inv_dx;
+          s4 = spq<DIR>(s0q) * inv_dx;
           s5 = sp<1>(col[b][5], p) * inv_dx;
         }
         if (zin5 && gx >= 5 && gx <= nx - 6) {
           s6 = sm<1>(col[b][4], p) * inv_dx;
-          s6 = fmaf(smq(s1q), inv_dx, s6);
+          s6 = fmaf(smq<DIR>(s1q), inv_dx, s6);
         }
         const float s3 = s4 + s5;
         const float t[3] = {w * fmaf(v[0], s1, v[1] * s2), -w * fmaf(v[2], s3, v[3] * s3),
@@ -1858,8 +1884,35 @@ __global__ __launch_bounds__(64) void k_adjm(const ShotDev* __restrict__ shots, 
     }
     cur = nxt;
     cur_h = m.hok && more;
-    cur_inj = row && more && x + 1 >= S.rx0 && x + 1 < S.rx0 + S.nrx;
+    cur_inj = row && more && x + DIR >= S.rx0 && x + DIR < S.rx0 + S.nrx;
   }
+}
+
+template <int SNAP>
+__device__ __forceinline__ void adjm_kernel(const ShotDev* __restrict__ shots, const TileRef* __restrict__ strips,
+                                            int nstrips, int L, int flags, int xw, int xcd) {
+  __shared__ float col[2][6][MZ];  // P1, P2, P3, P4, source Vx, source Vz of the centre column
+  const TileRef tr = strips[xcd ? xcd_tile(blockIdx.x, nstrips) : (int)blockIdx.x];
+  // neighbouring strips march in opposite directions (see k_fwdm)
+  if (((tr.x0 / xw) & 1) == 0)
+    adjm_strip<SNAP, 1>(shots[tr.shot], tr, L, flags, xw, col);
+  else
+    adjm_strip<SNAP, -1>(shots[tr.shot], tr, L, flags, xw, col);
+}
+// fp32 snapshot formats: held to 168 VGPRs (3 waves per SIMD; a few spills,
+// measured 9% faster than 2 waves without them)
+template <int SNAP>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void k_adjm(
+    const ShotDev* __restrict__ shots, const TileRef* __restrict__ strips, int nstrips, int L, int flags, int xw,
+    int xcd) {
+  adjm_kernel<SNAP>(shots, strips, nstrips, L, flags, xw, xcd);
+}
+// 16-bit snapshot formats: their decode needs more registers; limiting them
+// to 168 spills 30 and more, measured slower than 2 waves per SIMD
+template <int SNAP>
+__global__ __launch_bounds__(64) void k_adjm16(const ShotDev* __restrict__ shots, const TileRef* __restrict__ strips,
+                                              int nstrips, int L, int flags, int xw, int xcd) {
+  adjm_kernel<SNAP>(shots, strips, nstrips, L, flags, xw, xcd);
 }
 
 __global__ void k_fill(float* out, size_t n, float v) {
@@ -2071,8 +2124,8 @@ int launch_adjm(const ShotDev* shots, const TileRef* strips, int nstrips, int L,
   switch (snap) {
     case SNAP_HIST: hipLaunchKernelGGL(k_adjm<SNAP_HIST>, g, b, 0, st, shots, strips, nstrips, L, flags, xw, xcd); break;
     case SNAP_F32: hipLaunchKernelGGL(k_adjm<SNAP_F32>, g, b, 0, st, shots, strips, nstrips, L, flags, xw, xcd); break;
-    case SNAP_BF16: hipLaunchKernelGGL(k_adjm<SNAP_BF16>, g, b, 0, st, shots, strips, nstrips, L, flags, xw, xcd); break;
-    case SNAP_F16: hipLaunchKernelGGL(k_adjm<SNAP_F16>, g, b, 0, st, shots, strips, nstrips, L, flags, xw, xcd); break;
+    case SNAP_BF16: hipLaunchKernelGGL(k_adjm16<SNAP_BF16>, g, b, 0, st, shots, strips, nstrips, L, flags, xw, xcd); break;
+    case SNAP_F16: hipLaunchKernelGGL(k_adjm16<SNAP_F16>, g, b, 0, st, shots, strips, nstrips, L, flags, xw, xcd); break;
     default: set_error("unknown snapshot format %d", snap); return E_ARG;
   }
   HIPCHK(hipGetLastError());
