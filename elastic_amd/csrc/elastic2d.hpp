@@ -100,14 +100,15 @@ int launch_fwd2(const ShotDev* shots, const TileRef* tiles, int ntiles, int L, i
 // the drop-in history form).
 int launch_adj2(const ShotDev* shots, const TileRef* tiles, int ntiles, int L, int flags,
                 int txr, int snap_fmt, int xcd, hipStream_t st);
-// Marching forms: one 64-thread wave per strip of 64 rows x xw columns
-// (xw even; strip tables start strips on even columns). Bitwise the same
-// results as launch_fwd / launch_adj; the 16-bit snapshot strips are 64 x 2
-// (the layout of tile width 2).
-int launch_fwdm(const ShotDev* shots, const TileRef* strips, int nstrips, int L, int flags, int xw, int snap_fmt,
-                int xcd, hipStream_t st);
-int launch_adjm(const ShotDev* shots, const TileRef* strips, int nstrips, int L, int flags, int xw, int snap_fmt,
-                int xcd, hipStream_t st);
+// Marching forms: one workgroup of nw waves (nw in {1, 2, 4}) per strip of
+// 64*nw rows x xw columns (xw even; strip tables start strips on even
+// columns and rows on multiples of 64*nw). Bitwise the same results as
+// launch_fwd / launch_adj; the 16-bit snapshot strips are 64 x 2 (the layout
+// of tile width 2).
+int launch_fwdm(const ShotDev* shots, const TileRef* strips, int nstrips, int L, int flags, int xw, int nw,
+                int snap_fmt, int xcd, hipStream_t st);
+int launch_adjm(const ShotDev* shots, const TileRef* strips, int nstrips, int L, int flags, int xw, int nw,
+                int snap_fmt, int xcd, hipStream_t st);
 // Snapshot of the last level (no later step writes it): dV = 0.
 int launch_snap_last(const ShotDev* shots, const TileRef* tiles, int ntiles, int L, int txr,
                      int snap_fmt, hipStream_t st);

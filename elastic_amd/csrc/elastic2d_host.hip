@@ -93,13 +93,13 @@ static int op3(int nt, int nz, int nx, float* const* hist, const float* const* c
   const int txr = default_txr();
   // kernel form 2 (RTM_TBLOCK=2): one wave per strip of RTM_MARCH_W columns
   const bool march = default_tblock() == 2;
-  const int mw = default_march_w();
+  const int mw = default_march_w(), mnw = default_march_nw();
   if (march && (mw < 2 || (mw & 1))) { set_error("RTM_MARCH_W=%d is not a positive even number", mw); return E_ARG; }
-  const int tx = march ? mw : tile_x(txr);
-  const int nbz = (nz + TILE_Z - 1) / TILE_Z, nbx = (nx + tx - 1) / tx;
+  const int tx = march ? mw : tile_x(txr), tz = march ? TILE_Z * mnw : TILE_Z;
+  const int nbz = (nz + tz - 1) / tz, nbx = (nx + tx - 1) / tx;
   std::vector<TileRef> tiles;
   for (int bx = 0; bx < nbx; ++bx)
-    for (int bz = 0; bz < nbz; ++bz) tiles.push_back({0, bz * TILE_Z, bx * tx, bz + nbz * bx});
+    for (int bz = 0; bz < nbz; ++bz) tiles.push_back({0, bz * tz, bx * tx, bz + nbz * bx});
   STCHK(dd.alloc(sizeof(ShotDev), "descriptor"));
   STCHK(dt.alloc(tiles.size() * sizeof(TileRef), "tiles"));
   HIPCHK(hipMemcpyAsync(dd.p, &d, sizeof(ShotDev), hipMemcpyHostToDevice, st));
@@ -107,7 +107,7 @@ static int op3(int nt, int nz, int nx, float* const* hist, const float* const* c
   HIPCHK(hipEventRecord(e0, st));
   for (int it = nt - 3; it >= 0; --it) {
     if (march)
-      STCHK(launch_adjm(dd.as<ShotDev>(), dt.as<TileRef>(), (int)tiles.size(), it, A_XIN, mw, SNAP_HIST,
+      STCHK(launch_adjm(dd.as<ShotDev>(), dt.as<TileRef>(), (int)tiles.size(), it, A_XIN, mw, mnw, SNAP_HIST,
                         default_xcd(), st));
     else
       STCHK(launch_adj(dd.as<ShotDev>(), dt.as<TileRef>(), (int)tiles.size(), it, A_XIN, txr, SNAP_HIST,
@@ -143,6 +143,7 @@ static int single_engine(Engine& e, const rtm_shot_t* s, const float* coef5, con
   cfg.xcd = default_xcd();
   cfg.tblock = default_tblock();
   cfg.march_w = default_march_w();
+  cfg.march_nw = default_march_nw();
   cfg.use_graph = getenv("RTM_GRAPH") ? atoi(getenv("RTM_GRAPH")) : 0;
   ShotSpec p{};
   p.nz = s->nz; p.nx = s->nx; p.xs = s->xs; p.zs = s->zs; p.zr = s->zr; p.rx0 = s->rx0; p.nrx = s->nrx;
@@ -374,6 +375,7 @@ RTM_EXPORT int rtm_forward_movie_hip(const rtm_shot_t* s, const float* coef5, co
   cfg.xcd = default_xcd();
   cfg.tblock = default_tblock();
   cfg.march_w = default_march_w();
+  cfg.march_nw = default_march_nw();
   cfg.movie_every = every;
   cfg.movie_n = nsnap;
   cfg.movie_z0 = z0; cfg.movie_x0 = x0; cfg.movie_nz = wnz; cfg.movie_nx = wnx;
@@ -593,6 +595,7 @@ RTM_EXPORT int rtm_batch_create(rtm_batch** out, const rtm_batch_cfg_t* cfg, int
   ec.xcd = default_xcd();
   ec.tblock = default_tblock();
   ec.march_w = default_march_w();
+  ec.march_nw = default_march_nw();
   ec.use_graph = cfg->use_graph;
   ec.max_group = cfg->max_group;
   ec.stack_nz = cfg->stack_nz;
@@ -631,6 +634,7 @@ RTM_EXPORT int rtm_batch_autotune(const rtm_batch_cfg_t* cfg, int nshots, const 
   ec.xcd = default_xcd();
   ec.tblock = default_tblock();
   ec.march_w = default_march_w();
+  ec.march_nw = default_march_nw();
   ec.use_graph = cfg->use_graph;
   std::vector<ShotSpec> sp(nshots);
   std::vector<const float*> cf(coef5, coef5 + nshots);

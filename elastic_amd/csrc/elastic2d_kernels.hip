@@ -1390,7 +1390,13 @@ __global__ __launch_bounds__(256) void k_born_sources(const ShotDev* __restrict_
 // computed. Same arithmetic, in the same order, as k_fwd / k_adj, so the
 // results are bitwise the same.
 
-constexpr int MZ = TILE_Z + 10;  // LDS column: rows z0-5 .. z0+68
+// A strip is NW waves stacked along z (64*NW rows); its LDS column holds
+// rows z0-5 .. z0+64*NW+4, so the rows between its waves come from LDS and
+// only the 5 rows above and below the strip are loaded as halo.
+template <int NW>
+struct MarchCol {
+  static constexpr int MZ = TILE_Z * NW + 10;
+};
 
 // x-differences of a queue. The queue holds columns in the order they were
 // loaded, oldest first: q[k] is column x-5+k on a march towards +x (DIR 1)
@@ -1436,9 +1442,24 @@ __device__ __forceinline__ void wave_lds_sync() {
   __builtin_amdgcn_wave_barrier();
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
+// The same for the waves of a multi-wave strip: a workgroup barrier whose
+// fences cover LDS only, so the global loads of the next column stay in
+// flight across it.
+template <int NW>
+__device__ __forceinline__ void strip_lds_sync() {
+  if (NW == 1) {
+    wave_lds_sync();
+  } else {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+  }
+}
 
-// Lane geometry of a marching wave: own row gz = z0 + lane; lanes 0..9 also
-// carry one halo row hz (z0-5 .. z0-1, z0+64 .. z0+68) at LDS slot hs.
+// Lane geometry of a marching strip: own row gz = z0 + lane; lanes 0..9 of
+// its first wave also carry one halo row hz (z0-5 .. z0-1, z0+64*NW ..
+// z0+64*NW+4) at LDS slot hs.
+template <int NW>
 struct MarchLane {
   int lane, gz, hz, hs;
   bool zok, hok;
@@ -1446,16 +1467,16 @@ struct MarchLane {
     lane = threadIdx.x;
     gz = z0 + lane;
     zok = gz < nz;
-    hz = lane < 5 ? z0 - 5 + lane : z0 + 59 + lane;
-    hs = lane < 5 ? lane : TILE_Z + lane;
+    hz = lane < 5 ? z0 - 5 + lane : z0 + TILE_Z * NW - 5 + lane;
+    hs = lane < 5 ? lane : TILE_Z * NW + lane;
     hok = lane < 10 && hz >= 0 && hz < nz;
   }
 };
 
 // One strip of the forward step, marching towards +x (DIR 1) or -x (DIR -1).
-template <int SNAP, bool BORN, int DIR>
+template <int SNAP, bool BORN, int DIR, int NW>
 __device__ __forceinline__ void fwdm_strip(const ShotDev& S, const TileRef& tr, int L, int flags, int xw,
-                                           float (*col)[4][MZ]) {
+                                           float (*col)[4][MarchCol<NW>::MZ]) {
   const int nz = S.nz, nx = S.nx, ldz = S.ldz;
   const int fs = (int)S.fstride, n2 = (int)S.n2;
   float* const out = lev_ptr(S.lev, S.lev_stride, S.lev_mod, S.lev_off, L);
@@ -1465,7 +1486,7 @@ __device__ __forceinline__ void fwdm_strip(const ShotDev& S, const TileRef& tr, 
   const float* const bsrc = S.bsrc;
   const int z0 = tr.z0, xb = tr.x0, xe = min(tr.x0 + xw, nx);
   const int ncol = xe - xb, xs = DIR > 0 ? xb : xe - 1;  // columns, first column of the march
-  const MarchLane m(z0, nz);
+  const MarchLane<NW> m(z0, nz);
   const int gz = m.gz;
   const bool born = BORN;  // Born sources (F_BORN) in their own instance: 10 fewer live registers
 
@@ -1527,7 +1548,7 @@ __device__ __forceinline__ void fwdm_strip(const ShotDev& S, const TileRef& tr, 
 #pragma unroll
       for (int f = 0; f < 4; ++f) col[b][f][m.hs] = cur_h ? cur.h[f] : 0.f;
     }
-    wave_lds_sync();
+    strip_lds_sync<NW>();
 
     float sv[4] = {0.f, 0.f, 0.f, 0.f};
     if (m.zok) {
@@ -1610,7 +1631,7 @@ __device__ __forceinline__ void fwdm_strip(const ShotDev& S, const TileRef& tr, 
         // A pair is stored at its second column in march order; its first
         // (column x - DIR) is held in svp.
         unsigned short* q4 = static_cast<unsigned short*>(S.snap) + k0 * 4 * n2;
-        const int strip = (z0 >> 6) + S.snap_nbz * (x >> 1);
+        const int strip = (gz >> 6) + S.snap_nbz * (x >> 1);  // the wave's 64 rows
         const bool pair = DIR > 0 ? (x & 1) : x + 1 < xe;
 #pragma unroll
         for (int a = 0; a < 4; ++a) {
@@ -1619,7 +1640,9 @@ __device__ __forceinline__ void fwdm_strip(const ShotDev& S, const TileRef& tr, 
           mx = fmaxf(mx, fabsf(sv[a]));
           mx = wave_max(mx);
           const float inv = mx > kSnapFloor ? SnapQ<SNAP>::Q / mx : 0.f;
-          if (m.lane == 0) gst(S.snap_scale + (k0 * 4 + a) * S.snap_nstrip, strip, mx / SnapQ<SNAP>::Q);
+          // a wave wholly below the grid (last strip of a short grid) has no strip
+          if ((m.lane & 63) == 0 && m.zok)
+            gst(S.snap_scale + (k0 * 4 + a) * S.snap_nstrip, strip, mx / SnapQ<SNAP>::Q);
           if (m.zok) {
             if (pair) gst(q4 + a * n2, gz + ldz * (x - DIR), SnapQ<SNAP>::enc(svp[a] * inv));
             gst(q4 + a * n2, gz + ldz * x, SnapQ<SNAP>::enc(sv[a] * inv));
@@ -1638,27 +1661,27 @@ __device__ __forceinline__ void fwdm_strip(const ShotDev& S, const TileRef& tr, 
   }
 }
 
-template <int SNAP, bool BORN>
-__global__ __launch_bounds__(64) void k_fwdm(const ShotDev* __restrict__ shots, const TileRef* __restrict__ strips,
-                                            int nstrips, int L, int flags, int xw, int xcd) {
-  __shared__ float col[2][4][MZ];  // sigma_zz, sigma_xz, Vx, Vz of the centre column
+template <int SNAP, bool BORN, int NW>
+__global__ __launch_bounds__(64 * NW) void k_fwdm(const ShotDev* __restrict__ shots, const TileRef* __restrict__ strips,
+                                                 int nstrips, int L, int flags, int xw, int xcd) {
+  __shared__ float col[2][4][MarchCol<NW>::MZ];  // sigma_zz, sigma_xz, Vx, Vz of the centre column
   const TileRef tr = strips[xcd ? xcd_tile(blockIdx.x, nstrips) : (int)blockIdx.x];
   // neighbouring strips march in opposite directions, so the columns they
   // share (a strip's priming columns) are read by both at the same end of
   // their marches, close in time, and come from L2 for the second
   if (((tr.x0 / xw) & 1) == 0)
-    fwdm_strip<SNAP, BORN, 1>(shots[tr.shot], tr, L, flags, xw, col);
+    fwdm_strip<SNAP, BORN, 1, NW>(shots[tr.shot], tr, L, flags, xw, col);
   else
-    fwdm_strip<SNAP, BORN, -1>(shots[tr.shot], tr, L, flags, xw, col);
+    fwdm_strip<SNAP, BORN, -1, NW>(shots[tr.shot], tr, L, flags, xw, col);
 }
 
 // Adjoint step (gather form, as k_adj) marching along x: the products
 // P0 = m2m3 sxx + m2 szz, P1 = m2 sxx + m2m3 szz, P2 = m3 sxz, P3 = m1x Vx,
 // P4 = m1z Vz of level L+1 are formed once per column as it enters the
 // queues; imaging reads the source Vx, Vz the same way.
-template <int SNAP, int DIR>
+template <int SNAP, int DIR, int NW>
 __device__ __forceinline__ void adjm_strip(const ShotDev& S, const TileRef& tr, int L, int flags, int xw,
-                                           float (*col)[6][MZ]) {
+                                           float (*col)[6][MarchCol<NW>::MZ]) {
   const int nz = S.nz, nx = S.nx, ldz = S.ldz;
   const int fs = (int)S.fstride, n2 = (int)S.n2;
   float* const out = lev_ptr(S.lev, S.lev_stride, S.lev_mod, S.lev_off, L);
@@ -1667,7 +1690,7 @@ __device__ __forceinline__ void adjm_strip(const ShotDev& S, const TileRef& tr, 
   const float* const coef = S.coef;
   const int z0 = tr.z0, xb = tr.x0, xe = min(tr.x0 + xw, nx);
   const int ncol = xe - xb, xs = DIR > 0 ? xb : xe - 1;  // columns, first column of the march
-  const MarchLane m(z0, nz);
+  const MarchLane<NW> m(z0, nz);
   const int gz = m.gz;
   const bool imaging = flags & A_IMG;
   const bool next = flags & A_NEXT;
@@ -1798,7 +1821,7 @@ __device__ __forceinline__ void adjm_strip(const ShotDev& S, const TileRef& tr, 
         col[b][5][m.hs] = sv.value(cur.hs[1]);
       }
     }
-    wave_lds_sync();
+    strip_lds_sync<NW>();
 
     if (m.zok) {
       const int p = m.lane + 5;
@@ -1888,31 +1911,32 @@ __device__ __forceinline__ void adjm_strip(const ShotDev& S, const TileRef& tr, 
   }
 }
 
-template <int SNAP>
+template <int SNAP, int NW>
 __device__ __forceinline__ void adjm_kernel(const ShotDev* __restrict__ shots, const TileRef* __restrict__ strips,
                                             int nstrips, int L, int flags, int xw, int xcd) {
-  __shared__ float col[2][6][MZ];  // P1, P2, P3, P4, source Vx, source Vz of the centre column
+  __shared__ float col[2][6][MarchCol<NW>::MZ];  // P1, P2, P3, P4, source Vx, source Vz of the centre column
   const TileRef tr = strips[xcd ? xcd_tile(blockIdx.x, nstrips) : (int)blockIdx.x];
   // neighbouring strips march in opposite directions (see k_fwdm)
   if (((tr.x0 / xw) & 1) == 0)
-    adjm_strip<SNAP, 1>(shots[tr.shot], tr, L, flags, xw, col);
+    adjm_strip<SNAP, 1, NW>(shots[tr.shot], tr, L, flags, xw, col);
   else
-    adjm_strip<SNAP, -1>(shots[tr.shot], tr, L, flags, xw, col);
+    adjm_strip<SNAP, -1, NW>(shots[tr.shot], tr, L, flags, xw, col);
 }
 // fp32 snapshot formats: held to 168 VGPRs (3 waves per SIMD; a few spills,
 // measured 9% faster than 2 waves without them)
-template <int SNAP>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void k_adjm(
+template <int SNAP, int NW>
+__global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(3))) void k_adjm(
     const ShotDev* __restrict__ shots, const TileRef* __restrict__ strips, int nstrips, int L, int flags, int xw,
     int xcd) {
-  adjm_kernel<SNAP>(shots, strips, nstrips, L, flags, xw, xcd);
+  adjm_kernel<SNAP, NW>(shots, strips, nstrips, L, flags, xw, xcd);
 }
 // 16-bit snapshot formats: their decode needs more registers; limiting them
 // to 168 spills 30 and more, measured slower than 2 waves per SIMD
-template <int SNAP>
-__global__ __launch_bounds__(64) void k_adjm16(const ShotDev* __restrict__ shots, const TileRef* __restrict__ strips,
-                                              int nstrips, int L, int flags, int xw, int xcd) {
-  adjm_kernel<SNAP>(shots, strips, nstrips, L, flags, xw, xcd);
+template <int SNAP, int NW>
+__global__ __launch_bounds__(64 * NW) void k_adjm16(const ShotDev* __restrict__ shots,
+                                                   const TileRef* __restrict__ strips, int nstrips, int L, int flags,
+                                                   int xw, int xcd) {
+  adjm_kernel<SNAP, NW>(shots, strips, nstrips, L, flags, xw, xcd);
 }
 
 __global__ void k_fill(float* out, size_t n, float v) {
@@ -2094,39 +2118,68 @@ int launch_born_sources(const ShotDev* shots, const TileRef* tiles, int ntiles, 
   return OK;
 }
 
-int launch_fwdm(const ShotDev* shots, const TileRef* strips, int nstrips, int L, int flags, int xw, int snap,
-                int xcd, hipStream_t st) {
-  if (nstrips <= 0) return OK;
-  if (xw < 2 || (xw & 1)) { set_error("marching strip width %d is not a positive even number", xw); return E_ARG; }
-  if ((flags & F_BORN) && snap != SNAP_HIST) { set_error("Born sources need the fp32 history format"); return E_ARG; }
-  const dim3 g(nstrips), b(TILE_Z);
+template <int NW>
+static void fwdm_dispatch(int snap, int flags, dim3 g, hipStream_t st, const ShotDev* s, const TileRef* t, int n,
+                          int L, int xw, int xcd) {
+  const dim3 b(TILE_Z * NW);
   switch (snap) {
     case SNAP_HIST:
       if (flags & F_BORN)
-        hipLaunchKernelGGL((k_fwdm<SNAP_HIST, true>), g, b, 0, st, shots, strips, nstrips, L, flags, xw, xcd);
+        hipLaunchKernelGGL((k_fwdm<SNAP_HIST, true, NW>), g, b, 0, st, s, t, n, L, flags, xw, xcd);
       else
-        hipLaunchKernelGGL((k_fwdm<SNAP_HIST, false>), g, b, 0, st, shots, strips, nstrips, L, flags, xw, xcd);
+        hipLaunchKernelGGL((k_fwdm<SNAP_HIST, false, NW>), g, b, 0, st, s, t, n, L, flags, xw, xcd);
       break;
-    case SNAP_F32: hipLaunchKernelGGL((k_fwdm<SNAP_F32, false>), g, b, 0, st, shots, strips, nstrips, L, flags, xw, xcd); break;
-    case SNAP_BF16: hipLaunchKernelGGL((k_fwdm<SNAP_BF16, false>), g, b, 0, st, shots, strips, nstrips, L, flags, xw, xcd); break;
-    case SNAP_F16: hipLaunchKernelGGL((k_fwdm<SNAP_F16, false>), g, b, 0, st, shots, strips, nstrips, L, flags, xw, xcd); break;
-    default: set_error("unknown snapshot format %d", snap); return E_ARG;
+    case SNAP_F32: hipLaunchKernelGGL((k_fwdm<SNAP_F32, false, NW>), g, b, 0, st, s, t, n, L, flags, xw, xcd); break;
+    case SNAP_BF16: hipLaunchKernelGGL((k_fwdm<SNAP_BF16, false, NW>), g, b, 0, st, s, t, n, L, flags, xw, xcd); break;
+    case SNAP_F16: hipLaunchKernelGGL((k_fwdm<SNAP_F16, false, NW>), g, b, 0, st, s, t, n, L, flags, xw, xcd); break;
+    default: break;
+  }
+}
+template <int NW>
+static void adjm_dispatch(int snap, dim3 g, hipStream_t st, const ShotDev* s, const TileRef* t, int n, int L,
+                          int flags, int xw, int xcd) {
+  const dim3 b(TILE_Z * NW);
+  switch (snap) {
+    case SNAP_HIST: hipLaunchKernelGGL((k_adjm<SNAP_HIST, NW>), g, b, 0, st, s, t, n, L, flags, xw, xcd); break;
+    case SNAP_F32: hipLaunchKernelGGL((k_adjm<SNAP_F32, NW>), g, b, 0, st, s, t, n, L, flags, xw, xcd); break;
+    case SNAP_BF16: hipLaunchKernelGGL((k_adjm16<SNAP_BF16, NW>), g, b, 0, st, s, t, n, L, flags, xw, xcd); break;
+    case SNAP_F16: hipLaunchKernelGGL((k_adjm16<SNAP_F16, NW>), g, b, 0, st, s, t, n, L, flags, xw, xcd); break;
+    default: break;
+  }
+}
+
+static int march_args(int nstrips, int xw, int nw, int snap) {
+  if (xw < 2 || (xw & 1)) { set_error("marching strip width %d is not a positive even number", xw); return E_ARG; }
+  if (nw != 1 && nw != 2 && nw != 4) { set_error("marching strip height %d waves not in {1,2,4}", nw); return E_ARG; }
+  if (snap < SNAP_HIST || snap > SNAP_F16) { set_error("unknown snapshot format %d", snap); return E_ARG; }
+  (void)nstrips;
+  return OK;
+}
+
+int launch_fwdm(const ShotDev* shots, const TileRef* strips, int nstrips, int L, int flags, int xw, int nw, int snap,
+                int xcd, hipStream_t st) {
+  if (nstrips <= 0) return OK;
+  STCHK(march_args(nstrips, xw, nw, snap));
+  if ((flags & F_BORN) && snap != SNAP_HIST) { set_error("Born sources need the fp32 history format"); return E_ARG; }
+  const dim3 g(nstrips);
+  switch (nw) {
+    case 1: fwdm_dispatch<1>(snap, flags, g, st, shots, strips, nstrips, L, xw, xcd); break;
+    case 2: fwdm_dispatch<2>(snap, flags, g, st, shots, strips, nstrips, L, xw, xcd); break;
+    default: fwdm_dispatch<4>(snap, flags, g, st, shots, strips, nstrips, L, xw, xcd); break;
   }
   HIPCHK(hipGetLastError());
   return OK;
 }
 
-int launch_adjm(const ShotDev* shots, const TileRef* strips, int nstrips, int L, int flags, int xw, int snap,
+int launch_adjm(const ShotDev* shots, const TileRef* strips, int nstrips, int L, int flags, int xw, int nw, int snap,
                 int xcd, hipStream_t st) {
   if (nstrips <= 0) return OK;
-  if (xw < 2 || (xw & 1)) { set_error("marching strip width %d is not a positive even number", xw); return E_ARG; }
-  const dim3 g(nstrips), b(TILE_Z);
-  switch (snap) {
-    case SNAP_HIST: hipLaunchKernelGGL(k_adjm<SNAP_HIST>, g, b, 0, st, shots, strips, nstrips, L, flags, xw, xcd); break;
-    case SNAP_F32: hipLaunchKernelGGL(k_adjm<SNAP_F32>, g, b, 0, st, shots, strips, nstrips, L, flags, xw, xcd); break;
-    case SNAP_BF16: hipLaunchKernelGGL(k_adjm16<SNAP_BF16>, g, b, 0, st, shots, strips, nstrips, L, flags, xw, xcd); break;
-    case SNAP_F16: hipLaunchKernelGGL(k_adjm16<SNAP_F16>, g, b, 0, st, shots, strips, nstrips, L, flags, xw, xcd); break;
-    default: set_error("unknown snapshot format %d", snap); return E_ARG;
+  STCHK(march_args(nstrips, xw, nw, snap));
+  const dim3 g(nstrips);
+  switch (nw) {
+    case 1: adjm_dispatch<1>(snap, g, st, shots, strips, nstrips, L, flags, xw, xcd); break;
+    case 2: adjm_dispatch<2>(snap, g, st, shots, strips, nstrips, L, flags, xw, xcd); break;
+    default: adjm_dispatch<4>(snap, g, st, shots, strips, nstrips, L, flags, xw, xcd); break;
   }
   HIPCHK(hipGetLastError());
   return OK;

@@ -36,6 +36,7 @@ static int nt_stores() { return env_int("RTM_NT", 0); }
 // RTM_MARCH_W columns; the batch autotuner chooses the form per workload.
 int default_tblock() { return env_int("RTM_TBLOCK", 0); }
 int default_march_w() { return env_int("RTM_MARCH_W", 16); }
+int default_march_nw() { return env_int("RTM_MARCH_NW", 2); }
 
 // Sponge taper, W/abc-mod.f90:35-50, and per-row/column factors of K
 // (:59-78): rows first then columns, so corners are damped twice.
@@ -123,10 +124,11 @@ int Engine::setup(const EngineCfg& cfg, const std::vector<ShotSpec>& shots,
     return E_ARG;
   }
   if (cfg_.tblock < 0 || cfg_.tblock > 2 ||
-      (cfg_.tblock == 2 && (cfg_.txr != 2 || cfg_.march_w < 2 || (cfg_.march_w & 1)))) {
-    set_error("kernel form %d with tile width %d / strip width %d: two-level steps need width <= 4, marching "
-              "steps width 2 and an even strip width",
-              cfg_.tblock, cfg_.txr, cfg_.march_w);
+      (cfg_.tblock == 2 && (cfg_.txr != 2 || cfg_.march_w < 2 || (cfg_.march_w & 1) ||
+                            (cfg_.march_nw != 1 && cfg_.march_nw != 2 && cfg_.march_nw != 4)))) {
+    set_error("kernel form %d with tile width %d / strip %d columns x %d waves: marching steps need tile width 2, "
+              "an even strip width and 1, 2 or 4 waves",
+              cfg_.tblock, cfg_.txr, cfg_.march_w, cfg_.march_nw);
     return E_ARG;
   }
   if (cfg_.movie) {
@@ -252,9 +254,10 @@ int Engine::build_groups() {
       for (int bx = 0; bx < nbx; ++bx)
         for (int bz = 0; bz < nbz; ++bz) tiles.push_back({slot, bz * TILE_Z, bx * tx, bz + nbz * bx});
       if (march) {
-        const int nsx = (p.nx + cfg_.march_w - 1) / cfg_.march_w;
+        const int sz = TILE_Z * cfg_.march_nw;
+        const int nsx = (p.nx + cfg_.march_w - 1) / cfg_.march_w, nsz = (p.nz + sz - 1) / sz;
         for (int bx = 0; bx < nsx; ++bx)
-          for (int bz = 0; bz < nbz; ++bz) strips.push_back({slot, bz * TILE_Z, bx * cfg_.march_w, bz + nbz * bx});
+          for (int bz = 0; bz < nsz; ++bz) strips.push_back({slot, bz * sz, bx * cfg_.march_w, bz + nsz * bx});
       }
       g.shots.push_back(s);
       ++s;
@@ -371,7 +374,7 @@ int Engine::enqueue_forward(int g, int mode, hipStream_t st) {
     if (two)
       STCHK(launch_fwd2(desc, tiles, gr.ntiles, L, fl, cfg_.txr, snap, cfg_.xcd, st));
     else if (march)
-      STCHK(launch_fwdm(desc, strips, gr.nstrips, L, fl, cfg_.march_w, snap, cfg_.xcd, st));
+      STCHK(launch_fwdm(desc, strips, gr.nstrips, L, fl, cfg_.march_w, cfg_.march_nw, snap, cfg_.xcd, st));
     else
       STCHK(launch_fwd(desc, tiles, gr.ntiles, L, fl, cfg_.txr, snap, cfg_.xcd, st));
     for (int Lm = L; Lm <= L + (two ? 1 : 0); ++Lm) {
@@ -420,7 +423,7 @@ int Engine::enqueue_adjoint(int g, hipStream_t st) {
     if (two)
       STCHK(launch_adj2(desc, tiles, gr.ntiles, L, fl, cfg_.txr, cfg_.snap_fmt, cfg_.xcd, st));
     else if (march)
-      STCHK(launch_adjm(desc, strips, gr.nstrips, L, fl, cfg_.march_w, cfg_.snap_fmt, cfg_.xcd, st));
+      STCHK(launch_adjm(desc, strips, gr.nstrips, L, fl, cfg_.march_w, cfg_.march_nw, cfg_.snap_fmt, cfg_.xcd, st));
     else
       STCHK(launch_adj(desc, tiles, gr.ntiles, L, fl, cfg_.txr, cfg_.snap_fmt, cfg_.xcd, st));
     L -= two ? 2 : 1;

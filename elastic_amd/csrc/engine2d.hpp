@@ -35,6 +35,7 @@ struct EngineCfg {
                        // levels per launch (temporal blocking), 2 one level
                        // per launch marching along x (tile width 2 only)
   int march_w = 16;    // strip width (columns, even) of the marching form
+  int march_nw = 2;    // strip height of the marching form in waves (64 rows each): 1, 2, 4
   int max_group = 0;   // cap on shots per group (0 = memory bound only)
   int stack_nz = 0, stack_nx = 0;  // stacked image (0 = none)
   double mem_fraction = 0.85;      // share of free memory the working set may use
@@ -132,6 +133,7 @@ int default_txr();
 int default_xcd();
 int default_tblock();
 int default_march_w();
+int default_march_nw();
 // Tile autotuner (autotune.hip): times the engine on a short copy of the
 // workload for TXR 1, 2, 4, 8 with one level per launch and TXR 1, 2, 4 with
 // two, and returns the fastest in *best / *best_tblock (ms_out[tblock*4 +

@@ -146,15 +146,19 @@ def test_tile_widths_agree_and_autotuner(hip, j):
     b.close()
 
 
-@pytest.mark.parametrize("prec,j,mw", [("fp32", 1, 32), ("fp32", 3, 6), ("bf16", 4, 2), ("fp16", 2, 10)])
-def test_marching_form_equals_tiles(hip, monkeypatch, prec, j, mw):
-    """The marching step kernels (one wave per strip of mw columns walking
-    along x, register queues for the x-differences) give bitwise the same
-    observed data and images as the tile kernels of width 2; 16-bit snapshots
-    use the same 64 x 2 scale strips."""
+@pytest.mark.parametrize("prec,j,mw,nw", [("fp32", 1, 32, 4), ("fp32", 3, 6, 1), ("bf16", 4, 2, 2),
+                                          ("fp16", 2, 10, 4), ("bf16", 4, 16, 4)])
+def test_marching_form_equals_tiles(hip, monkeypatch, prec, j, mw, nw):
+    """The marching step kernels (a workgroup of nw waves per strip of
+    64*nw rows x mw columns walking along x, neighbouring strips in opposite
+    directions, register queues for the x-differences) give bitwise the same
+    observed data and images as the tile kernels of width 2; 16-bit
+    snapshots use the same 64 x 2 scale strips. The 90-row grid leaves whole
+    waves of a 4-wave strip below the grid."""
     specs, ctrue, cmig, wav = _survey(hip)
     out = {}
     monkeypatch.setenv("RTM_MARCH_W", str(mw))
+    monkeypatch.setenv("RTM_MARCH_NW", str(nw))
     for tb in ("0", "2"):
         monkeypatch.setenv("RTM_TBLOCK", tb)
         b = _batch(hip, specs, ctrue, cmig, wav, txr=2, snap_every=j, snap_prec=prec, comp_image=int(prec != "fp32"))
@@ -175,6 +179,10 @@ def test_marching_form_needs_width_two(hip, monkeypatch):
     with pytest.raises(hip.RtmError):
         hip.Batch(NT, DX, DT, specs, cmig, wav, abc=ABC, txr=4)
     monkeypatch.setenv("RTM_MARCH_W", "7")
+    with pytest.raises(hip.RtmError):
+        hip.Batch(NT, DX, DT, specs, cmig, wav, abc=ABC, txr=2)
+    monkeypatch.setenv("RTM_MARCH_W", "16")
+    monkeypatch.setenv("RTM_MARCH_NW", "3")
     with pytest.raises(hip.RtmError):
         hip.Batch(NT, DX, DT, specs, cmig, wav, abc=ABC, txr=2)
 

@@ -289,12 +289,12 @@ def test_sponge_free_forward_and_migrate(hip, monkeypatch, tblock):
         assert O.rel_l2(a, b) <= IMG_TOL
 
 
-@pytest.mark.parametrize("mw", ["2", "6", "32"])
-def test_marching_form_single_shot_paths_bitwise(hip, monkeypatch, mw):
-    """The marching step kernels (RTM_TBLOCK=2, strips of mw columns) give
-    bitwise the tile kernels' results on the drop-in adjoint (ragged grid),
-    the forward with receivers, Born modelling and migration with the exact
-    sponge adjoint and compensated images."""
+@pytest.mark.parametrize("mw,nw", [("2", "1"), ("6", "2"), ("32", "4")])
+def test_marching_form_single_shot_paths_bitwise(hip, monkeypatch, mw, nw):
+    """The marching step kernels (RTM_TBLOCK=2, strips of mw columns and nw
+    waves) give bitwise the tile kernels' results on the drop-in adjoint
+    (ragged grid), the forward with receivers, Born modelling and migration
+    with the exact sponge adjoint and compensated images."""
     hist, coefs5 = _adjoint_inputs(7, 70, 130, 25)
     p, w, coefs, m, geo = _shot(nt=160)
     rng = np.random.default_rng(2)
@@ -302,6 +302,7 @@ def test_marching_form_single_shot_paths_bitwise(hip, monkeypatch, mw):
     d, cs = _observed(p, w, coefs, m, geo, 1e-3)
     out = {}
     monkeypatch.setenv("RTM_MARCH_W", mw)
+    monkeypatch.setenv("RTM_MARCH_NW", nw)
     for tb in ("0", "2"):
         monkeypatch.setenv("RTM_TBLOCK", tb)
         op3 = hip.rtm_op3(hist, coefs5)
