@@ -79,13 +79,23 @@ __device__ __forceinline__ void halo_cell(int h, int& lx, int& lz) {
   }
 }
 
+// Global loads through address-space-1 pointers (GLOBAL, not FLAT,
+// instructions: FLAT ones also count against the LDS wait counter).
+template <class T>
+__device__ __forceinline__ T g1(const T* p, int i) {
+  return ((const __attribute__((address_space(1))) T*)p)[i];
+}
+
 // The march along y is software-pipelined: the global loads of plane y+1
 // (its new queue entry, its LDS halo, and the own-cell p_prev, c, snapshot
 // and image operands) are issued before plane y is computed, so their
-// latency hides behind the arithmetic of the current plane.
+// latency hides behind the arithmetic of the current plane. The plane tile
+// is double-buffered in LDS with one barrier per plane whose fences cover LDS
+// only, so those loads stay in flight across it; they are issued with
+// clamped indices and masked where the values are taken.
 template <bool IMG>
 __global__ __launch_bounds__(256) void k_ac3_step(const Ac3Args a) {
-  __shared__ float sh[ALX * ALZ];
+  __shared__ float sh[2][ALX * ALZ];
   const int nbz = (a.n1 - 10 + AZ - 1) / AZ;
   const int nbx = (a.n2 - 10 + AX - 1) / AX;
   int b = blockIdx.x;
@@ -115,8 +125,8 @@ __global__ __launch_bounds__(256) void k_ac3_step(const Ac3Args a) {
     if (h < AHALO) {
       halo_cell(h, lx, lz);
       hlds[k] = lx * ALZ + lz;
-      const int gx = x0 - AH + lx, g1 = z0 - AH + lz;
-      if (gx >= 0 && gx < a.n2 && g1 >= 0 && g1 < a.n1) hoff[k] = g1 + gx * s2;
+      const int gx = x0 - AH + lx, g1v = z0 - AH + lz;
+      if (gx >= 0 && gx < a.n2 && g1v >= 0 && g1v < a.n1) hoff[k] = g1v + gx * s2;
     }
   }
   int coff[ATXR];  // own columns (-1: outside the interior)
@@ -130,49 +140,58 @@ __global__ __launch_bounds__(256) void k_ac3_step(const Ac3Args a) {
 #pragma unroll
   for (int k = 0; k < 10; ++k)
 #pragma unroll
-    for (int j = 0; j < ATXR; ++j) q[k][j] = coff[j] >= 0 ? a.p[coff[j] + (y0 - AH + k) * s3] : 0.f;
-  float hv[ANH], ppv[ATXR], cv[ATXR], snv[ATXR], imv[ATXR];
-  auto load_plane = [&](int y, float (&h)[ANH], float (&qn)[ATXR], float (&pv)[ATXR], float (&c)[ATXR],
+    for (int j = 0; j < ATXR; ++j) {
+      const bool ok = coff[j] >= 0;
+      const float v = g1(a.p, ok ? coff[j] + (y0 - AH + k) * s3 : 0);
+      q[k][j] = ok ? v : 0.f;
+    }
+  float hv[ANH], ppv[ATXR], cv[ATXR], snv[ATXR], imv[ATXR], qn[ATXR];
+  // raw loads of plane y (clamped indices; own-cell values are used only
+  // where coff >= 0, queue and halo values are masked where taken)
+  auto load_plane = [&](int y, float (&h)[ANH], float (&qv)[ATXR], float (&pv)[ATXR], float (&c)[ATXR],
                         float (&sn)[ATXR], float (&im)[ATXR]) {
     const int py = y * s3;
 #pragma unroll
     for (int j = 0; j < ATXR; ++j) {
-      const bool ok = coff[j] >= 0;
-      const int o = coff[j] + py;
-      qn[j] = ok ? a.p[o + AH * s3] : 0.f;
-      pv[j] = ok ? a.pp[o] : 0.f;
-      c[j] = ok ? a.c[o] : 0.f;
+      const int o = coff[j] >= 0 ? coff[j] + py : 0;
+      qv[j] = g1(a.p, coff[j] >= 0 ? o + AH * s3 : 0);
+      pv[j] = g1(static_cast<const float*>(a.pp), o);
+      c[j] = g1(a.c, o);
       sn[j] = 0.f;
       im[j] = 0.f;
-      if (imaging && ok) {
-        sn[j] = a.snap_bf16 ? bf16_to_f(static_cast<const unsigned short*>(a.snap)[o]) * a.snap_scale
-                            : static_cast<const float*>(a.snap)[o];
-        im[j] = a.img[o];
+      if (imaging) {
+        sn[j] = a.snap_bf16 ? bf16_to_f(g1(static_cast<const unsigned short*>(a.snap), o)) * a.snap_scale
+                            : g1(static_cast<const float*>(a.snap), o);
+        im[j] = g1(static_cast<const float*>(a.img), o);
       }
     }
 #pragma unroll
-    for (int k = 0; k < ANH; ++k) h[k] = hoff[k] >= 0 ? a.p[hoff[k] + py] : 0.f;
+    for (int k = 0; k < ANH; ++k) h[k] = g1(a.p, hoff[k] >= 0 ? hoff[k] + py : 0);
   };
-  float qn[ATXR];
   if (y0 < y1) load_plane(y0, hv, qn, ppv, cv, snv, imv);
   float mx = 0.f;
   for (int y = y0; y < y1; ++y) {
+    const int bb = (y - y0) & 1;
+    float* const sp = sh[bb];
 #pragma unroll
-    for (int j = 0; j < ATXR; ++j) q[10][j] = qn[j];
-    __syncthreads();  // the previous plane's readers are done
+    for (int j = 0; j < ATXR; ++j) q[10][j] = coff[j] >= 0 ? qn[j] : 0.f;
 #pragma unroll
-    for (int j = 0; j < ATXR; ++j) sh[(ty * ATXR + j + AH) * ALZ + tz + AH] = q[5][j];
+    for (int j = 0; j < ATXR; ++j) sp[(ty * ATXR + j + AH) * ALZ + tz + AH] = q[5][j];
 #pragma unroll
     for (int k = 0; k < ANH; ++k)
-      if (hlds[k] >= 0) sh[hlds[k]] = hv[k];
-    __syncthreads();
+      if (hlds[k] >= 0) sp[hlds[k]] = hoff[k] >= 0 ? hv[k] : 0.f;
+    // the tile of plane y is complete; plane y-1's buffer is free again
+    // once every wave has passed this barrier
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
     // issue the next plane's loads before this plane's arithmetic
     float hv2[ANH], pp2[ATXR], c2[ATXR], sn2[ATXR], im2[ATXR];
-    if (y + 1 < y1) load_plane(y + 1, hv2, qn, pp2, c2, sn2, im2);
+    load_plane(y + 1 < y1 ? y + 1 : y, hv2, qn, pp2, c2, sn2, im2);
     const int py = y * s3;
     if (zok) {
-      const float kzv = a.kz ? a.kz[gz] : 1.f;
-      const float kyv = a.kz ? a.ky[y] : 1.f;
+      const float kzv = a.kz ? g1(a.kz, gz) : 1.f;
+      const float kyv = a.kz ? g1(a.ky, y) : 1.f;
 #pragma unroll
       for (int j = 0; j < ATXR; ++j) {
         const int lxo = ty * ATXR + j;
@@ -180,13 +199,13 @@ __global__ __launch_bounds__(256) void k_ac3_step(const Ac3Args a) {
         if (coff[j] < 0) continue;
         const int p0 = (lxo + AH) * ALZ + tz + AH;
         float lap = 3.f * W0 * q[5][j];
-        lap += W1 * ((sh[p0 + 1] + sh[p0 - 1]) + (sh[p0 + ALZ] + sh[p0 - ALZ]) + (q[6][j] + q[4][j]));
-        lap += W2 * ((sh[p0 + 2] + sh[p0 - 2]) + (sh[p0 + 2 * ALZ] + sh[p0 - 2 * ALZ]) + (q[7][j] + q[3][j]));
-        lap += W3 * ((sh[p0 + 3] + sh[p0 - 3]) + (sh[p0 + 3 * ALZ] + sh[p0 - 3 * ALZ]) + (q[8][j] + q[2][j]));
-        lap += W4 * ((sh[p0 + 4] + sh[p0 - 4]) + (sh[p0 + 4 * ALZ] + sh[p0 - 4 * ALZ]) + (q[9][j] + q[1][j]));
-        lap += W5 * ((sh[p0 + 5] + sh[p0 - 5]) + (sh[p0 + 5 * ALZ] + sh[p0 - 5 * ALZ]) + (q[10][j] + q[0][j]));
+        lap += W1 * ((sp[p0 + 1] + sp[p0 - 1]) + (sp[p0 + ALZ] + sp[p0 - ALZ]) + (q[6][j] + q[4][j]));
+        lap += W2 * ((sp[p0 + 2] + sp[p0 - 2]) + (sp[p0 + 2 * ALZ] + sp[p0 - 2 * ALZ]) + (q[7][j] + q[3][j]));
+        lap += W3 * ((sp[p0 + 3] + sp[p0 - 3]) + (sp[p0 + 3 * ALZ] + sp[p0 - 3 * ALZ]) + (q[8][j] + q[2][j]));
+        lap += W4 * ((sp[p0 + 4] + sp[p0 - 4]) + (sp[p0 + 4 * ALZ] + sp[p0 - 4 * ALZ]) + (q[9][j] + q[1][j]));
+        lap += W5 * ((sp[p0 + 5] + sp[p0 - 5]) + (sp[p0 + 5 * ALZ] + sp[p0 - 5 * ALZ]) + (q[10][j] + q[0][j]));
         const int gi = coff[j] + py;
-        const float kk = a.kz ? kzv * a.kx[gx] * kyv : 1.f;
+        const float kk = a.kz ? kzv * g1(a.kx, gx) * kyv : 1.f;
         float v = kk * (2.f * q[5][j] - ppv[j] + cv[j] * lap);
         if (a.mode == 0) {
           if (gz == a.sz && gx == a.sx && y == a.sy) v += a.amp;
