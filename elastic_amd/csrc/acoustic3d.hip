@@ -98,7 +98,9 @@ __global__ __launch_bounds__(256) void k_ac3_step(const Ac3Args a) {
   __shared__ float sh[2][ALX * ALZ];
   const int nbz = (a.n1 - 10 + AZ - 1) / AZ;
   const int nbx = (a.n2 - 10 + AX - 1) / AX;
-  int b = blockIdx.x;
+  // z-fastest tile order, renumbered so each XCD gets a contiguous range:
+  // the z- and x-neighbours whose halos a tile reads share its L2
+  int b = xcd_tile(blockIdx.x, gridDim.x);
   const int bz = b % nbz;
   b /= nbz;
   const int bx = b % nbx;

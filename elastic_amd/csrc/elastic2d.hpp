@@ -81,6 +81,15 @@ enum : int {
   A_IMG2 = 128 // two-level step: also image level L-1
 };
 
+// Workgroups are dispatched round-robin over the 8 XCDs; renumber them so
+// each XCD receives a contiguous range of tiles (shot-major, column-major),
+// which keeps neighbouring tiles -- and their shared halos -- in one L2.
+__device__ __forceinline__ int xcd_tile(int L, int N) {
+  const int q = N >> 3, r = N & 7;
+  const int x = L & 7, pos = L >> 3;
+  return (x < r) ? x * (q + 1) + pos : r * (q + 1) + (x - r) * q + pos;
+}
+
 // Tile geometry: 64 z-cells (one wave64 along the fast axis) by 4*TXR
 // x-cells, 256 threads, radius-5 halo staged in LDS.
 constexpr int TILE_Z = 64;

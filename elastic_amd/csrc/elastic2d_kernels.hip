@@ -55,14 +55,6 @@ __device__ __forceinline__ float sm(const float* f, int p) {
   return r;
 }
 
-// Workgroups are dispatched round-robin over the 8 XCDs; renumber them so
-// each XCD receives a contiguous range of tiles (shot-major, column-major),
-// which keeps neighbouring tiles -- and their shared halos -- in one L2.
-__device__ __forceinline__ int xcd_tile(int L, int N) {
-  const int q = N >> 3, r = N & 7;
-  const int x = L & 7, pos = L >> 3;
-  return (x < r) ? x * (q + 1) + pos : r * (q + 1) + (x - r) * q + pos;
-}
 
 __device__ __forceinline__ void neumaier(float& s, float& c, float t) {
 #pragma clang fp contract(off)
