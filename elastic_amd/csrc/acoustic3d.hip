@@ -20,6 +20,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <memory>
 #include <vector>
@@ -345,7 +346,13 @@ static int ac3_launch(ac3d* a, const float* p, float* pp, const float* c, int mo
   x.snap_scale = scale;
   x.snap_bf16 = g.snap_prec == 1;
   x.img = img;
-  x.ychunk = 64;
+  // planes per workgroup along y (RTM_AC3_YCHUNK overrides; experiments)
+  static const int ychunk = [] {
+    const char* e = getenv("RTM_AC3_YCHUNK");
+    const int v = e ? atoi(e) : 64;
+    return v >= 8 ? v : 64;
+  }();
+  x.ychunk = ychunk;
   const int nbz = (g.n1 - 10 + AZ - 1) / AZ, nbx = (g.n2 - 10 + AX - 1) / AX;
   const int nby = (g.n3 - 10 + x.ychunk - 1) / x.ychunk;
   if (mode == 1 && img)
