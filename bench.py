@@ -275,7 +275,7 @@ def main():
     # copy kernel) times this run's cell-updates per launch
     # kernel form the batch runs (autotuned unless --txr / RTM_TXR / RTM_TBLOCK fix it)
     form, txr = b.tblock, b.txr
-    kname = ("k_" + dom) + {0: "", 1: "2", 2: "m"}[form]
+    kname = ("k_" + dom) + {0: "", 1: "2", 2: "m", 3: ("m" if dom == "fwd" else "")}[form]
     cells_per_launch = (info["adj_cells"] / info["adj_launches"]) if dom == "adj" else \
         (info["fwd_cells"] / info["fwd_launches"])
     tr = load_traffic(args.variant, kname)
@@ -317,7 +317,8 @@ def main():
                    "variant": args.variant, "shots": args.shots, "nt": args.nt,
                    "shots_per_group": int(info["group_size"]),
                    "kernel_form": {0: "tiles, one level per launch", 1: "tiles, two levels per launch",
-                                   2: "one wave per strip marching along x"}[form],
+                                   2: "strips marching along x",
+                                   3: "strips marching along x (forward), tiles (adjoint)"}[form],
                    "tile_width": txr},
         "shots_per_hour": round(shots_per_hour, 1),
         "mcells_computational": round(comp / (ms_step * 1e-3) / 1e6, 1),

@@ -61,7 +61,7 @@ static void load_file_cache() {
   int txr = 0;
   while (std::fscanf(f, "%511s %d", key, &txr) == 2) {
     const int w = txr & 15, tb = txr >> 4;
-    if ((w == 1 || w == 2 || w == 4 || w == 8) && (tb == 0 || (tb == 1 && w <= 4) || (tb == 2 && w == 2)))
+    if ((w == 1 || w == 2 || w == 4 || w == 8) && (tb == 0 || (tb == 1 && w <= 4) || (tb >= 2 && tb <= 3 && w == 2)))
       g_tune_cache[key] = txr;
   }
   std::fclose(f);
@@ -108,26 +108,37 @@ int autotune_txr(const EngineCfg& cfg, const std::vector<ShotSpec>& shots, const
   c.max_group = 0;
   // candidates (width, form): widths 1, 2, 4, 8 with one level per launch
   // (slots 0..3), widths 1, 2, 4 with two levels per launch (slots 4..6) and
-  // the marching form (slot 7)
-  const int cand[8][2] = {{1, 0}, {2, 0}, {4, 0}, {8, 0}, {1, 1}, {2, 1}, {4, 1}, {2, 2}};
+  // the marching forms (slot 7: the faster of marching forward and adjoint,
+  // form 2, and marching forward with the tile adjoint, form 3)
+  int cand[8][2] = {{1, 0}, {2, 0}, {4, 0}, {8, 0}, {1, 1}, {2, 1}, {4, 1}, {2, 2}};
   double t[8];
   int arg = -1;
-  for (int i = 0; i < 8; ++i) {
-    t[i] = -1.0;
-    c.txr = cand[i][0];
-    c.tblock = cand[i][1];
+  auto time_form = [&](int txr, int form, double* ms) -> int {
+    c.txr = txr;
+    c.tblock = form;
     Engine e;
     STCHK(e.setup(c, sp, cf, wavelet));
     STCHK(e.migrate());  // warm-up (and graph instantiation)
     STCHK(e.migrate());
-    t[i] = e.timing().total_ms;
+    *ms = e.timing().total_ms;
+    return OK;
+  };
+  for (int i = 0; i < 8; ++i) {
+    STCHK(time_form(cand[i][0], cand[i][1], &t[i]));
+    if (i == 7) {
+      double t3 = 0;
+      STCHK(time_form(2, 3, &t3));
+      if (t3 < t[7]) {
+        t[7] = t3;
+        cand[7][1] = 3;
+      }
+    }
     if (arg < 0 || t[i] < t[arg]) arg = i;
   }
-  // The marching form reads 1.3x (forward) to 1.8x (adjoint) the HBM bytes
-  // of the tile form (its priming columns and halo rows are re-read from
-  // HBM by neighbouring strips); it is taken only when clearly faster on
-  // the short run, not on a near-tie that a longer, bandwidth-bound run
-  // would lose.
+  // The marching forms read 1.1x (forward) to 1.4x (adjoint) the HBM bytes
+  // of the tile form (halo rows and priming columns re-read by neighbouring
+  // strips); they are taken only when clearly faster on the short run, not
+  // on a near-tie that a longer, bandwidth-bound run would lose.
   if (arg == 7) {
     int alt = -1;
     for (int i = 0; i < 7; ++i)

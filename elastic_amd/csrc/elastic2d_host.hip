@@ -91,7 +91,8 @@ static int op3(int nt, int nz, int nx, float* const* hist, const float* const* c
   d.lev = hp; d.lev_stride = n2; d.fstride = (long long)nt * n2; d.lev_mod = nt; d.lev_off = 0;
   d.coef = cf.as<float>();
   const int txr = default_txr();
-  // kernel form 2 (RTM_TBLOCK=2): one wave per strip of RTM_MARCH_W columns
+  // kernel form 2 (RTM_TBLOCK=2): strips of RTM_MARCH_W columns marching
+  // along x (form 3 marches the forward only: tiles here)
   const bool march = default_tblock() == 2;
   const int mw = default_march_w(), mnw = default_march_nw();
   if (march && (mw < 2 || (mw & 1))) { set_error("RTM_MARCH_W=%d is not a positive even number", mw); return E_ARG; }

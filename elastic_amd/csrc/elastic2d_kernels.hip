@@ -724,6 +724,25 @@ struct SrcView {
     if (SNAP == SNAP_HIST || SNAP == SNAP_F32) return r.in ? r.v : 0.f;
     return r.in ? SnapQ<SNAP>::dec(r.h) * r.v : 0.f;
   }
+  // The element alone (16-bit formats: the scale comes from scale_u).
+  __device__ __forceinline__ void load_q(int a, int gi, bool in, Raw& r) const {
+    r.in = in;
+    if (SNAP == SNAP_HIST || SNAP == SNAP_F32)
+      r.v = gld(f[a], in ? gi : 0);
+    else
+      r.h = gld(q[a], in ? gi : 0);
+  }
+  // 16-bit formats: the scale of the strip of 64-row block zb and column gx,
+  // read through the constant address space, so that a wave-uniform block and
+  // column make it one scalar load into a scalar register.
+  __device__ __forceinline__ float scale_u(int a, int zb, int gx) const {
+    if (SNAP == SNAP_HIST || SNAP == SNAP_F32) return 1.f;
+    return ((const __attribute__((address_space(4))) float*)sc[a])[zb + nbz * (gx >> sw)];
+  }
+  __device__ __forceinline__ float value_s(const Raw& r, float s) const {
+    if (SNAP == SNAP_HIST || SNAP == SNAP_F32) return r.in ? r.v : 0.f;
+    return r.in ? SnapQ<SNAP>::dec(r.h) * s : 0.f;
+  }
   // the same as a branch-free global load, zero where !in
   __device__ __forceinline__ float getm(int a, int gi, int gz, int gx, bool in) const {
     if (SNAP == SNAP_HIST || SNAP == SNAP_F32) return gldm(f[a], gi, in);
@@ -1689,6 +1708,12 @@ __device__ __forceinline__ void adjm_strip(const ShotDev& S, const TileRef& tr, 
   const bool comp = flags & A_COMP;
   const bool xin = flags & A_XIN;
   const SrcView<SNAP> sv(S, L);
+  // 64-row blocks of the 16-bit snapshot scales: the wave's own rows (clamped
+  // for a wave wholly below the grid) and the halo rows above and below the
+  // strip; all wave-uniform, so their scales are scalar loads
+  const int zbw = min(__builtin_amdgcn_readfirstlane(gz) >> 6, S.snap_nbz - 1);
+  const int zbt = max((z0 >> 6) - 1, 0), zbb = min((z0 >> 6) + NW, S.snap_nbz - 1);
+  auto cx = [&](int gx) { return min(max(gx, 0), nx - 1); };
 
   // the ten level-(L+1) operands at element i (clamped index), as loaded
   struct Raw {
@@ -1736,6 +1761,7 @@ __device__ __forceinline__ void adjm_strip(const ShotDev& S, const TileRef& tr, 
     float b[5], add[5];  // level L+2; the level's own content (drop-in form) or the injected receivers
     Raw h;
     typename SrcView<SNAP>::Raw hs[2], nt2[2];
+    float hsc[2][2], ntsc[2];  // their 16-bit scales (halo: above, below)
     float im[3], imc[3];
   };
   auto load_col = [&](int gx, bool more, Col& c) {
@@ -1754,13 +1780,20 @@ __device__ __forceinline__ void adjm_strip(const ShotDev& S, const TileRef& tr, 
     }
     raw(ih, c.h);
     if (imaging) {
-      sv.load(0, ih, m.hz, gx, m.hok && more, c.hs[0]);
-      sv.load(1, ih, m.hz, gx, m.hok && more, c.hs[1]);
+#pragma unroll
+      for (int a = 0; a < 2; ++a) {
+        sv.load_q(a, ih, m.hok && more, c.hs[a]);
+        c.hsc[a][0] = sv.scale_u(a, zbt, cx(gx));
+        c.hsc[a][1] = sv.scale_u(a, zbb, cx(gx));
+      }
       if (next) {  // arrays 2, 3 exist only below the last level
-        sv.load(2, ic, gz, gx, m.zok && more, c.nt2[0]);
-        sv.load(3, ic, gz, gx, m.zok && more, c.nt2[1]);
+        sv.load_q(2, ic, m.zok && more, c.nt2[0]);
+        sv.load_q(3, ic, m.zok && more, c.nt2[1]);
+        c.ntsc[0] = sv.scale_u(2, zbw, cx(gx));
+        c.ntsc[1] = sv.scale_u(3, zbw, cx(gx));
       } else {
         c.nt2[0].in = c.nt2[1].in = false;
+        c.ntsc[0] = c.ntsc[1] = 0.f;
       }
 #pragma unroll
       for (int a = 0; a < 3; ++a) {
@@ -1786,9 +1819,12 @@ __device__ __forceinline__ void adjm_strip(const ShotDev& S, const TileRef& tr, 
     Raw nr;
     raw(iq, nr);
     typename SrcView<SNAP>::Raw ns0, ns1;
+    float nsc0 = 0.f, nsc1 = 0.f;
     if (imaging) {
-      sv.load(0, iq, gz, xq, qok, ns0);
-      sv.load(1, iq, gz, xq, qok, ns1);
+      sv.load_q(0, iq, qok, ns0);
+      sv.load_q(1, iq, qok, ns1);
+      nsc0 = sv.scale_u(0, zbw, cx(xq));
+      nsc1 = sv.scale_u(1, zbw, cx(xq));
     }
     Col nxt;
     load_col(x + DIR, more, nxt);
@@ -1809,8 +1845,8 @@ __device__ __forceinline__ void adjm_strip(const ShotDev& S, const TileRef& tr, 
       col[b][2][m.hs] = cur_h ? h.m1x * h.vx : 0.f;
       col[b][3][m.hs] = cur_h ? h.m1z * h.vz : 0.f;
       if (imaging) {
-        col[b][4][m.hs] = sv.value(cur.hs[0]);
-        col[b][5][m.hs] = sv.value(cur.hs[1]);
+        col[b][4][m.hs] = sv.value_s(cur.hs[0], m.lane < 5 ? cur.hsc[0][0] : cur.hsc[0][1]);
+        col[b][5][m.hs] = sv.value_s(cur.hs[1], m.lane < 5 ? cur.hsc[1][0] : cur.hsc[1][1]);
       }
     }
     strip_lds_sync<NW>();
@@ -1853,11 +1889,11 @@ __device__ __forceinline__ void adjm_strip(const ShotDev& S, const TileRef& tr, 
         float s1 = 0.f, s2 = 0.f;
         if (next) {
           if (SNAP == SNAP_HIST) {  // arrays 2, 3 are Vx, Vz of level L+1
-            s1 = inv_dt * (sv.value(cur.nt2[0]) - s0q[5]);
-            s2 = inv_dt * (sv.value(cur.nt2[1]) - s1q[5]);
+            s1 = inv_dt * (sv.value_s(cur.nt2[0], cur.ntsc[0]) - s0q[5]);
+            s2 = inv_dt * (sv.value_s(cur.nt2[1], cur.ntsc[1]) - s1q[5]);
           } else {
-            s1 = inv_dt * sv.value(cur.nt2[0]);
-            s2 = inv_dt * sv.value(cur.nt2[1]);
+            s1 = inv_dt * sv.value_s(cur.nt2[0], cur.ntsc[0]);
+            s2 = inv_dt * sv.value_s(cur.nt2[1], cur.ntsc[1]);
           }
         }
         float s4 = 0.f, s5 = 0.f, s6 = 0.f;
@@ -1894,8 +1930,8 @@ __device__ __forceinline__ void adjm_strip(const ShotDev& S, const TileRef& tr, 
     for (int k = 0; k < 5; ++k) d1[k] = d1[k + 1];
     d1[5] = qok ? fmaf(nr.m2, nr.xx, nr.m2m3 * nr.zz) : 0.f;
     if (imaging) {
-      push(s0q, sv.value(ns0));
-      push(s1q, sv.value(ns1));
+      push(s0q, sv.value_s(ns0, nsc0));
+      push(s1q, sv.value_s(ns1, nsc1));
     }
     cur = nxt;
     cur_h = m.hok && more;

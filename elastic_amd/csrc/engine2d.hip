@@ -123,8 +123,8 @@ int Engine::setup(const EngineCfg& cfg, const std::vector<ShotSpec>& shots,
     set_error("tile width %d not in {1,2,4,8}", cfg_.txr);
     return E_ARG;
   }
-  if (cfg_.tblock < 0 || cfg_.tblock > 2 ||
-      (cfg_.tblock == 2 && (cfg_.txr != 2 || cfg_.march_w < 2 || (cfg_.march_w & 1) ||
+  if (cfg_.tblock < 0 || cfg_.tblock > 3 ||
+      (cfg_.tblock >= 2 && (cfg_.txr != 2 || cfg_.march_w < 2 || (cfg_.march_w & 1) ||
                             (cfg_.march_nw != 1 && cfg_.march_nw != 2 && cfg_.march_nw != 4)))) {
     set_error("kernel form %d with tile width %d / strip %d columns x %d waves: marching steps need tile width 2, "
               "an even strip width and 1, 2 or 4 waves",
@@ -235,7 +235,7 @@ int Engine::build_groups() {
   const int ns = (int)specs_.size();
   std::vector<TileRef> tiles, strips;
   const int tx = tile_x(cfg_.txr);
-  const bool march = cfg_.tblock == 2;
+  const bool march = cfg_.tblock >= 2;  // forms 2, 3 march (3: the forward only)
   int s = 0;
   G_ = 0;
   while (s < ns) {
@@ -337,7 +337,7 @@ int Engine::enqueue_forward(int g, int mode, hipStream_t st) {
   const int variant = (mode == 1) ? 1 : (mode >= 4 ? 2 : 0);
   const ShotDev* desc = d_desc_.as<ShotDev>() + (size_t)variant * specs_.size() + gr.desc0;
   const TileRef* tiles = d_tiles_.as<TileRef>() + gr.tile0;
-  const bool march = cfg_.tblock == 2;
+  const bool march = cfg_.tblock >= 2;
   const TileRef* strips = march ? d_strips_.as<TileRef>() + gr.strip0 : nullptr;
   const int snap = (mode == 0) ? cfg_.snap_fmt : SNAP_HIST;
   for (int s : gr.shots) {
@@ -402,7 +402,7 @@ int Engine::enqueue_adjoint(int g, hipStream_t st) {
   const Group& gr = groups_[g];
   const ShotDev* desc = d_desc_.as<ShotDev>() + gr.desc0;
   const TileRef* tiles = d_tiles_.as<TileRef>() + gr.tile0;
-  const bool march = cfg_.tblock == 2;
+  const bool march = cfg_.tblock == 2;  // form 3 marches the forward only
   const TileRef* strips = march ? d_strips_.as<TileRef>() + gr.strip0 : nullptr;
   for (size_t k = 0; k < gr.shots.size(); ++k) {
     Res& r = res_[gr.shots[k]];

@@ -33,7 +33,8 @@ struct EngineCfg {
   int use_graph = 0;   // capture the time loops in HIP graphs
   int tblock = 0;      // kernel form: 0 one level per launch on tiles, 1 two
                        // levels per launch (temporal blocking), 2 one level
-                       // per launch marching along x (tile width 2 only)
+                       // per launch marching along x (tile width 2 only), 3
+                       // the marching forward with the tile adjoint
   int march_w = 16;    // strip width (columns, even) of the marching form
   int march_nw = 2;    // strip height of the marching form in waves (64 rows each): 1, 2, 4
   int max_group = 0;   // cap on shots per group (0 = memory bound only)

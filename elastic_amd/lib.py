@@ -316,7 +316,8 @@ class Batch:
     @property
     def tblock(self):
         """Kernel form: 0 one level per launch on tiles, 1 two levels per
-        launch (temporal blocking), 2 one level per launch marching along x."""
+        launch (temporal blocking), 2 one level per launch marching along x,
+        3 the marching forward with the tile adjoint."""
         return int(load().rtm_batch_tblock(self._h))
 
     def info(self):
@@ -341,7 +342,8 @@ def autotune(nt, dx, dt, shots, coefs, wavelet, abc=0, adj_abc=0, snap_every=1, 
              use_graph=0):
     """Run the tile autotuner on a workload (rtm_batch_autotune): returns
     ((txr, form), {(txr, form): ms}); form 0 one level per launch on tiles,
-    1 two levels per launch, 2 marching along x."""
+    1 two levels per launch, 2 marching along x, 3 marching forward with
+    the tile adjoint (the (2, 2) entry times the faster of forms 2 and 3)."""
     lib = load()
     n = len(shots)
     arr = (RtmBatchShot * n)()

@@ -207,16 +207,17 @@ int rtm_batch_get_stack(rtm_batch *b, float *img3);            /* 3*(stack_nz,st
 int rtm_batch_info(rtm_batch *b, double *out8);
 int rtm_batch_destroy(rtm_batch *b);
 /* Tile width and kernel form the batch runs with (form 0: one level per
- * launch on tiles, 1: two levels per launch, 2: one level per launch, one
- * wave marching along x per strip). */
+ * launch on tiles, 1: two levels per launch, 2: one level per launch,
+ * strips marching along x, 3: the marching forward with the tile adjoint). */
 int rtm_batch_txr(rtm_batch *b);
 int rtm_batch_tblock(rtm_batch *b);
 /* Tile autotuner: runs a short copy of the workload (up to 8 of the shots,
  * max(48, 8*snap_every) levels, same options) with tile widths 1, 2, 4, 8 at
  * one level per launch, 1, 2, 4 at two levels per launch (temporal
- * blocking) and the marching form, and returns the fastest in *txr, *tblock;
+ * blocking) and the marching forms, and returns the fastest in *txr, *tblock;
  * ms8[tblock*4 + log2(txr)] receives the times of the first seven and ms8[7]
- * that of the marching form (ms, -1 for candidates not run). The result
+ * that of the faster marching form, 2 or 3 (ms, -1 for candidates not run).
+ * The result
  * is cached per process and, when RTM_TUNE_CACHE names a file, in it;
  * rtm_batch_create with cfg->txr == 0 uses the cached or tuned choice unless
  * RTM_TXR / RTM_TBLOCK are set. */
