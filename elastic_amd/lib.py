@@ -433,3 +433,106 @@ class Acoustic3D:
             self.close()
         except Exception:
             pass
+
+
+# --- 3D elastic adjoint step (include/elastic3d.h) ---------------------------
+# Blocks are (k, ny+10, nx+10, nz+10) float32, C-ordered: k fields in the
+# order Vy, Vx, Vz, syy, sxx, szz, sxy, sxz, syz (or the 6 coefficients
+# m1_y, m1_x, m1_z, m2, m3, m2m3), each padded by 5 cells per face, z fastest.
+
+def _el3_dims(block):
+    ny, nx, nz = (d - 10 for d in block.shape[1:])
+    return ny, nz, nx
+
+
+def el3_gpu_func(fin, fin1, fout, coef):
+    """The reference's host interface (rtm3d_gpu_init / func / final): one
+    step on host arrays. Returns (new out block, [copy-in, compute, copy-out] s)."""
+    lib = load()
+    fin, fin1, coef = (np.ascontiguousarray(a, np.float32) for a in (fin, fin1, coef))
+    out = np.array(fout, dtype=np.float32, order="C", copy=True)
+    ny, nz, nx = _el3_dims(out)
+    t = np.zeros(3, np.float32)
+    _chk(lib.rtm3d_gpu_init(ny, nz, nx))
+    try:
+        ptrs = [_p(a[k]) for a in (fin, fin1, out) for k in range(9)] + [_p(coef[k]) for k in range(6)]
+        _chk(lib.rtm3d_gpu_func(C.c_int(ny), C.c_int(nz), C.c_int(nx), *ptrs, None, _p(t)))
+    finally:
+        lib.rtm3d_gpu_final()
+    return out, t
+
+
+class DeviceArray:
+    """A float32 array in device memory, allocated through the HIP runtime
+    the library itself links (torch bundles a runtime of its own, and two
+    runtimes in one process do not share a device)."""
+
+    def __init__(self, host):
+        h = np.ascontiguousarray(host, np.float32)
+        self.shape = h.shape
+        self.nbytes = h.nbytes
+        rt = _hiprt()
+        p = C.c_void_p()
+        if rt.hipMalloc(C.byref(p), C.c_size_t(self.nbytes)) != 0:
+            raise RtmError(-2, f"hipMalloc({self.nbytes}) failed")
+        self.ptr = p.value
+        self.put(h)
+
+    def put(self, host):
+        h = np.ascontiguousarray(host, np.float32)
+        assert h.nbytes == self.nbytes
+        if _hiprt().hipMemcpy(C.c_void_p(self.ptr), _p(h), C.c_size_t(self.nbytes), 1) != 0:
+            raise RtmError(-3, "hipMemcpy to device failed")
+
+    def get(self):
+        h = np.empty(self.shape, np.float32)
+        if _hiprt().hipMemcpy(_p(h), C.c_void_p(self.ptr), C.c_size_t(self.nbytes), 2) != 0:
+            raise RtmError(-3, "hipMemcpy to host failed")
+        return h
+
+    def slab(self, k):
+        """Device address of the k-th slab along the first axis."""
+        return self.ptr + k * (self.nbytes // self.shape[0])
+
+    def free(self):
+        if self.ptr:
+            _hiprt().hipFree(C.c_void_p(self.ptr))
+            self.ptr = 0
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+_HIPRT = None
+
+
+def _hiprt():
+    global _HIPRT
+    if _HIPRT is None:
+        load()  # pulls in the runtime librtm_hip.so was linked with
+        _HIPRT = C.CDLL("libamdhip64.so.7")
+    return _HIPRT
+
+
+def el3_steps_dev(bufs, coef, nsteps):
+    """nsteps on device blocks (rtm3d_adj_steps_dev): bufs = [in, in1, out]
+    DeviceArrays of shape (9, ny+10, nx+10, nz+10), coef (6, ...). Returns
+    the three blocks in their rotated roles (in, in1, out)."""
+    lib = load()
+    ny, nz, nx = _el3_dims(bufs[0])
+    P9 = C.c_void_p * 9
+    arrs = [P9(*[C.c_void_p(b.slab(k)) for k in range(9)]) for b in bufs]
+    cp = (C.c_void_p * 6)(*[C.c_void_p(coef.slab(k)) for k in range(6)])
+    _chk(lib.rtm3d_adj_steps_dev(C.c_int(ny), C.c_int(nz), C.c_int(nx), arrs[0], arrs[1], arrs[2], cp,
+                                 C.c_int(nsteps)))
+    by_ptr = {b.ptr: b for b in bufs}
+    return tuple(by_ptr[arrs[i][0]] for i in range(3))
+
+
+def el3_timings():
+    out = np.zeros(4, np.float64)
+    _chk(load().rtm3d_timings(_p(out)))
+    return out

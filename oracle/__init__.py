@@ -311,3 +311,57 @@ def ac3_run(c, amp, src, zr, abc=0, mode=0, rec=None, j=5, snap=None, want_snap=
     if mode == 0:
         return r, s
     return img
+
+
+# --- 3D elastic adjoint step (elastic3d_oracle.cpp) --------------------------
+# Arrays are numpy (k, ny+10, nx+10, nz+10) C-ordered: k fields in the
+# reference's order, each padded by 5 cells per face, z fastest.
+EL3_FIELDS = ("Vy", "Vx", "Vz", "syy", "sxx", "szz", "sxy", "sxz", "syz")
+EL3_COEFS = ("m1_y", "m1_x", "m1_z", "m2", "m3", "m2m3")
+
+
+def _el3_dims(a):
+    ny, nx, nz = (d - 10 for d in a.shape[1:])
+    return ny, nz, nx
+
+
+def el3_step(fin, fin1, fout, coef, dtype=np.float64, nthreads=1):
+    """One step out = out + in1 + gather(in); returns the new out."""
+    fin, fin1, coef = (np.ascontiguousarray(a, dtype) for a in (fin, fin1, coef))
+    out = np.array(fout, dtype=dtype, order="C", copy=True)
+    ny, nz, nx = _el3_dims(out)
+    getattr(lib(), "el3_adj_step_" + _suffix(dtype))(
+        C.c_int(ny), C.c_int(nz), C.c_int(nx), _p(fin), _p(fin1), _p(out), _p(coef), C.c_int(nthreads))
+    return out
+
+
+def el3_steps(b0, b1, b2, coef, nsteps, dtype=np.float64, nthreads=1):
+    """nsteps with the reference's pointer rotation, buffers starting as
+    (in, in1, out) = (b0, b1, b2). Returns (in, in1, out) after the run."""
+    bufs = [np.array(b, dtype=dtype, order="C", copy=True) for b in (b0, b1, b2)]
+    coef = np.ascontiguousarray(coef, dtype)
+    ny, nz, nx = _el3_dims(bufs[0])
+    slots = np.zeros(3, np.int32)
+    getattr(lib(), "el3_adj_steps_" + _suffix(dtype))(
+        C.c_int(ny), C.c_int(nz), C.c_int(nx), _p(bufs[0]), _p(bufs[1]), _p(bufs[2]), _p(coef), C.c_int(nsteps),
+        C.c_int(nthreads), _p(slots))
+    return tuple(bufs[s] for s in slots)
+
+
+def el3_fwd(x, coef, nthreads=1):
+    """The transpose of the step's gather, as the velocity-stress system (FP64)."""
+    x = np.ascontiguousarray(x, np.float64)
+    coef = np.ascontiguousarray(coef, np.float64)
+    y = np.zeros_like(x)
+    ny, nz, nx = _el3_dims(x)
+    lib().el3_fwd_apply_f64(C.c_int(ny), C.c_int(nz), C.c_int(nx), _p(x), _p(y), _p(coef), C.c_int(nthreads))
+    return y
+
+
+def el3_random(ny, nz, nx, seed, k=9):
+    """Reference-style test data (K3/rtmlib-modC.c:190-242): values
+    (r % 1000) / 500 - 1 over the interior, a zero 5-cell rim."""
+    rng = np.random.default_rng(seed)
+    a = np.zeros((k, ny + 10, nx + 10, nz + 10), np.float32)
+    a[:, 5:-5, 5:-5, 5:-5] = rng.integers(0, 1000, size=(k, ny, nx, nz)) / 500.0 - 1.0
+    return a
