@@ -200,6 +200,73 @@ def acoustic3d(args):
     print(json.dumps(out), flush=True)
 
 
+def elastic3d(args):
+    """9-field 3D elastic adjoint step (include/elastic3d.h) on the reference's
+    200^3 test shape (K3/gpu.h) with its data distribution, nt3e steps with the
+    reference's buffer rotation, device-resident: prints its own JSON line
+    (not the headline metric)."""
+    from elastic_amd import lib
+    import oracle as O
+
+    n = args.n3e
+    # parity gate: one step of a 40 x 70 x 30 grid against FP64
+    v = [O.el3_random(40, 70, 30, s) for s in (1, 2, 3)]
+    vc = O.el3_random(40, 70, 30, 4, k=6)
+    dv = [lib.DeviceArray(a) for a in v]
+    got = lib.el3_steps_dev(dv, lib.DeviceArray(vc), 1)[2].get()
+    err = O.rel_l2(got, O.el3_step(v[0], v[1], v[2], vc, nthreads=8))
+    if not err <= 1e-5:
+        raise SystemExit(f"bench: 3D elastic step differs from the FP64 oracle (rel-L2 {err:.3e})")
+    del dv, got
+    log(f"bench: 3D elastic parity vs FP64 rel-L2 {err:.2e}; building {n}^3 data")
+    blocks = [O.el3_random(n, n, n, args.seed + s) for s in range(3)]
+    coef = O.el3_random(n, n, n, args.seed + 3, k=6)
+    dev = [lib.DeviceArray(b) for b in blocks]
+    dc = lib.DeviceArray(coef)
+    cpu = None
+    if not args.no_cpu:
+        nthr = min(16, os.cpu_count() or 1)
+        t0 = time.perf_counter()
+        O.el3_step(blocks[0], blocks[1], blocks[2], coef, dtype=np.float32, nthreads=1)
+        t1 = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        O.el3_step(blocks[0], blocks[1], blocks[2], coef, dtype=np.float32, nthreads=nthr)
+        tn = time.perf_counter() - t0
+        cells = float(n) ** 3
+        cpu = {"value": round(cells / tn / 1e6, 2), "unit": "Mcell-updates/s", "cores": nthr, "kind": "port",
+               "sample": f"one step of the {n}^3 workload, oracle FP32 in the reference's term order "
+                         f"(-O3, no FMA), OpenMP over y/x; 1 thread: {round(cells / t1 / 1e6, 2)} Mcell-updates/s"}
+    del blocks, coef
+    for _ in range(args.warmup):
+        lib.el3_steps_dev(dev, dc, min(args.nt3e, 20))
+    lib.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        dev = list(lib.el3_steps_dev(dev, dc, args.nt3e))
+    lib.synchronize()
+    ms = (time.perf_counter() - t0) * 1e3 / args.steps
+    tm = lib.el3_timings()
+    per_launch_ms = tm[0] / tm[1]
+    cells = tm[2]
+    gbs = 168.0 * cells / (per_launch_ms * 1e-3) / 1e9
+    tr = load_traffic("elastic3d", "k_el3_adj")
+    out = {"metric": "Mcell-updates/s, 3D elastic 9-field adjoint step (one GPU)",
+           "value": round(cells * args.nt3e / (ms * 1e-3) / 1e6, 1), "unit": "Mcell-updates/s", "n_gpus": 1,
+           "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms, 2), "higher_is_better": True,
+           "scaling": "weak", "vs_baseline": None, "dtype": "fp32",
+           "data": "synthetic (the reference's test distribution: (r % 1000)/500 - 1 inside a zero 5-cell rim)",
+           "config": {"workload": f"3D elastic adjoint step, {n}^3 interior + 5-cell padding, 9 fields + 6 "
+                                  f"coefficients, {args.nt3e} steps per run with the reference's buffer rotation",
+                      "parity_rel_l2_fp64": float(f"{err:.3e}")},
+           "roofline": {"bound": "hbm", "kernel": "k_el3_adj", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS,
+                        "unit": "GB/s", "frac": round(gbs / HBM_PEAK_GBS, 4),
+                        "traffic": None if tr is None else round(tr * cells),
+                        "algo_bytes_per_cell": 168, "cells_per_launch": int(cells),
+                        "ms_per_launch": round(per_launch_ms, 5), "planes_per_chunk": int(tm[3])},
+           "cpu_baseline": cpu}
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -218,10 +285,13 @@ def main():
     ap.add_argument("--no-graph", action="store_true",
                     help="launch every level from the host instead of replaying HIP graphs (same kernels; "
                          "for tracing tools that do not follow graph launches)")
-    ap.add_argument("--workload", default="elastic2d", choices=("elastic2d", "acoustic3d"),
-                    help="acoustic3d: the 3D acoustic subdomain instead of the headline 2D workload")
+    ap.add_argument("--workload", default="elastic2d", choices=("elastic2d", "acoustic3d", "elastic3d"),
+                    help="acoustic3d: the 3D acoustic subdomain; elastic3d: the 9-field 3D elastic adjoint "
+                         "step; both instead of the headline 2D workload")
     ap.add_argument("--n3d", type=int, default=512)
     ap.add_argument("--nt3d", type=int, default=1500)
+    ap.add_argument("--n3e", type=int, default=200)
+    ap.add_argument("--nt3e", type=int, default=500)
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -234,6 +304,9 @@ def main():
     lib.load()
     if args.workload == "acoustic3d":
         acoustic3d(args)
+        return
+    if args.workload == "elastic3d":
+        elastic3d(args)
         return
     dx, dt, fp, abc = 10.0, 8e-4, 15.0, 40
     t0 = time.perf_counter()
