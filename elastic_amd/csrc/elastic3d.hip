@@ -11,14 +11,15 @@
 //
 // Here the gathers of one output along one axis are merged: the products
 // coefficient * field that share an axis and a tap set are summed first
-// (e.g. m2m3 sxx + m2 syy + m2 szz for Vx along x), leaving 18 product
-// arrays: 6 gathered along x, 6 along z, 6 along y. A workgroup of 512
-// threads owns a 64 (z) x 8 (x) tile of the (z, x) plane and marches along y
-// over a chunk of planes:
-//   * the 6 x-products of the plane over the tile plus its 5-column x-halo
-//     and the 6 z-products over the tile plus its 5-row z-halo go to LDS;
-//   * the 6 y-products of the thread's own column live in an 11-deep
-//     register queue (planes y-5 .. y+5), refilled one plane ahead.
+// (e.g. m2m3 sxx + m2 syy + m2 szz for Vx along x), which leaves 18 gathers
+// over 9 distinct product arrays. A workgroup of 64 x EW threads (EW = 8 or
+// 16 waves) owns a 64 (z) x EW (x) tile of the (z, x) plane and marches
+// along y over a chunk of planes:
+//   * the 6 products gathered along x, over the tile plus its 5-column
+//     x-halo, and the 6 gathered along z, over the tile plus its 5-row
+//     z-halo, go to LDS;
+//   * the 6 products gathered along y live in an 11-deep register queue of
+//     the thread's own column (planes y-5 .. y+5), refilled 5 planes ahead.
 // Each step moves 168 algorithmic bytes per cell (in, in1, out and the 6
 // coefficients read, out written): HBM-bound.
 
@@ -40,13 +41,9 @@ enum { VY, VX, VZ, SYY, SXX, SZZ, SXY, SXZ, SYZ };
 enum { M1Y, M1X, M1Z, M2, M3, M2M3 };
 
 constexpr int EZ = 64;               // tile rows (z): one wave64
-constexpr int EW = 8;                // waves per workgroup = tile columns (x)
-constexpr int ET = EZ * EW;          // threads
 constexpr int EH = 5;                // stencil radius
-constexpr int ELX = EW + 2 * EH;     // x-band columns (tile + x-halo)
 constexpr int ELZ = EZ + 2 * EH;     // z-band rows (tile + z-halo)
 constexpr int NXH = 2 * EH * EZ;     // x-halo cells (10 columns x 64 rows)
-constexpr int NZH = 2 * EH * EW;     // z-halo cells (10 rows x 8 columns)
 
 struct El3Args {
   int ny, nz, nx;
@@ -64,45 +61,38 @@ __device__ __forceinline__ T gl(const T* p, int i) {
   return ((const __attribute__((address_space(1))) T*)p)[i];
 }
 
-// The 18 products of one cell. x: XVX XVY XVZ XSXX XSXY XSXZ;
-// z: ZVX ZVY ZVZ ZSZZ ZSXZ ZSYZ; y: YVX YVY YVZ YSYY YSXY YSYZ.
-struct Cell {
-  float vy, vx, vz, syy, sxx, szz, sxy, sxz, syz;
-  float m1y, m1x, m1z, m2, m3, m2m3;
-};
+// The 9 distinct products of one cell that the 18 gathers read:
+//   PX = m2m3 sxx + m2 (syy + szz)  (x, set A, Vx)   Pxy = m3 sxy (x: Vy, y: Vx)
+//   PY = m2m3 syy + m2 (szz + sxx)  (y, set A, Vy)   Pxz = m3 sxz (x: Vz, z: Vx)
+//   PZ = m2m3 szz + m2 (sxx + syy)  (z, set A, Vz)   Pyz = m3 syz (z: Vy, y: Vz)
+//   Qx = m1_x Vx (x: sxx, y: sxy, z: sxz)   Qy = m1_y Vy (x: sxy, y: syy, z: syz)
+//   Qz = m1_z Vz (x: sxz, y: syz, z: szz); every other gather uses set B.
+enum { PX, PY, PZ, PXY, PXZ, PYZ, QX, QY, QZ };
 
-__device__ __forceinline__ void load_cell(const El3Args& a, int gi, Cell& c) {
-  c.vy = gl(a.in[VY], gi); c.vx = gl(a.in[VX], gi); c.vz = gl(a.in[VZ], gi);
-  c.syy = gl(a.in[SYY], gi); c.sxx = gl(a.in[SXX], gi); c.szz = gl(a.in[SZZ], gi);
-  c.sxy = gl(a.in[SXY], gi); c.sxz = gl(a.in[SXZ], gi); c.syz = gl(a.in[SYZ], gi);
-  c.m1y = gl(a.m[M1Y], gi); c.m1x = gl(a.m[M1X], gi); c.m1z = gl(a.m[M1Z], gi);
-  c.m2 = gl(a.m[M2], gi); c.m3 = gl(a.m[M3], gi); c.m2m3 = gl(a.m[M2M3], gi);
+__device__ __forceinline__ void products(const El3Args& a, int gi, float p[9]) {
+  const float vy = gl(a.in[VY], gi), vx = gl(a.in[VX], gi), vz = gl(a.in[VZ], gi);
+  const float syy = gl(a.in[SYY], gi), sxx = gl(a.in[SXX], gi), szz = gl(a.in[SZZ], gi);
+  const float sxy = gl(a.in[SXY], gi), sxz = gl(a.in[SXZ], gi), syz = gl(a.in[SYZ], gi);
+  const float m1y = gl(a.m[M1Y], gi), m1x = gl(a.m[M1X], gi), m1z = gl(a.m[M1Z], gi);
+  const float m2 = gl(a.m[M2], gi), m3 = gl(a.m[M3], gi), m2m3 = gl(a.m[M2M3], gi);
+  p[PX] = fmaf(m2m3, sxx, m2 * (syy + szz));
+  p[PY] = fmaf(m2m3, syy, m2 * (szz + sxx));
+  p[PZ] = fmaf(m2m3, szz, m2 * (sxx + syy));
+  p[PXY] = m3 * sxy;
+  p[PXZ] = m3 * sxz;
+  p[PYZ] = m3 * syz;
+  p[QX] = m1x * vx;
+  p[QY] = m1y * vy;
+  p[QZ] = m1z * vz;
 }
 
-__device__ __forceinline__ void x_products(const Cell& c, float p[6]) {
-  p[0] = fmaf(c.m2m3, c.sxx, c.m2 * c.syy) + c.m2 * c.szz;  // Vx, set A
-  p[1] = c.m3 * c.sxy;                                      // Vy, B
-  p[2] = c.m3 * c.sxz;                                      // Vz, B
-  p[3] = c.m1x * c.vx;                                      // sxx, B
-  p[4] = c.m1y * c.vy;                                      // sxy, B
-  p[5] = c.m1z * c.vz;                                      // sxz, B
-}
-__device__ __forceinline__ void z_products(const Cell& c, float p[6]) {
-  p[0] = c.m3 * c.sxz;                                      // Vx, B
-  p[1] = c.m3 * c.syz;                                      // Vy, B
-  p[2] = fmaf(c.m2m3, c.szz, c.m2 * c.sxx) + c.m2 * c.syy;  // Vz, A
-  p[3] = c.m1z * c.vz;                                      // szz, B
-  p[4] = c.m1x * c.vx;                                      // sxz, B
-  p[5] = c.m1y * c.vy;                                      // syz, B
-}
-__device__ __forceinline__ void y_products(const Cell& c, float p[6]) {
-  p[0] = c.m3 * c.sxy;                                      // Vx, B
-  p[1] = fmaf(c.m2m3, c.syy, c.m2 * c.szz) + c.m2 * c.sxx;  // Vy, A
-  p[2] = c.m3 * c.syz;                                      // Vz, B
-  p[3] = c.m1y * c.vy;                                      // syy, B
-  p[4] = c.m1x * c.vx;                                      // sxy, B
-  p[5] = c.m1z * c.vz;                                      // syz, B
-}
+// LDS slots of the x-band (gathers along x) and the z-band (along z).
+constexpr int XB[6] = {PX, PXY, PXZ, QX, QY, QZ};
+constexpr int ZB[6] = {PXZ, PYZ, PZ, QZ, QX, QY};
+// register queue along y: planes y-5 .. y+5 of these products
+constexpr int YQ[6] = {PY, PXY, PYZ, QX, QY, QZ};
+// the own cell's x/z products that are not in the y-queue wait 5 planes
+constexpr int DL[3] = {PX, PZ, PXZ};
 
 // Gathers at cell i of products P along one axis, P(k) = product at i + k:
 //   set A: sum_d c_(6-d) (P(-d) - P(d-1))     set B: sum_d c_(6-d) (P(1-d) - P(d))
@@ -131,9 +121,17 @@ __device__ __forceinline__ void lds_barrier() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
 }
 
-__global__ __launch_bounds__(ET) __attribute__((amdgpu_waves_per_eu(4))) void k_el3_adj(const El3Args a) {
-  __shared__ float xs[6][ELX][EZ];   // x-products: [k][x0-5+lx][z0+lz]
-  __shared__ float zs[6][EW][ELZ];   // z-products: [k][x0+lx][z0-5+lz]
+// EW waves per workgroup, one tile column (x) each: a 64 x EW tile.
+// Every input column is loaded once per plane by its owner, 5 planes ahead
+// (plane y+5): its y-products enter the queue and its other x/z products a
+// 5-plane delay line, so plane y's own cells reach LDS from registers; only
+// the x- and z-halo cells are loaded at plane y.
+template <int EW>
+__global__ __launch_bounds__(EZ * EW) __attribute__((amdgpu_waves_per_eu(4)))
+void k_el3_adj(const El3Args a) {
+  constexpr int ET = EZ * EW, ELX = EW + 2 * EH, NZH = 2 * EH * EW;
+  __shared__ float xs[6][ELX][EZ];   // x-band: [k][x0-5+lx][z0+lz]
+  __shared__ float zs[6][EW][ELZ];   // z-band: [k][x0+lx][z0-5+lz]
   const int b = a.xcd ? xcd_tile(blockIdx.x, a.nblk) : (int)blockIdx.x;
   const int bz = b % a.nbz;
   const int r = b / a.nbz;
@@ -141,90 +139,77 @@ __global__ __launch_bounds__(ET) __attribute__((amdgpu_waves_per_eu(4))) void k_
   const int yc = r / a.nbx;
   const int z0 = bz * EZ, x0 = bx * EW, y0 = yc * a.ych;
   const int y1 = min(y0 + a.ych, a.ny);
-  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int t = threadIdx.x;
+  const int lane = t & 63, w = t >> 6;
   const int gz = z0 + lane, gx = x0 + w;
   const bool own = gz < a.nz && gx < a.nx;
+  // the own column's products are needed (as LDS halo of valid cells) up to
+  // nz+5 / nx+5; beyond that the padded array ends
+  const bool ownp = gz < a.nz + EH && gx < a.nx + EH;
   const int sx = a.sx, sy = a.sy;
-  // padded index of interior (iz, ix, iy)
   auto at = [&](int iz, int ix, int iy) { return (iz + EH) + sx * (ix + EH) + sy * (iy + EH); };
 
-  // x-halo cells of this thread (columns x0-5..x0-1, x0+8..x0+12): h = t, t + 512
-  // z-halo cell of this thread (rows z0-5..z0-1, z0+64..z0+68): t < 80
-  const int t = threadIdx.x;
-
-  float q[6][11];
-  // prime planes y0-5 .. y0+4 into slots 0..9
+  float q[6][11];  // YQ products, slot j = plane y-5+j
+  float dl[3][6];  // DL products, slot j = plane y+j
+  // prime planes y0-5 .. y0+4
 #pragma unroll
   for (int j = 0; j < 10; ++j) {
-    float p[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
-    if (own) {
-      Cell c;
-      load_cell(a, at(gz, gx, y0 - 5 + j), c);
-      y_products(c, p);
-    }
+    float p[9] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    if (ownp) products(a, at(gz, gx, y0 - 5 + j), p);
 #pragma unroll
-    for (int k = 0; k < 6; ++k) q[k][j] = p[k];
+    for (int k = 0; k < 6; ++k) q[k][j] = p[YQ[k]];
+    if (j >= 5) {
+#pragma unroll
+      for (int k = 0; k < 3; ++k) dl[k][j - 5] = p[DL[k]];
+    }
   }
 
   for (int y = y0; y < y1; ++y) {
-    // queue slot 10: plane y+5
+    // own column, plane y+5
     {
-      float p[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
-      if (own) {
-        Cell c;
-        load_cell(a, at(gz, gx, y + 5), c);
-        y_products(c, p);
-      }
+      float p[9] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+      if (ownp) products(a, at(gz, gx, y + 5), p);
 #pragma unroll
-      for (int k = 0; k < 6; ++k) q[k][10] = p[k];
+      for (int k = 0; k < 6; ++k) q[k][10] = p[YQ[k]];
+#pragma unroll
+      for (int k = 0; k < 3; ++k) dl[k][5] = p[DL[k]];
     }
-    // own cell: x- and z-products (rows past nz+5 or columns past nx+5 are
-    // outside the padded array and stay zero; no valid output reads them)
+    // own cells of plane y into LDS, from the queue (slot 5) and the delay line (slot 0)
     {
-      float px[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f}, pz[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
-      if (gz < a.nz + EH && gx < a.nx + EH) {
-        Cell c;
-        load_cell(a, at(gz, gx, y), c);
-        x_products(c, px);
-        z_products(c, pz);
-      }
+      float c[9];
+#pragma unroll
+      for (int k = 0; k < 6; ++k) c[YQ[k]] = q[k][5];
+#pragma unroll
+      for (int k = 0; k < 3; ++k) c[DL[k]] = dl[k][0];
 #pragma unroll
       for (int k = 0; k < 6; ++k) {
-        xs[k][w + EH][lane] = px[k];
-        zs[k][w][lane + EH] = pz[k];
+        xs[k][w + EH][lane] = c[XB[k]];
+        zs[k][w][lane + EH] = c[ZB[k]];
       }
     }
-    // x-halo
+    // x-halo: columns x0-5..x0-1 and x0+EW..x0+EW+4, 64 rows
 #pragma unroll
     for (int h0 = 0; h0 < NXH; h0 += ET) {
       const int h = h0 + t;
       if (h < NXH) {
-        const int hc = h >> 6;                          // 0..9
-        const int lx = hc < EH ? hc : hc + EW;          // LDS column
+        const int hc = h >> 6;
+        const int lx = hc < EH ? hc : hc + EW;
         const int hx = x0 - EH + lx, hz = z0 + (h & 63);
-        float px[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
-        if (hz < a.nz + EH && hx < a.nx + EH) {
-          Cell c;
-          load_cell(a, at(hz, hx, y), c);
-          x_products(c, px);
-        }
+        float p[9] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+        if (hz < a.nz + EH && hx < a.nx + EH) products(a, at(hz, hx, y), p);
 #pragma unroll
-        for (int k = 0; k < 6; ++k) xs[k][lx][h & 63] = px[k];
+        for (int k = 0; k < 6; ++k) xs[k][lx][h & 63] = p[XB[k]];
       }
     }
-    // z-halo
+    // z-halo: rows z0-5..z0-1 and z0+64..z0+68 of the EW columns
     if (t < NZH) {
       const int rr = t % (2 * EH), cc = t / (2 * EH);
-      const int lz = rr < EH ? rr : rr + EZ;            // LDS row
+      const int lz = rr < EH ? rr : rr + EZ;
       const int hz = z0 - EH + lz, hx = x0 + cc;
-      float pz[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
-      if (hz < a.nz + EH && hx < a.nx + EH) {
-        Cell c;
-        load_cell(a, at(hz, hx, y), c);
-        z_products(c, pz);
-      }
+      float p[9] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+      if (hz < a.nz + EH && hx < a.nx + EH) products(a, at(hz, hx, y), p);
 #pragma unroll
-      for (int k = 0; k < 6; ++k) zs[k][cc][lz] = pz[k];
+      for (int k = 0; k < 6; ++k) zs[k][cc][lz] = p[ZB[k]];
     }
     lds_barrier();
 
@@ -234,16 +219,19 @@ __global__ __launch_bounds__(ET) __attribute__((amdgpu_waves_per_eu(4))) void k_
 #pragma unroll
       for (int f = 0; f < 9; ++f) o[f] = gl(a.out[f], gi) + gl(a.in1[f], gi);
       const int cx = w + EH, cz = lane + EH;
+      // XB / ZB / YQ slot of each product
       auto X = [&](int k) { return [&, k](int d) { return xs[k][cx + d][lane]; }; };
       auto Z = [&](int k) { return [&, k](int d) { return zs[k][w][cz + d]; }; };
       auto Y = [&](int k) { return [&, k](int d) { return q[k][5 + d]; }; };
-      o[VX] += gather_a(X(0)) + gather_b(Y(0)) + gather_b(Z(0));
-      o[VY] += gather_a(Y(1)) + gather_b(Z(1)) + gather_b(X(1));
+      // XB = {PX, PXY, PXZ, QX, QY, QZ}, ZB = {PXZ, PYZ, PZ, QZ, QX, QY},
+      // YQ = {PY, PXY, PYZ, QX, QY, QZ}
+      o[VX] += gather_a(X(0)) + gather_b(Y(1)) + gather_b(Z(0));
+      o[VY] += gather_a(Y(0)) + gather_b(Z(1)) + gather_b(X(1));
       o[VZ] += gather_a(Z(2)) + gather_b(Y(2)) + gather_b(X(2));
       o[SXX] += gather_b(X(3));
-      o[SYY] += gather_b(Y(3));
+      o[SYY] += gather_b(Y(4));
       o[SZZ] += gather_b(Z(3));
-      o[SXY] += gather_b(X(4)) + gather_b(Y(4));
+      o[SXY] += gather_b(X(4)) + gather_b(Y(3));
       o[SXZ] += gather_b(Z(4)) + gather_b(X(5));
       o[SYZ] += gather_b(Z(5)) + gather_b(Y(5));
 #pragma unroll
@@ -253,8 +241,18 @@ __global__ __launch_bounds__(ET) __attribute__((amdgpu_waves_per_eu(4))) void k_
     for (int k = 0; k < 6; ++k)
 #pragma unroll
       for (int j = 0; j < 10; ++j) q[k][j] = q[k][j + 1];
+#pragma unroll
+    for (int k = 0; k < 3; ++k)
+#pragma unroll
+      for (int j = 0; j < 5; ++j) dl[k][j] = dl[k][j + 1];
     lds_barrier();
   }
+}
+
+// tile width in x (= waves per workgroup): RTM_EL3_W = 8 or 16
+int default_el3_width() {
+  const char* s = std::getenv("RTM_EL3_W");
+  return (s && std::atoi(s) == 16) ? 16 : 8;
 }
 
 int default_el3_ychunk(int ny, int tiles) {
@@ -262,10 +260,10 @@ int default_el3_ychunk(int ny, int tiles) {
     const int v = std::atoi(s);
     if (v > 0) return std::min(v, ny);
   }
-  // about 4096 workgroups, at least 10 planes per chunk (a chunk primes its
-  // queue with 10 planes; measured best at 10-50 planes on 200^3 and 400^3)
-  const int chunks = std::max(1, (4096 + tiles - 1) / tiles);
-  return std::min(ny, std::max(10, (ny + chunks - 1) / chunks));
+  // about 2048 workgroups, 20 to 40 planes per chunk (a chunk primes its
+  // queue with 10 planes; 20-40 measured best on 200^3 and 400^3)
+  const int chunks = std::max(1, (2048 + tiles - 1) / tiles);
+  return std::min(ny, std::min(40, std::max(20, (ny + chunks - 1) / chunks)));
 }
 
 struct El3Timing {
@@ -297,8 +295,9 @@ static int el3_steps(int ny, int nz, int nx, float* in[9], float* in1[9], float*
   a.sx = nz + 10;
   a.sy = (nz + 10) * (nx + 10);
   for (int k = 0; k < 6; ++k) a.m[k] = m[k];
+  const int ew = default_el3_width();
   a.nbz = (nz + EZ - 1) / EZ;
-  a.nbx = (nx + EW - 1) / EW;
+  a.nbx = (nx + ew - 1) / ew;
   a.ych = default_el3_ychunk(ny, a.nbz * a.nbx);
   const int nyc = (ny + a.ych - 1) / a.ych;
   a.nblk = a.nbz * a.nbx * nyc;
@@ -317,7 +316,10 @@ static int el3_steps(int ny, int nz, int nx, float* in[9], float* in1[9], float*
       a.in1[f] = in1[f];
       a.out[f] = out[f];
     }
-    hipLaunchKernelGGL(k_el3_adj, dim3(a.nblk), dim3(ET), 0, st, a);
+    if (ew == 16)
+      hipLaunchKernelGGL(k_el3_adj<16>, dim3(a.nblk), dim3(EZ * 16), 0, st, a);
+    else
+      hipLaunchKernelGGL(k_el3_adj<8>, dim3(a.nblk), dim3(EZ * 8), 0, st, a);
     HIPCHK(hipGetLastError());
     if (s < nsteps - 1) {
       for (int f = 0; f < 9; ++f) {

@@ -46,10 +46,12 @@ def test_reference_shape_200_cubed_host_interface(hip):
     assert np.all(t > 0)
 
 
+@pytest.mark.parametrize("width", ["8", "16"])
 @pytest.mark.parametrize("shape", [(7, 5, 3), (13, 70, 11), (20, 129, 17), (9, 64, 8)])
-def test_odd_shapes_device_interface(hip, shape):
+def test_odd_shapes_device_interface(hip, shape, width, monkeypatch):
+    monkeypatch.setenv("RTM_EL3_W", width)
     ny, nz, nx = shape
-    print("elastic3d device interface", shape, flush=True)
+    print("elastic3d device interface", shape, "width", width, flush=True)
     fin, fin1, fout, coef = _data(ny, nz, nx, 7)
     dev = [hip.DeviceArray(a) for a in (fin, fin1, fout)]
     dc = hip.DeviceArray(coef)
@@ -98,17 +100,18 @@ def test_chunking_and_tile_order_do_not_change_results(hip):
     fin, fin1, fout, coef = _data(ny, nz, nx, 61)
     dc = hip.DeviceArray(coef)
     outs = []
-    saved = {k: os.environ.get(k) for k in ("RTM_EL3_YCHUNK", "RTM_XCD_REMAP")}
+    saved = {k: os.environ.get(k) for k in ("RTM_EL3_YCHUNK", "RTM_XCD_REMAP", "RTM_EL3_W")}
     try:
-        for ych, xcd in [(None, None), ("1", "1"), ("7", "0"), ("40", "1"), ("1000", "0")]:
-            for k, v in (("RTM_EL3_YCHUNK", ych), ("RTM_XCD_REMAP", xcd)):
+        for ych, xcd, ew in [(None, None, None), ("1", "1", "16"), ("7", "0", "8"), ("40", "1", "16"),
+                             ("1000", "0", "16")]:
+            for k, v in (("RTM_EL3_YCHUNK", ych), ("RTM_XCD_REMAP", xcd), ("RTM_EL3_W", ew)):
                 if v is None:
                     os.environ.pop(k, None)
                 else:
                     os.environ[k] = v
             dev = [hip.DeviceArray(a) for a in (fin, fin1, fout)]
             outs.append(hip.el3_steps_dev(dev, dc, 1)[2].get())
-            print("ychunk", ych, "xcd", xcd, "planes per chunk", hip.el3_timings()[3], flush=True)
+            print("ychunk", ych, "xcd", xcd, "width", ew, "planes per chunk", hip.el3_timings()[3], flush=True)
     finally:
         for k, v in saved.items():
             if v is None:
