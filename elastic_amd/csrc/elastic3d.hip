@@ -249,10 +249,11 @@ void k_el3_adj(const El3Args a) {
   }
 }
 
-// tile width in x (= waves per workgroup): RTM_EL3_W = 8 or 16
+// tile width in x (= waves per workgroup): RTM_EL3_W = 4, 8 or 16
 int default_el3_width() {
   const char* s = std::getenv("RTM_EL3_W");
-  return (s && std::atoi(s) == 16) ? 16 : 8;
+  const int v = s ? std::atoi(s) : 8;
+  return (v == 4 || v == 16) ? v : 8;
 }
 
 int default_el3_ychunk(int ny, int tiles) {
@@ -318,6 +319,8 @@ static int el3_steps(int ny, int nz, int nx, float* in[9], float* in1[9], float*
     }
     if (ew == 16)
       hipLaunchKernelGGL(k_el3_adj<16>, dim3(a.nblk), dim3(EZ * 16), 0, st, a);
+    else if (ew == 4)
+      hipLaunchKernelGGL(k_el3_adj<4>, dim3(a.nblk), dim3(EZ * 4), 0, st, a);
     else
       hipLaunchKernelGGL(k_el3_adj<8>, dim3(a.nblk), dim3(EZ * 8), 0, st, a);
     HIPCHK(hipGetLastError());

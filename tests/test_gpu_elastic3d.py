@@ -46,7 +46,7 @@ def test_reference_shape_200_cubed_host_interface(hip):
     assert np.all(t > 0)
 
 
-@pytest.mark.parametrize("width", ["8", "16"])
+@pytest.mark.parametrize("width", ["4", "8", "16"])
 @pytest.mark.parametrize("shape", [(7, 5, 3), (13, 70, 11), (20, 129, 17), (9, 64, 8)])
 def test_odd_shapes_device_interface(hip, shape, width, monkeypatch):
     monkeypatch.setenv("RTM_EL3_W", width)
@@ -103,7 +103,7 @@ def test_chunking_and_tile_order_do_not_change_results(hip):
     saved = {k: os.environ.get(k) for k in ("RTM_EL3_YCHUNK", "RTM_XCD_REMAP", "RTM_EL3_W")}
     try:
         for ych, xcd, ew in [(None, None, None), ("1", "1", "16"), ("7", "0", "8"), ("40", "1", "16"),
-                             ("1000", "0", "16")]:
+                             ("9", "1", "4"), ("1000", "0", "16")]:
             for k, v in (("RTM_EL3_YCHUNK", ych), ("RTM_XCD_REMAP", xcd), ("RTM_EL3_W", ew)):
                 if v is None:
                     os.environ.pop(k, None)
