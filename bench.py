@@ -348,10 +348,12 @@ def main():
     # copy kernel) times this run's cell-updates per launch
     # kernel form the batch runs (autotuned unless --txr / RTM_TXR / RTM_TBLOCK fix it)
     form, txr = b.tblock, b.txr
+    mw = b.march_w if form >= 2 else 0
     kname = ("k_" + dom) + {0: "", 1: "2", 2: "m", 3: ("m" if dom == "fwd" else "")}[form]
     cells_per_launch = (info["adj_cells"] / info["adj_launches"]) if dom == "adj" else \
         (info["fwd_cells"] / info["fwd_launches"])
-    tr = load_traffic(args.variant, kname)
+    # marching figures are measured per strip width (the 16-column ones unsuffixed)
+    tr = load_traffic(args.variant, kname + (f"_w{mw}" if kname.endswith("m") and mw not in (0, 16) else ""))
     traffic = round(tr["bytes_per_cell_update"] * cells_per_launch) if tr else None
     b.close()
 
@@ -392,7 +394,7 @@ def main():
                    "kernel_form": {0: "tiles, one level per launch", 1: "tiles, two levels per launch",
                                    2: "strips marching along x",
                                    3: "strips marching along x (forward), tiles (adjoint)"}[form],
-                   "tile_width": txr},
+                   "tile_width": txr, "strip_width": mw or None},
         "shots_per_hour": round(shots_per_hour, 1),
         "mcells_computational": round(comp / (ms_step * 1e-3) / 1e6, 1),
         "image_rel_l2_vs_fp64": max_err,

@@ -60,8 +60,9 @@ static void load_file_cache() {
   char key[512];
   int txr = 0;
   while (std::fscanf(f, "%511s %d", key, &txr) == 2) {
-    const int w = txr & 15, tb = txr >> 4;
-    if ((w == 1 || w == 2 || w == 4 || w == 8) && (tb == 0 || (tb == 1 && w <= 4) || (tb >= 2 && tb <= 3 && w == 2)))
+    const int w = txr & 15, tb = (txr >> 4) & 15, mw = txr >> 8;
+    if ((w == 1 || w == 2 || w == 4 || w == 8) && (tb == 0 || (tb == 1 && w <= 4) || (tb >= 2 && tb <= 3 && w == 2)) &&
+        mw >= 0 && (mw & 1) == 0)
       g_tune_cache[key] = txr;
   }
   std::fclose(f);
@@ -77,7 +78,8 @@ static void append_file_cache(const std::string& key, int txr) {
 }
 
 int autotune_txr(const EngineCfg& cfg, const std::vector<ShotSpec>& shots, const std::vector<const float*>& coef5,
-                 const float* wavelet, int* best, int* best_tblock, double ms_out[8], bool use_cache) {
+                 const float* wavelet, int* best, int* best_tblock, double ms_out[8], bool use_cache,
+                 int* best_march_w) {
   if (shots.empty() || coef5.size() != shots.size() || !wavelet || !best || !best_tblock) {
     set_error("autotune: bad arguments");
     return E_ARG;
@@ -89,7 +91,8 @@ int autotune_txr(const EngineCfg& cfg, const std::vector<ShotSpec>& shots, const
     auto it = g_tune_cache.find(key);
     if (use_cache && it != g_tune_cache.end()) {
       *best = it->second & 15;
-      *best_tblock = it->second >> 4;
+      *best_tblock = (it->second >> 4) & 15;
+      if (best_march_w && (it->second >> 8) > 0) *best_march_w = it->second >> 8;
       if (ms_out)
         for (int i = 0; i < 8; ++i) ms_out[i] = -1.0;
       return OK;
@@ -113,9 +116,14 @@ int autotune_txr(const EngineCfg& cfg, const std::vector<ShotSpec>& shots, const
   int cand[8][2] = {{1, 0}, {2, 0}, {4, 0}, {8, 0}, {1, 1}, {2, 1}, {4, 1}, {2, 2}};
   double t[8];
   int arg = -1;
+  // marching strip widths tried (RTM_MARCH_W fixes one)
+  const int mws[2] = {16, 32};
+  const int nmw = getenv("RTM_MARCH_W") ? 1 : 2;
+  int mw_run = cfg.march_w, mw_best = cfg.march_w;
   auto time_form = [&](int txr, int form, double* ms) -> int {
     c.txr = txr;
     c.tblock = form;
+    c.march_w = mw_run;
     Engine e;
     STCHK(e.setup(c, sp, cf, wavelet));
     STCHK(e.migrate());  // warm-up (and graph instantiation)
@@ -124,14 +132,32 @@ int autotune_txr(const EngineCfg& cfg, const std::vector<ShotSpec>& shots, const
     return OK;
   };
   for (int i = 0; i < 8; ++i) {
-    STCHK(time_form(cand[i][0], cand[i][1], &t[i]));
-    if (i == 7) {
-      double t3 = 0;
-      STCHK(time_form(2, 3, &t3));
-      if (t3 < t[7]) {
-        t[7] = t3;
-        cand[7][1] = 3;
+    if (i < 7) {
+      STCHK(time_form(cand[i][0], cand[i][1], &t[i]));
+    } else {
+      // marching: forms 2 and 3 at each strip width; the fastest time
+      // stands for the slot. Between widths the wider strip (fewer priming
+      // columns re-read per column, fewer HBM bytes) is kept unless the
+      // narrower one is more than 3% faster on the short run, the same rule
+      // as between the marching and the tile forms below.
+      t[7] = -1;
+      double tw[2] = {-1, -1};
+      int fw[2] = {2, 2};
+      for (int m = 0; m < nmw; ++m) {
+        mw_run = nmw == 1 ? cfg.march_w : mws[m];
+        for (int form = 2; form <= 3; ++form) {
+          double tm = 0;
+          STCHK(time_form(2, form, &tm));
+          if (tw[m] < 0 || tm < tw[m]) {
+            tw[m] = tm;
+            fw[m] = form;
+          }
+        }
+        if (t[7] < 0 || tw[m] < t[7]) t[7] = tw[m];
       }
+      const int mi = (nmw == 2 && tw[1] > 1.03 * tw[0]) ? 0 : nmw - 1;
+      cand[7][1] = fw[mi];
+      mw_best = nmw == 1 ? cfg.march_w : mws[mi];
     }
     if (arg < 0 || t[i] < t[arg]) arg = i;
   }
@@ -147,10 +173,12 @@ int autotune_txr(const EngineCfg& cfg, const std::vector<ShotSpec>& shots, const
   }
   *best = cand[arg][0];
   *best_tblock = cand[arg][1];
+  const int mw_out = cand[arg][1] >= 2 ? mw_best : 0;
+  if (best_march_w && mw_out > 0) *best_march_w = mw_out;
   if (ms_out)
     for (int i = 0; i < 8; ++i) ms_out[i] = t[i];
   std::lock_guard<std::mutex> lk(g_tune_mu);
-  const int code = *best | (*best_tblock << 4);
+  const int code = *best | (*best_tblock << 4) | (mw_out << 8);
   g_tune_cache[key] = code;
   append_file_cache(key, code);
   return OK;

@@ -306,7 +306,7 @@ struct rtm_ctx {
 
 struct rtm_batch {
   Engine eng;
-  int nshots = 0, nt = 0, txr = 0, tblock = 0;
+  int nshots = 0, nt = 0, txr = 0, tblock = 0, march_w = 0;
   std::vector<int> nz, nx, nrx;
 };
 
@@ -615,9 +615,10 @@ RTM_EXPORT int rtm_batch_create(rtm_batch** out, const rtm_batch_cfg_t* cfg, int
   }
   b->nshots = nshots;
   b->nt = cfg->nt;
-  if (tune) STCHK(autotune_txr(ec, sp, cf, wavelet, &ec.txr, &ec.tblock, nullptr, true));
+  if (tune) STCHK(autotune_txr(ec, sp, cf, wavelet, &ec.txr, &ec.tblock, nullptr, true, &ec.march_w));
   b->txr = ec.txr;
   b->tblock = ec.tblock;
+  b->march_w = ec.march_w;
   STCHK(b->eng.setup(ec, sp, cf, wavelet));
   *out = b.release();
   return OK;
@@ -648,6 +649,7 @@ RTM_EXPORT int rtm_batch_autotune(const rtm_batch_cfg_t* cfg, int nshots, const 
 
 RTM_EXPORT int rtm_batch_txr(rtm_batch* b) { return b ? b->txr : 0; }
 RTM_EXPORT int rtm_batch_tblock(rtm_batch* b) { return b ? b->tblock : 0; }
+RTM_EXPORT int rtm_batch_march_w(rtm_batch* b) { return b ? b->march_w : 0; }
 
 RTM_EXPORT int rtm_batch_set_true_model(rtm_batch* b, const float* const* coef5_true) {
   if (!b || !coef5_true) { set_error("bad argument"); return E_ARG; }

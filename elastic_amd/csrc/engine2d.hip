@@ -35,7 +35,7 @@ static int nt_stores() { return env_int("RTM_NT", 0); }
 // per launch, 2 one level per launch marching along x in strips of
 // RTM_MARCH_W columns; the batch autotuner chooses the form per workload.
 int default_tblock() { return env_int("RTM_TBLOCK", 0); }
-int default_march_w() { return env_int("RTM_MARCH_W", 16); }
+int default_march_w() { return env_int("RTM_MARCH_W", 32); }
 int default_march_nw() { return env_int("RTM_MARCH_NW", 2); }
 
 // Sponge taper, W/abc-mod.f90:35-50, and per-row/column factors of K

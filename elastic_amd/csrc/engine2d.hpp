@@ -35,7 +35,7 @@ struct EngineCfg {
                        // levels per launch (temporal blocking), 2 one level
                        // per launch marching along x (tile width 2 only), 3
                        // the marching forward with the tile adjoint
-  int march_w = 16;    // strip width (columns, even) of the marching form
+  int march_w = 32;    // strip width (columns, even) of the marching form
   int march_nw = 2;    // strip height of the marching form in waves (64 rows each): 1, 2, 4
   int max_group = 0;   // cap on shots per group (0 = memory bound only)
   int stack_nz = 0, stack_nx = 0;  // stacked image (0 = none)
@@ -136,11 +136,13 @@ int default_tblock();
 int default_march_w();
 int default_march_nw();
 // Tile autotuner (autotune.hip): times the engine on a short copy of the
-// workload for TXR 1, 2, 4, 8 with one level per launch and TXR 1, 2, 4 with
-// two, and returns the fastest in *best / *best_tblock (ms_out[tblock*4 +
-// log2 txr]: the times, -1 for a candidate not run or when the answer came
-// from the cache).
+// workload for TXR 1, 2, 4, 8 with one level per launch, TXR 1, 2, 4 with
+// two, and the marching forms (strips of 16 and 32 columns unless
+// RTM_MARCH_W fixes one), and returns the fastest in *best / *best_tblock
+// and, when a marching form wins, its strip width in *best_march_w (if not
+// null). ms_out: the 8 slots' times, -1 when the answer came from the cache.
 int autotune_txr(const EngineCfg& cfg, const std::vector<ShotSpec>& shots, const std::vector<const float*>& coef5,
-                 const float* wavelet, int* best, int* best_tblock, double ms_out[8], bool use_cache);
+                 const float* wavelet, int* best, int* best_tblock, double ms_out[8], bool use_cache,
+                 int* best_march_w = nullptr);
 
 }  // namespace erm

@@ -320,6 +320,11 @@ class Batch:
         3 the marching forward with the tile adjoint."""
         return int(load().rtm_batch_tblock(self._h))
 
+    @property
+    def march_w(self):
+        """Strip width (columns) of the marching forms."""
+        return int(load().rtm_batch_march_w(self._h))
+
     def info(self):
         out = np.zeros(8, np.float64)
         _chk(load().rtm_batch_info(self._h, _p(out)))

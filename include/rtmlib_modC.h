@@ -211,6 +211,9 @@ int rtm_batch_destroy(rtm_batch *b);
  * strips marching along x, 3: the marching forward with the tile adjoint). */
 int rtm_batch_txr(rtm_batch *b);
 int rtm_batch_tblock(rtm_batch *b);
+/* Strip width (columns) of the marching forms the batch runs with (chosen by
+ * the autotuner among 16 and 32 unless RTM_MARCH_W fixes it). */
+int rtm_batch_march_w(rtm_batch *b);
 /* Tile autotuner: runs a short copy of the workload (up to 8 of the shots,
  * max(48, 8*snap_every) levels, same options) with tile widths 1, 2, 4, 8 at
  * one level per launch, 1, 2, 4 at two levels per launch (temporal
