@@ -98,9 +98,11 @@ int autotune_txr(const EngineCfg& cfg, const std::vector<ShotSpec>& shots, const
       return OK;
     }
   }
-  // A short copy of the workload: up to 8 shots, enough levels for several
-  // snapshot periods, the same options; one group, no stack.
-  const int ns = (int)std::min<size_t>(shots.size(), 8);
+  // A short copy of the workload: the shots of the full run's first group
+  // (at most 8: how many strips or tiles one launch holds decides how the
+  // kernel forms fill the GPU), enough levels for several snapshot periods,
+  // the same options; one group, no stack.
+  const int ns = std::min<int>({(int)shots.size(), 8, first_group_size(cfg, shots)});
   std::vector<ShotSpec> sp(shots.begin(), shots.begin() + ns);
   std::vector<const float*> cf(coef5.begin(), coef5.begin() + ns);
   for (ShotSpec& s : sp) s.img_nz = s.img_nx = 0;

@@ -198,22 +198,49 @@ int Engine::setup(const EngineCfg& cfg, const std::vector<ShotSpec>& shots,
   return build_groups();
 }
 
-// Working set of one shot: the two-level ring (10 fields) followed by its
-// source-wavefield snapshots (and their per-strip scales).
+// Working set of one shot: the level ring (5 fields per level) followed by
+// its source-wavefield snapshots (and their per-strip scales).
+static void shot_work(const EngineCfg& c, long long n2, int nstrip, size_t* snap, size_t* scale, size_t* work) {
+  const long long K = c.nt / c.snap_every;
+  switch (c.snap_fmt) {
+    case SNAP_HIST: *snap = (size_t)2 * c.nt * n2 * sizeof(float); *scale = 0; break;
+    case SNAP_F32: *snap = (size_t)4 * K * n2 * sizeof(float); *scale = 0; break;
+    default:
+      *snap = (size_t)4 * K * n2 * sizeof(unsigned short);
+      *scale = (size_t)4 * K * nstrip * sizeof(float);
+  }
+  *snap = (*snap + 255) / 256 * 256;
+  *scale = (*scale + 255) / 256 * 256;
+  *work = (size_t)5 * kEngineRing * n2 * sizeof(float) + *snap + *scale;
+}
+
+int first_group_size(const EngineCfg& c, const std::vector<ShotSpec>& shots) {
+  size_t freeb = 0, totb = 0;
+  if (hipMemGetInfo(&freeb, &totb) != hipSuccess) return (int)shots.size();
+  // per-shot buffers allocated before the working arena
+  size_t fixed = 0;
+  for (const ShotSpec& p : shots) {
+    const long long n2 = (long long)device_ldz(p.nz) * p.nx;
+    fixed += (size_t)5 * n2 * sizeof(float) + (size_t)5 * c.nt * std::max(p.nrx, 1) * sizeof(float) +
+             (size_t)(c.comp ? 6 : 3) * n2 * sizeof(float) + (size_t)(p.nz + p.nx) * sizeof(float);
+  }
+  const size_t avail = fixed < freeb ? (size_t)(c.mem_fraction * (double)(freeb - fixed)) : 0;
+  size_t used = 0;
+  int g = 0;
+  for (const ShotSpec& p : shots) {
+    size_t snap, scale, work;
+    shot_work(c, (long long)device_ldz(p.nz) * p.nx, nstrips(p.nz, p.nx, c.txr), &snap, &scale, &work);
+    if (used + work > avail || (c.max_group > 0 && g >= c.max_group)) break;
+    used += work;
+    ++g;
+  }
+  return std::max(g, 1);
+}
+
 int Engine::alloc_working() {
-  const long long K = cfg_.nt / cfg_.snap_every;
   size_t largest = 0, total = 0;
   for (auto& r : res_) {
-    switch (cfg_.snap_fmt) {
-      case SNAP_HIST: r.snap_bytes = (size_t)2 * cfg_.nt * r.n2 * sizeof(float); r.scale_bytes = 0; break;
-      case SNAP_F32: r.snap_bytes = (size_t)4 * K * r.n2 * sizeof(float); r.scale_bytes = 0; break;
-      default:
-        r.snap_bytes = (size_t)4 * K * r.n2 * sizeof(unsigned short);
-        r.scale_bytes = (size_t)4 * K * r.nstrip * sizeof(float);
-    }
-    r.snap_bytes = (r.snap_bytes + 255) / 256 * 256;
-    r.scale_bytes = (r.scale_bytes + 255) / 256 * 256;
-    r.work_bytes = (size_t)5 * kRing * r.n2 * sizeof(float) + r.snap_bytes + r.scale_bytes;
+    shot_work(cfg_, r.n2, r.nstrip, &r.snap_bytes, &r.scale_bytes, &r.work_bytes);
     largest = std::max(largest, r.work_bytes);
     total += r.work_bytes;
   }

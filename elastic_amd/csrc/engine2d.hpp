@@ -55,6 +55,10 @@ struct Timing {
   double fwd_cells = 0, adj_cells = 0;  // cell-updates (computational grid)
 };
 
+// The level ring of a shot's working set holds four levels, so that a
+// two-level forward step writes levels other than the two it reads.
+constexpr int kEngineRing = 4;
+
 class Engine {
  public:
   ~Engine();
@@ -89,9 +93,7 @@ class Engine {
     DevBuf coef, coef_true, kfac, rec, img, imgc;
   };
   float* ring_of(int s) const { return reinterpret_cast<float*>(work_.as<char>() + res_[s].work_off); }
-  // The ring holds kRing levels (four, so that a two-level forward step
-  // writes levels other than the two it reads).
-  static constexpr int kRing = 4;
+  static constexpr int kRing = kEngineRing;
   char* snap_of(int s) const {
     return work_.as<char>() + res_[s].work_off + (size_t)5 * kRing * res_[s].n2 * sizeof(float);
   }
@@ -135,6 +137,10 @@ int default_xcd();
 int default_tblock();
 int default_march_w();
 int default_march_nw();
+// Shots in the first group a full run of cfg would form: those, in order,
+// whose working sets fit the arena the engine would allocate from the
+// memory free now.
+int first_group_size(const EngineCfg& cfg, const std::vector<ShotSpec>& shots);
 // Tile autotuner (autotune.hip): times the engine on a short copy of the
 // workload for TXR 1, 2, 4, 8 with one level per launch, TXR 1, 2, 4 with
 // two, and the marching forms (strips of 16 and 32 columns unless
