@@ -99,10 +99,14 @@ int autotune_txr(const EngineCfg& cfg, const std::vector<ShotSpec>& shots, const
     }
   }
   // A short copy of the workload: the shots of the full run's first group
-  // (at most 8: how many strips or tiles one launch holds decides how the
-  // kernel forms fill the GPU), enough levels for several snapshot periods,
-  // the same options; one group, no stack.
-  const int ns = std::min<int>({(int)shots.size(), 8, first_group_size(cfg, shots)});
+  // (at most RTM_TUNE_MAXSHOTS, default 24: how many strips or tiles one
+  // launch holds decides how the kernel forms fill the GPU; with 8 the
+  // 23-shot BF16 groups of the 64-shot survey were ranked wrongly), enough
+  // levels for several snapshot periods, the same options; one group, no
+  // stack.
+  const char* cap_env = getenv("RTM_TUNE_MAXSHOTS");
+  const int cap = std::max(1, cap_env ? std::atoi(cap_env) : 24);
+  const int ns = std::min<int>({(int)shots.size(), cap, first_group_size(cfg, shots)});
   std::vector<ShotSpec> sp(shots.begin(), shots.begin() + ns);
   std::vector<const float*> cf(coef5.begin(), coef5.begin() + ns);
   for (ShotSpec& s : sp) s.img_nz = s.img_nx = 0;
