@@ -123,8 +123,9 @@ int autotune_txr(const EngineCfg& cfg, const std::vector<ShotSpec>& shots, const
   double t[8];
   int arg = -1;
   // marching strip widths tried (RTM_MARCH_W fixes one)
-  const int mws[2] = {16, 32};
-  const int nmw = getenv("RTM_MARCH_W") ? 1 : 2;
+  constexpr int kMW = 4;
+  const int mws[kMW] = {16, 24, 28, 32};
+  const int nmw = getenv("RTM_MARCH_W") ? 1 : kMW;
   int mw_run = cfg.march_w, mw_best = cfg.march_w;
   auto time_form = [&](int txr, int form, double* ms) -> int {
     c.txr = txr;
@@ -141,14 +142,11 @@ int autotune_txr(const EngineCfg& cfg, const std::vector<ShotSpec>& shots, const
     if (i < 7) {
       STCHK(time_form(cand[i][0], cand[i][1], &t[i]));
     } else {
-      // marching: forms 2 and 3 at each strip width; the fastest time
-      // stands for the slot. Between widths the wider strip (fewer priming
-      // columns re-read per column, fewer HBM bytes) is kept unless the
-      // narrower one is more than 3% faster on the short run, the same rule
-      // as between the marching and the tile forms below.
+      // marching: forms 2 and 3 at each strip width; the fastest stands for
+      // the slot.
       t[7] = -1;
-      double tw[2] = {-1, -1};
-      int fw[2] = {2, 2};
+      double tw[kMW] = {-1, -1, -1, -1};
+      int fw[kMW] = {2, 2, 2, 2};
       for (int m = 0; m < nmw; ++m) {
         mw_run = nmw == 1 ? cfg.march_w : mws[m];
         for (int form = 2; form <= 3; ++form) {
@@ -161,7 +159,9 @@ int autotune_txr(const EngineCfg& cfg, const std::vector<ShotSpec>& shots, const
         }
         if (t[7] < 0 || tw[m] < t[7]) t[7] = tw[m];
       }
-      const int mi = (nmw == 2 && tw[1] > 1.03 * tw[0]) ? 0 : nmw - 1;
+      int mi = 0;
+      for (int m = 0; m < nmw; ++m)
+        if (tw[m] < tw[mi]) mi = m;
       cand[7][1] = fw[mi];
       mw_best = nmw == 1 ? cfg.march_w : mws[mi];
     }

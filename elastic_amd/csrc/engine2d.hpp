@@ -143,7 +143,7 @@ int default_march_nw();
 int first_group_size(const EngineCfg& cfg, const std::vector<ShotSpec>& shots);
 // Tile autotuner (autotune.hip): times the engine on a short copy of the
 // workload for TXR 1, 2, 4, 8 with one level per launch, TXR 1, 2, 4 with
-// two, and the marching forms (strips of 16 and 32 columns unless
+// two, and the marching forms (strips of 16, 24, 28 and 32 columns unless
 // RTM_MARCH_W fixes one), and returns the fastest in *best / *best_tblock
 // and, when a marching form wins, its strip width in *best_march_w (if not
 // null). ms_out: the 8 slots' times, -1 when the answer came from the cache.

@@ -212,13 +212,13 @@ int rtm_batch_destroy(rtm_batch *b);
 int rtm_batch_txr(rtm_batch *b);
 int rtm_batch_tblock(rtm_batch *b);
 /* Strip width (columns) of the marching forms the batch runs with (chosen by
- * the autotuner among 16 and 32 unless RTM_MARCH_W fixes it). */
+ * the autotuner among 16, 24, 28 and 32 unless RTM_MARCH_W fixes it). */
 int rtm_batch_march_w(rtm_batch *b);
 /* Tile autotuner: runs a short copy of the workload (the shots of the full
  * run's first group, at most 24 or RTM_TUNE_MAXSHOTS; max(48, 8*snap_every)
  * levels; same options) with tile widths 1, 2, 4, 8 at one level per
  * launch, 1, 2, 4 at two levels per launch (temporal blocking) and the
- * marching forms (strips of 16 and 32 columns), and returns the fastest in
+ * marching forms (strips of 16, 24, 28 and 32 columns), and returns the fastest in
  * *txr, *tblock;
  * ms8[tblock*4 + log2(txr)] receives the times of the first seven and ms8[7]
  * that of the faster marching form, 2 or 3 (ms, -1 for candidates not run).
