@@ -352,8 +352,17 @@ def main():
     kname = ("k_" + dom) + {0: "", 1: "2", 2: "m", 3: ("m" if dom == "fwd" else "")}[form]
     cells_per_launch = (info["adj_cells"] / info["adj_launches"]) if dom == "adj" else \
         (info["fwd_cells"] / info["fwd_launches"])
-    # marching figures are measured per strip width (the 16-column ones unsuffixed)
-    tr = load_traffic(args.variant, kname + (f"_w{mw}" if kname.endswith("m") and mw not in (0, 16) else ""))
+    # marching figures are measured per strip width (the 16-column ones
+    # unsuffixed); a width without its own figures takes the nearest measured
+    tr = None
+    if kname.endswith("m") and mw:
+        for w in sorted((16, 24, 28, 32), key=lambda w: abs(w - mw)):
+            tr = load_traffic(args.variant, kname + ("" if w == 16 else f"_w{w}"))
+            if tr:
+                tr = dict(tr, strip_width=w)
+                break
+    else:
+        tr = load_traffic(args.variant, kname)
     traffic = round(tr["bytes_per_cell_update"] * cells_per_launch) if tr else None
     b.close()
 
@@ -403,6 +412,7 @@ def main():
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": traffic,
                      "traffic_per_cell_update": tr["bytes_per_cell_update"] if traffic else None,
+                     "traffic_strip_width": tr.get("strip_width") if traffic else None,
                      "ms_per_launch": round(ms_launch, 5),
                      "cells_per_launch": round(cells_per_launch),
                      "algo_bytes_per_cell": fb if dom == "fwd" else abytes,

@@ -125,6 +125,8 @@ int autotune_txr(const EngineCfg& cfg, const std::vector<ShotSpec>& shots, const
   // marching strip widths tried (RTM_MARCH_W fixes one)
   constexpr int kMW = 4;
   const int mws[kMW] = {16, 24, 28, 32};
+  // (groups whose strips fit one round at some width get that width from
+  // the engine whatever the candidate; the candidates matter for the others)
   const int nmw = getenv("RTM_MARCH_W") ? 1 : kMW;
   int mw_run = cfg.march_w, mw_best = cfg.march_w;
   auto time_form = [&](int txr, int form, double* ms) -> int {

@@ -618,8 +618,8 @@ RTM_EXPORT int rtm_batch_create(rtm_batch** out, const rtm_batch_cfg_t* cfg, int
   if (tune) STCHK(autotune_txr(ec, sp, cf, wavelet, &ec.txr, &ec.tblock, nullptr, true, &ec.march_w));
   b->txr = ec.txr;
   b->tblock = ec.tblock;
-  b->march_w = ec.march_w;
   STCHK(b->eng.setup(ec, sp, cf, wavelet));
+  b->march_w = b->eng.first_march_w();
   *out = b.release();
   return OK;
 }

@@ -2176,6 +2176,29 @@ static void adjm_dispatch(int snap, dim3 g, hipStream_t st, const ShotDev* s, co
   }
 }
 
+template <int NW>
+static int adjm_occupancy(int snap) {
+  int n = 0;
+  const int b = TILE_Z * NW;
+  hipError_t e = hipSuccess;
+  switch (snap) {
+    case SNAP_HIST: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_adjm<SNAP_HIST, NW>, b, 0); break;
+    case SNAP_F32: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_adjm<SNAP_F32, NW>, b, 0); break;
+    case SNAP_BF16: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_adjm16<SNAP_BF16, NW>, b, 0); break;
+    default: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_adjm16<SNAP_F16, NW>, b, 0); break;
+  }
+  return e == hipSuccess ? n : 0;
+}
+
+int adjm_resident_blocks(int nw, int snap) {
+  int dev = 0, cus = 0;
+  if (hipGetDevice(&dev) != hipSuccess ||
+      hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+    return 0;
+  const int per = nw == 1 ? adjm_occupancy<1>(snap) : nw == 2 ? adjm_occupancy<2>(snap) : adjm_occupancy<4>(snap);
+  return per * cus;
+}
+
 static int march_args(int nstrips, int xw, int nw, int snap) {
   if (xw < 2 || (xw & 1)) { set_error("marching strip width %d is not a positive even number", xw); return E_ARG; }
   if (nw != 1 && nw != 2 && nw != 4) { set_error("marching strip height %d waves not in {1,2,4}", nw); return E_ARG; }

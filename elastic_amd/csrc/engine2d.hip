@@ -37,6 +37,9 @@ static int nt_stores() { return env_int("RTM_NT", 0); }
 int default_tblock() { return env_int("RTM_TBLOCK", 0); }
 int default_march_w() { return env_int("RTM_MARCH_W", 32); }
 int default_march_nw() { return env_int("RTM_MARCH_NW", 2); }
+// Per-group strip widths fitted to one round of resident workgroups, unless
+// RTM_MARCH_W fixes the width or RTM_MARCH_FIT=0.
+bool march_fit() { return !getenv("RTM_MARCH_W") && env_int("RTM_MARCH_FIT", 1) != 0; }
 
 // Sponge taper, W/abc-mod.f90:35-50, and per-row/column factors of K
 // (:59-78): rows first then columns, so corners are damped twice.
@@ -259,6 +262,8 @@ int Engine::alloc_working() {
 // (and at most max_group shots); one tile table spans all shots of a group.
 int Engine::build_groups() {
   groups_.clear();
+  march_fit_ = march_fit();
+  resident_ = march_fit_ ? adjm_resident_blocks(cfg_.march_nw, cfg_.snap_fmt) : 0;
   const int ns = (int)specs_.size();
   std::vector<TileRef> tiles, strips;
   const int tx = tile_x(cfg_.txr);
@@ -280,14 +285,35 @@ int Engine::build_groups() {
       const int nbz = (p.nz + TILE_Z - 1) / TILE_Z, nbx = (p.nx + tx - 1) / tx;
       for (int bx = 0; bx < nbx; ++bx)
         for (int bz = 0; bz < nbz; ++bz) tiles.push_back({slot, bz * TILE_Z, bx * tx, bz + nbz * bx});
-      if (march) {
-        const int sz = TILE_Z * cfg_.march_nw;
-        const int nsx = (p.nx + cfg_.march_w - 1) / cfg_.march_w, nsz = (p.nz + sz - 1) / sz;
-        for (int bx = 0; bx < nsx; ++bx)
-          for (int bz = 0; bz < nsz; ++bz) strips.push_back({slot, bz * sz, bx * cfg_.march_w, bz + nsz * bx});
-      }
       g.shots.push_back(s);
       ++s;
+    }
+    if (march) {
+      // strip width of this group: cfg_.march_w, or with fitting the
+      // narrowest even width from 16 whose strips all fit one round of
+      // resident marching-adjoint workgroups (a launch that spills a few
+      // workgroups into a second round costs most of a round)
+      const int sz = TILE_Z * cfg_.march_nw;
+      auto count = [&](int w) {
+        long long n = 0;
+        for (int si : g.shots) n += (long long)((specs_[si].nz + sz - 1) / sz) * ((specs_[si].nx + w - 1) / w);
+        return n;
+      };
+      int w = cfg_.march_w;
+      if (march_fit_ && resident_ > 0) {
+        for (int c = 16; c <= 64; c += 2)
+          if (count(c) <= resident_) {
+            w = c;
+            break;
+          }
+      }
+      g.march_w = w;
+      for (int slot = 0; slot < (int)g.shots.size(); ++slot) {
+        const ShotSpec& p = specs_[g.shots[slot]];
+        const int nsx = (p.nx + w - 1) / w, nsz = (p.nz + sz - 1) / sz;
+        for (int bx = 0; bx < nsx; ++bx)
+          for (int bz = 0; bz < nsz; ++bz) strips.push_back({slot, bz * sz, bx * w, bz + nsz * bx});
+      }
     }
     g.ntiles = (int)tiles.size() - g.tile0;
     g.nstrips = (int)strips.size() - g.strip0;
@@ -401,7 +427,7 @@ int Engine::enqueue_forward(int g, int mode, hipStream_t st) {
     if (two)
       STCHK(launch_fwd2(desc, tiles, gr.ntiles, L, fl, cfg_.txr, snap, cfg_.xcd, st));
     else if (march)
-      STCHK(launch_fwdm(desc, strips, gr.nstrips, L, fl, cfg_.march_w, cfg_.march_nw, snap, cfg_.xcd, st));
+      STCHK(launch_fwdm(desc, strips, gr.nstrips, L, fl, gr.march_w, cfg_.march_nw, snap, cfg_.xcd, st));
     else
       STCHK(launch_fwd(desc, tiles, gr.ntiles, L, fl, cfg_.txr, snap, cfg_.xcd, st));
     for (int Lm = L; Lm <= L + (two ? 1 : 0); ++Lm) {
@@ -450,7 +476,7 @@ int Engine::enqueue_adjoint(int g, hipStream_t st) {
     if (two)
       STCHK(launch_adj2(desc, tiles, gr.ntiles, L, fl, cfg_.txr, cfg_.snap_fmt, cfg_.xcd, st));
     else if (march)
-      STCHK(launch_adjm(desc, strips, gr.nstrips, L, fl, cfg_.march_w, cfg_.march_nw, cfg_.snap_fmt, cfg_.xcd, st));
+      STCHK(launch_adjm(desc, strips, gr.nstrips, L, fl, gr.march_w, cfg_.march_nw, cfg_.snap_fmt, cfg_.xcd, st));
     else
       STCHK(launch_adj(desc, tiles, gr.ntiles, L, fl, cfg_.txr, cfg_.snap_fmt, cfg_.xcd, st));
     L -= two ? 2 : 1;

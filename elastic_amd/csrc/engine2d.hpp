@@ -80,6 +80,11 @@ class Engine {
   int get_stack(float* img3);                    // 3*(stack_nz,stack_nx)
   int get_last_levels(int shot, float* last10);  // levels nt-1, nt of the last run
   const Timing& timing() const { return tm_; }
+  // strip width of the first group's marching launches (cfg march_w when
+  // the form does not march)
+  int first_march_w() const {
+    return (!groups_.empty() && groups_[0].march_w > 0) ? groups_[0].march_w : cfg_.march_w;
+  }
   int group_size() const { return G_; }
   int ngroups() const { return (int)groups_.size(); }
 
@@ -101,6 +106,7 @@ class Engine {
     std::vector<int> shots;
     int tile0 = 0, ntiles = 0;  // slice of the tile table
     int strip0 = 0, nstrips = 0;  // slice of the strip table (marching form)
+    int march_w = 0;              // strip width of this group (marching form)
     int desc0 = 0;              // first descriptor
     hipGraphExec_t gfwd = nullptr, gadj = nullptr, gsyn = nullptr;
   };
@@ -119,6 +125,8 @@ class Engine {
   std::vector<Res> res_;
   std::vector<Group> groups_;
   int G_ = 0;
+  bool march_fit_ = false;  // per-group strip widths (march_fit())
+  int resident_ = 0;        // resident marching-adjoint workgroups on the device
   long long n2max_ = 0;
   size_t arena_bytes_ = 0;
   DevBuf src_amp_, work_, d_desc_, d_tiles_, d_strips_, stack_, born_buf_;
@@ -137,6 +145,7 @@ int default_xcd();
 int default_tblock();
 int default_march_w();
 int default_march_nw();
+bool march_fit();
 // Shots in the first group a full run of cfg would form: those, in order,
 // whose working sets fit the arena the engine would allocate from the
 // memory free now.

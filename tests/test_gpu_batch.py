@@ -152,7 +152,8 @@ def test_tile_widths_agree_and_autotuner(hip, j):
 
 
 @pytest.mark.parametrize("prec,j,mw,nw", [("fp32", 1, 32, 4), ("fp32", 3, 6, 1), ("bf16", 4, 2, 2),
-                                          ("fp16", 2, 10, 4), ("bf16", 4, 16, 4)])
+                                          ("fp16", 2, 10, 4), ("bf16", 4, 16, 4), ("fp32", 1, None, 2),
+                                          ("bf16", 2, None, 1)])
 def test_marching_form_equals_tiles(hip, monkeypatch, prec, j, mw, nw):
     """The marching step kernels (a workgroup of nw waves per strip of
     64*nw rows x mw columns walking along x, neighbouring strips in opposite
@@ -162,12 +163,17 @@ def test_marching_form_equals_tiles(hip, monkeypatch, prec, j, mw, nw):
     waves of a 4-wave strip below the grid."""
     specs, ctrue, cmig, wav = _survey(hip)
     out = {}
-    monkeypatch.setenv("RTM_MARCH_W", str(mw))
+    if mw is None:  # strip widths fitted per group to the resident workgroups
+        monkeypatch.delenv("RTM_MARCH_W", raising=False)
+    else:
+        monkeypatch.setenv("RTM_MARCH_W", str(mw))
     monkeypatch.setenv("RTM_MARCH_NW", str(nw))
     for tb in ("0", "2", "3"):
         monkeypatch.setenv("RTM_TBLOCK", tb)
         b = _batch(hip, specs, ctrue, cmig, wav, txr=2, snap_every=j, snap_prec=prec, comp_image=int(prec != "fp32"))
         assert b.tblock == int(tb)
+        if tb != "0":
+            assert b.march_w >= 2 and b.march_w % 2 == 0 and (mw is None or b.march_w == mw)
         b.migrate()
         out[tb] = ([b.get_data(i) for i in range(len(specs))], [b.image(i) for i in range(len(specs))])
         b.close()
