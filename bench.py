@@ -356,7 +356,7 @@ def main():
     # unsuffixed); a width without its own figures takes the nearest measured
     tr = None
     if kname.endswith("m") and mw:
-        for w in sorted((16, 24, 28, 32), key=lambda w: abs(w - mw)):
+        for w in sorted((16, 24, 26, 28, 32), key=lambda w: abs(w - mw)):
             tr = load_traffic(args.variant, kname + ("" if w == 16 else f"_w{w}"))
             if tr:
                 tr = dict(tr, strip_width=w)
