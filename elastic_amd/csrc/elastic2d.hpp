@@ -122,6 +122,8 @@ int launch_adjm(const ShotDev* shots, const TileRef* strips, int nstrips, int L,
 // the current device at once: per-CU occupancy times the CU count (0 if
 // unknown).
 int adjm_resident_blocks(int nw, int snap_fmt);
+// The same for the marching forward (without Born sources).
+int fwdm_resident_blocks(int nw, int snap_fmt);
 // Snapshot of the last level (no later step writes it): dV = 0.
 int launch_snap_last(const ShotDev* shots, const TileRef* tiles, int ntiles, int L, int txr,
                      int snap_fmt, hipStream_t st);

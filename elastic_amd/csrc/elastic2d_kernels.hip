@@ -2190,13 +2190,36 @@ static int adjm_occupancy(int snap) {
   return e == hipSuccess ? n : 0;
 }
 
-int adjm_resident_blocks(int nw, int snap) {
+template <int NW>
+static int fwdm_occupancy(int snap) {
+  int n = 0;
+  const int b = TILE_Z * NW;
+  hipError_t e = hipSuccess;
+  switch (snap) {
+    case SNAP_HIST: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_fwdm<SNAP_HIST, false, NW>, b, 0); break;
+    case SNAP_F32: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_fwdm<SNAP_F32, false, NW>, b, 0); break;
+    case SNAP_BF16: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_fwdm<SNAP_BF16, false, NW>, b, 0); break;
+    default: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_fwdm<SNAP_F16, false, NW>, b, 0); break;
+  }
+  return e == hipSuccess ? n : 0;
+}
+
+static int device_cus() {
   int dev = 0, cus = 0;
   if (hipGetDevice(&dev) != hipSuccess ||
       hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
     return 0;
+  return cus;
+}
+
+int fwdm_resident_blocks(int nw, int snap) {
+  const int per = nw == 1 ? fwdm_occupancy<1>(snap) : nw == 2 ? fwdm_occupancy<2>(snap) : fwdm_occupancy<4>(snap);
+  return per * device_cus();
+}
+
+int adjm_resident_blocks(int nw, int snap) {
   const int per = nw == 1 ? adjm_occupancy<1>(snap) : nw == 2 ? adjm_occupancy<2>(snap) : adjm_occupancy<4>(snap);
-  return per * cus;
+  return per * device_cus();
 }
 
 static int march_args(int nstrips, int xw, int nw, int snap) {

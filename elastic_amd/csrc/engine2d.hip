@@ -264,6 +264,7 @@ int Engine::build_groups() {
   groups_.clear();
   march_fit_ = march_fit();
   resident_ = march_fit_ ? adjm_resident_blocks(cfg_.march_nw, cfg_.snap_fmt) : 0;
+  fresident_ = march_fit_ ? fwdm_resident_blocks(cfg_.march_nw, cfg_.snap_fmt) : 0;
   const int ns = (int)specs_.size();
   std::vector<TileRef> tiles, strips;
   const int tx = tile_x(cfg_.txr);
@@ -299,24 +300,36 @@ int Engine::build_groups() {
         for (int si : g.shots) n += (long long)((specs_[si].nz + sz - 1) / sz) * ((specs_[si].nx + w - 1) / w);
         return n;
       };
-      int w = cfg_.march_w;
-      if (march_fit_ && resident_ > 0) {
-        for (int c = 16; c <= 64; c += 2)
-          if (count(c) <= resident_) {
-            w = c;
-            break;
-          }
-      }
-      g.march_w = w;
-      for (int slot = 0; slot < (int)g.shots.size(); ++slot) {
-        const ShotSpec& p = specs_[g.shots[slot]];
-        const int nsx = (p.nx + w - 1) / w, nsz = (p.nz + sz - 1) / sz;
-        for (int bx = 0; bx < nsx; ++bx)
-          for (int bz = 0; bz < nsz; ++bz) strips.push_back({slot, bz * sz, bx * w, bz + nsz * bx});
+      auto fit = [&](int resident) {
+        if (march_fit_ && resident > 0)
+          for (int c = 16; c <= 64; c += 2)
+            if (count(c) <= resident) return c;
+        return cfg_.march_w;
+      };
+      auto add = [&](int w) {
+        for (int slot = 0; slot < (int)g.shots.size(); ++slot) {
+          const ShotSpec& p = specs_[g.shots[slot]];
+          const int nsx = (p.nx + w - 1) / w, nsz = (p.nz + sz - 1) / sz;
+          for (int bx = 0; bx < nsx; ++bx)
+            for (int bz = 0; bz < nsz; ++bz) strips.push_back({slot, bz * sz, bx * w, bz + nsz * bx});
+        }
+      };
+      // the adjoint's strips, then the forward's (the forward kernel holds
+      // more workgroups per CU, so its fitted strips may be narrower)
+      g.march_w = fit(resident_);
+      add(g.march_w);
+      g.nstrips = (int)strips.size() - g.strip0;
+      g.fmarch_w = fit(fresident_);
+      g.fstrip0 = g.strip0;
+      g.fnstrips = g.nstrips;
+      if (g.fmarch_w != g.march_w) {
+        g.fstrip0 = (int)strips.size();
+        add(g.fmarch_w);
+        g.fnstrips = (int)strips.size() - g.fstrip0;
       }
     }
     g.ntiles = (int)tiles.size() - g.tile0;
-    g.nstrips = (int)strips.size() - g.strip0;
+    if (!march) g.nstrips = 0;
     G_ = std::max(G_, (int)g.shots.size());
     groups_.push_back(g);
   }
@@ -391,7 +404,7 @@ int Engine::enqueue_forward(int g, int mode, hipStream_t st) {
   const ShotDev* desc = d_desc_.as<ShotDev>() + (size_t)variant * specs_.size() + gr.desc0;
   const TileRef* tiles = d_tiles_.as<TileRef>() + gr.tile0;
   const bool march = cfg_.tblock >= 2;
-  const TileRef* strips = march ? d_strips_.as<TileRef>() + gr.strip0 : nullptr;
+  const TileRef* strips = march ? d_strips_.as<TileRef>() + gr.fstrip0 : nullptr;
   const int snap = (mode == 0) ? cfg_.snap_fmt : SNAP_HIST;
   for (int s : gr.shots) {
     STCHK(launch_fill(ring_of(s), (size_t)5 * kRing * res_[s].n2, 0.f, st));
@@ -427,7 +440,7 @@ int Engine::enqueue_forward(int g, int mode, hipStream_t st) {
     if (two)
       STCHK(launch_fwd2(desc, tiles, gr.ntiles, L, fl, cfg_.txr, snap, cfg_.xcd, st));
     else if (march)
-      STCHK(launch_fwdm(desc, strips, gr.nstrips, L, fl, gr.march_w, cfg_.march_nw, snap, cfg_.xcd, st));
+      STCHK(launch_fwdm(desc, strips, gr.fnstrips, L, fl, gr.fmarch_w, cfg_.march_nw, snap, cfg_.xcd, st));
     else
       STCHK(launch_fwd(desc, tiles, gr.ntiles, L, fl, cfg_.txr, snap, cfg_.xcd, st));
     for (int Lm = L; Lm <= L + (two ? 1 : 0); ++Lm) {

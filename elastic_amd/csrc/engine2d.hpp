@@ -106,7 +106,9 @@ class Engine {
     std::vector<int> shots;
     int tile0 = 0, ntiles = 0;  // slice of the tile table
     int strip0 = 0, nstrips = 0;  // slice of the strip table (marching form)
-    int march_w = 0;              // strip width of this group (marching form)
+    int march_w = 0;              // strip width of this group (marching adjoint)
+    int fstrip0 = 0, fnstrips = 0;  // slice of the strip table for the marching forward
+    int fmarch_w = 0;               // its strip width
     int desc0 = 0;              // first descriptor
     hipGraphExec_t gfwd = nullptr, gadj = nullptr, gsyn = nullptr;
   };
@@ -127,6 +129,7 @@ class Engine {
   int G_ = 0;
   bool march_fit_ = false;  // per-group strip widths (march_fit())
   int resident_ = 0;        // resident marching-adjoint workgroups on the device
+  int fresident_ = 0;       // resident marching-forward workgroups
   long long n2max_ = 0;
   size_t arena_bytes_ = 0;
   DevBuf src_amp_, work_, d_desc_, d_tiles_, d_strips_, stack_, born_buf_;
