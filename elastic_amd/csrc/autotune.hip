@@ -4,15 +4,16 @@
 // z-cells, TXR in {1,2,4,8}). Which one is fastest depends on the grid sizes,
 // the snapshot format and the imaging decimation: wider tiles re-read less
 // halo but hold more registers per thread. The tuner runs the real engine on
-// a short copy of the workload (the first shots of the batch, a few dozen
-// levels, the same options) once per width and keeps the fastest. Results
-// are cached for the process, keyed by the workload signature, and, when
-// RTM_TUNE_CACHE names a file, in that file (one "key code" line per entry,
-// code = txr + 16 * form). The two-level (temporally blocked) kernels are
-// candidates too: they halve the HBM traffic of the fields but recompute a
-// radius-5 ring of the first level and run at lower occupancy; so is the
-// marching form (one wave per strip walking along x, halo columns read once
-// per strip). Which form wins is measured, not assumed.
+// a short copy of the workload (the shots of the full run's first group, a
+// few dozen levels, the same options) once per candidate and keeps the
+// fastest. Results are cached for the process, keyed by the workload
+// signature, and, when RTM_TUNE_CACHE names a file, in that file (one "key
+// code" line per entry, code = txr + 16 * form + 256 * strip width). The
+// two-level (temporally blocked) kernels are candidates too: they halve the
+// HBM traffic of the fields but recompute a radius-5 ring of the first level
+// and run at lower occupancy; so are the marching forms (workgroups per
+// strip walking along x, halo columns read once per strip) at strip widths
+// 16, 24, 28 and 32. Which form wins is measured, not assumed.
 #include "engine2d.hpp"
 
 #include <cstdio>

@@ -32,8 +32,10 @@ int default_xcd() { return env_int("RTM_XCD_REMAP", 1); }
 // Non-temporal stores of the new level (RTM_NT=1).
 static int nt_stores() { return env_int("RTM_NT", 0); }
 // Kernel form (RTM_TBLOCK): 0 one level per launch on tiles, 1 two levels
-// per launch, 2 one level per launch marching along x in strips of
-// RTM_MARCH_W columns; the batch autotuner chooses the form per workload.
+// per launch, 2 one level per launch marching along x in strips (3: the
+// marching forward with the tile adjoint); the batch autotuner chooses the
+// form per workload. RTM_MARCH_W fixes the strip width (default for groups
+// whose strips fit one round at no width).
 int default_tblock() { return env_int("RTM_TBLOCK", 0); }
 int default_march_w() { return env_int("RTM_MARCH_W", 32); }
 int default_march_nw() { return env_int("RTM_MARCH_NW", 2); }
