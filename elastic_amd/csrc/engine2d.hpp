@@ -80,10 +80,13 @@ class Engine {
   int get_stack(float* img3);                    // 3*(stack_nz,stack_nx)
   int get_last_levels(int shot, float* last10);  // levels nt-1, nt of the last run
   const Timing& timing() const { return tm_; }
-  // strip width of the first group's marching launches (cfg march_w when
-  // the form does not march)
+  // strip width of the first group's marching launches: the adjoint's for
+  // form 2, the forward's for form 3 (cfg march_w when the form does not
+  // march)
   int first_march_w() const {
-    return (!groups_.empty() && groups_[0].march_w > 0) ? groups_[0].march_w : cfg_.march_w;
+    if (groups_.empty()) return cfg_.march_w;
+    const int w = cfg_.tblock == 3 ? groups_[0].fmarch_w : groups_[0].march_w;
+    return w > 0 ? w : cfg_.march_w;
   }
   int group_size() const { return G_; }
   int ngroups() const { return (int)groups_.size(); }

@@ -211,11 +211,12 @@ int rtm_batch_destroy(rtm_batch *b);
  * strips marching along x, 3: the marching forward with the tile adjoint). */
 int rtm_batch_txr(rtm_batch *b);
 int rtm_batch_tblock(rtm_batch *b);
-/* Strip width (columns) of the first group's marching launches. A group's
- * width is the narrowest even width from 16 to 64 whose strips fit one round
- * of resident workgroups, else the autotuner's choice among 16, 24, 28 and
- * 32; RTM_MARCH_W fixes one width for all groups, RTM_MARCH_FIT=0 turns the
- * fitting off. */
+/* Strip width (columns) of the first group's marching launches (the
+ * adjoint's for form 2, the forward's for form 3). A group's
+ * width, per kernel, is the narrowest even width from 16 to 64 whose strips
+ * fit one round of that kernel's resident workgroups, else the autotuner's
+ * choice among 16, 24, 28 and 32; RTM_MARCH_W fixes one width for all
+ * groups, RTM_MARCH_FIT=0 turns the fitting off. */
 int rtm_batch_march_w(rtm_batch *b);
 /* Tile autotuner: runs a short copy of the workload (the shots of the full
  * run's first group, at most 24 or RTM_TUNE_MAXSHOTS; max(48, 8*snap_every)
