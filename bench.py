@@ -16,9 +16,21 @@ before the timed region.
 
 Prints ONE JSON line. ``value`` is Mcell-updates/s on the physical grid
 (forward + adjoint cell-updates of every shot per second); shots/hour and the
-computational-grid rate are reported beside it. The run is valid only when the
-images of a 200 x 400 sub-run through the same engine are within 1e-3
-relative L2 of the FP64 oracle; that check runs in the same process.
+computational-grid rate are reported beside it. The run is valid only when
+(a) shot 0 of the timed batch, run with the batch's own kernel form and strip
+widths, equals the same shot run alone on the tile kernels bit for bit, and
+(b) the images of a 200 x 400 sub-run on those tile kernels are within 1e-3
+relative L2 of the FP64 oracle; both checks run in the same process.
+
+After the timed steps the same survey and observed data run once through the
+plain FP32 GPU form (one shot per launch, no graphs) and once through the
+BF16 form (snapshots every 4th level with per-strip scales, compensated
+images): ``variants`` reports their shots/hour, the ratios, and the stacked
+images' relative L2 against the batched FP32 run.
+
+Other workloads (``--workload``): acoustic3d (3D acoustic subdomain),
+elastic3d (3D elastic 9-field adjoint step), probe (the library's streaming
+kernel: attainable HBM rate and counter calibration).
 """
 from __future__ import annotations
 
@@ -84,9 +96,33 @@ def build_batch(lib, synth, nz, nx, nshots, nt, dx, dt, fp, abc, variant, seed, 
     return b, specs, cmig, wav, layout
 
 
-def validate(lib, synth, variant, seed):
+class env_pinned:
+    """Temporarily set RTM_* environment variables (the library reads them
+    at each call)."""
+
+    def __init__(self, **kv):
+        self.kv = {k: str(v) for k, v in kv.items() if v is not None}
+        self.old = {}
+
+    def __enter__(self):
+        for k, v in self.kv.items():
+            self.old[k] = os.environ.get(k)
+            os.environ[k] = v
+        return self
+
+    def __exit__(self, *exc):
+        for k, v in self.old.items():
+            if v is None:
+                os.environ.pop(k, None)
+            else:
+                os.environ[k] = v
+
+
+def validate(lib, synth, variant, seed, txr):
     """Sub-run on a 200 x 400 model (2 shots, 1000 levels) through the same
-    engine configuration, images against the FP64 oracle."""
+    engine options, images against the FP64 oracle. It runs the tile kernels
+    at the timed batch's tile width; shot_matches_single() ties the timed
+    batch's own kernel form to that path bit for bit."""
     import oracle as O
 
     nz, nx, nt, dx, dt, fp, abc = 200, 400, 1000, 10.0, 8e-4, 15.0, 40
@@ -99,12 +135,12 @@ def validate(lib, synth, variant, seed):
         cmig.append(cm)
     wav = lib.ricker(nt, dt, fp)
     opts = VARIANTS[variant]
-    b = lib.Batch(nt, dx, dt, specs, cmig, wav, abc=abc, adj_abc=1, stack_shape=(nz, nx), **opts)
+    b = lib.Batch(nt, dx, dt, specs, cmig, wav, abc=abc, adj_abc=1, stack_shape=(nz, nx), txr=txr, **opts)
     b.set_true_model(ctrue)
     b.synthesize()
     b.migrate()
     errs = []
-    nth = min(16, os.cpu_count() or 1)
+    nth = cpu_threads()
     for i, sp in enumerate(specs):
         rec = b.get_data(i)
         img = b.image(i)
@@ -116,6 +152,65 @@ def validate(lib, synth, variant, seed):
         errs.append([O.rel_l2(a[sl], r[sl]) for a, r in zip(img, ref)])
     b.close()
     return errs
+
+
+def shot_matches_single(lib, b, specs, cmig, wav, nt, dx, dt, abc, variant, k=0):
+    """Shot k of a timed batch (its kernel form and strip widths) against the
+    same shot run alone through the single-shot entry point on the tile
+    kernels of the batch's tile width: the images must agree bit for bit."""
+    v = VARIANTS[variant]
+    sp = specs[k]
+    rec = b.get_data(k)
+    img_b = b.image(k)
+    s = lib.shot(sp["nz"], sp["nx"], nt, dx, dt, sp["xs"], sp["zs"], sp["zr"], sp["rx0"], sp["nrx"], abc, adj_abc=1,
+                 snap_every=v["snap_every"], snap_prec=lib.SNAP_PREC[v["snap_prec"]], comp_image=v["comp_image"])
+    with env_pinned(RTM_TBLOCK=0, RTM_TXR=b.txr, RTM_GRAPH=0):
+        img_s = lib.migrate(s, cmig[k], wav, rec)
+    return all(np.array_equal(a, c) for a, c in zip(img_b, img_s)) and all(np.isfinite(a).all() for a in img_b)
+
+
+def cpu_threads():
+    """Threads of the CPU baseline: every CPU this process may run on, within
+    the CPU share the job is granted (cgroup quota; at most 16 on the
+    one-GPU hosts this runs on)."""
+    return host_info()["threads"]
+
+
+_HOST = None
+
+
+def host_info():
+    global _HOST
+    if _HOST is None:
+        import platform
+        import subprocess
+
+        aff = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+        quota = None
+        try:
+            q, per = open("/sys/fs/cgroup/cpu.max").read().split()[:2]
+            if q != "max":
+                quota = int(q) / int(per)
+        except (OSError, ValueError):
+            pass
+        model = platform.processor()
+        try:
+            model = next(l.split(":", 1)[1].strip() for l in open("/proc/cpuinfo") if l.startswith("model name"))
+        except (OSError, StopIteration):
+            pass
+        try:
+            gxx = subprocess.run(["g++", "--version"], capture_output=True, text=True).stdout.splitlines()[0]
+        except (OSError, IndexError):
+            gxx = "unknown"
+        share = int(os.environ.get("OMP_NUM_THREADS", "0") or 0)
+        threads = aff
+        if quota:
+            threads = min(threads, max(1, int(quota)))
+        if share > 0:
+            threads = min(threads, share)
+        _HOST = dict(nproc=os.cpu_count(), affinity=aff, cgroup_cpus=quota, omp_num_threads=share or None,
+                     cpu_model=model, compiler=gxx, threads=threads)
+    return _HOST
 
 
 def cpu_baseline(lib, synth, seed, dx, dt, fp, abc, nt_sample, nthreads):
@@ -267,6 +362,59 @@ def elastic3d(args):
     print(json.dumps(out), flush=True)
 
 
+def probe(args):
+    """Streaming copy / read / write kernel of the library over a buffer of
+    --probe-bytes, timed with HIP events: the attainable HBM rate, and the
+    kernel of known traffic that calibrates the FETCH_SIZE / WRITE_SIZE
+    counters (run it under rocprofv3 --pmc). Prints its own JSON line."""
+    from elastic_amd import lib
+
+    modes = {0: "copy", 1: "copy, non-temporal loads and stores", 2: "read only", 3: "write only"}
+    gbs, ms = lib.bandwidth_probe(args.probe_bytes, args.probe_reps, args.probe_mode)
+    moved = args.probe_bytes * (2 if args.probe_mode in (0, 1) else 1)
+    out = {"metric": "streaming kernel bandwidth (bytes read + written per second)", "value": round(gbs, 1),
+           "unit": "GB/s", "n_gpus": 1, "steps": args.probe_reps, "ms_per_step": round(ms, 4),
+           "higher_is_better": True, "dtype": "fp32", "data": "synthetic",
+           "config": {"workload": f"{modes[args.probe_mode]} of {args.probe_bytes} bytes",
+                      "bytes_per_launch": moved},
+           "roofline": {"bound": "hbm", "kernel": "k_copy", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS,
+                        "unit": "GB/s", "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": moved}}
+    print(json.dumps(out), flush=True)
+
+
+def run_variant(lib, synth, args, variant, steps, warmup, data=None, graph=True):
+    """Build the survey's batch for one variant, put the observed data in
+    place (given, or modelled on the GPU), run warmup + timed migrations.
+    Returns (batch, specs, cmig, wav, seconds per migration, info)."""
+    dx, dt, fp, abc = 10.0, 8e-4, 15.0, 40
+    t0 = time.perf_counter()
+    b, specs, cmig, wav, _ = build_batch(lib, synth, args.nz, args.nx, args.shots, args.nt, dx, dt, fp, abc,
+                                         variant, args.seed, args.txr, graph=graph)
+    log(f"bench: {variant}: setup {time.perf_counter() - t0:.1f}s (form {b.tblock}, tile width {b.txr}, "
+        f"strip width {b.march_w})")
+    t1 = time.perf_counter()
+    if data is None:
+        b.synthesize()
+        log(f"bench: {variant}: data synthesis on the GPU {time.perf_counter() - t1:.1f}s")
+    else:
+        for i, d in enumerate(data):
+            b.set_data(i, d)
+        log(f"bench: {variant}: data upload {time.perf_counter() - t1:.1f}s")
+    for w in range(warmup):
+        t1 = time.perf_counter()
+        b.migrate()
+        log(f"bench: {variant}: warmup {w} {time.perf_counter() - t1:.2f}s")
+    lib.synchronize()
+    ts = time.perf_counter()
+    info = None
+    for k in range(steps):
+        b.migrate()
+        info = b.info()
+        log(f"bench: {variant}: step {k} done")
+    lib.synchronize()
+    return b, specs, cmig, wav, (time.perf_counter() - ts) / max(steps, 1), info
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -281,17 +429,23 @@ def main():
     ap.add_argument("--seed", type=int, default=20260102)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-validate", action="store_true")
-    ap.add_argument("--cpu-steps", type=int, default=600)
+    ap.add_argument("--no-compare", action="store_true",
+                    help="skip the plain-FP32 and BF16 runs of the same survey (one timed migration each)")
+    ap.add_argument("--cpu-steps", type=int, default=400)
     ap.add_argument("--no-graph", action="store_true",
                     help="launch every level from the host instead of replaying HIP graphs (same kernels; "
                          "for tracing tools that do not follow graph launches)")
-    ap.add_argument("--workload", default="elastic2d", choices=("elastic2d", "acoustic3d", "elastic3d"),
+    ap.add_argument("--workload", default="elastic2d", choices=("elastic2d", "acoustic3d", "elastic3d", "probe"),
                     help="acoustic3d: the 3D acoustic subdomain; elastic3d: the 9-field 3D elastic adjoint "
-                         "step; both instead of the headline 2D workload")
+                         "step; probe: the streaming-kernel bandwidth probe; all instead of the headline 2D "
+                         "workload")
     ap.add_argument("--n3d", type=int, default=512)
     ap.add_argument("--nt3d", type=int, default=1500)
     ap.add_argument("--n3e", type=int, default=200)
     ap.add_argument("--nt3e", type=int, default=500)
+    ap.add_argument("--probe-bytes", type=int, default=1 << 30)
+    ap.add_argument("--probe-reps", type=int, default=3)
+    ap.add_argument("--probe-mode", type=int, default=0, choices=(0, 1, 2, 3))
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -308,30 +462,14 @@ def main():
     if args.workload == "elastic3d":
         elastic3d(args)
         return
+    if args.workload == "probe":
+        probe(args)
+        return
     dx, dt, fp, abc = 10.0, 8e-4, 15.0, 40
-    t0 = time.perf_counter()
-    log(f"bench: building {args.shots}-shot survey {args.nz}x{args.nx}, nt={args.nt}, variant={args.variant}")
-    b, specs, cmig, wav, layout = build_batch(lib, synth, args.nz, args.nx, args.shots, args.nt, dx, dt, fp, abc,
-                                              args.variant, args.seed, args.txr, graph=not args.no_graph)
-    log(f"bench: setup {time.perf_counter() - t0:.1f}s; synthesizing observed data on the GPU")
-    t1 = time.perf_counter()
-    b.synthesize()
-    log(f"bench: data synthesis {time.perf_counter() - t1:.1f}s")
-    for w in range(args.warmup):
-        t1 = time.perf_counter()
-        b.migrate()
-        log(f"bench: warmup {w} {time.perf_counter() - t1:.2f}s")
-    lib.synchronize()
-    ts = time.perf_counter()
-    infos = []
-    for k in range(args.steps):
-        b.migrate()
-        infos.append(b.info())
-        log(f"bench: step {k} done")
-    lib.synchronize()
-    elapsed = time.perf_counter() - ts
-    ms_step = elapsed * 1e3 / args.steps
-    info = infos[-1]
+    log(f"bench: {args.shots}-shot survey {args.nz}x{args.nx}, nt={args.nt}, variant={args.variant}")
+    b, specs, cmig, wav, sec_step, info = run_variant(lib, synth, args, args.variant, args.steps, args.warmup,
+                                                      graph=not args.no_graph)
+    ms_step = sec_step * 1e3
     phys = sum(2.0 * (s["img_nz"] * s["img_nx"]) * args.nt for s in specs)
     comp = info["fwd_cells"] + info["adj_cells"]
     value = phys / (ms_step * 1e-3) / 1e6
@@ -343,45 +481,84 @@ def main():
     achieved = adj_gbs if dom == "adj" else fwd_gbs
     launches = info["adj_launches"] if dom == "adj" else info["fwd_launches"]
     ms_launch = (info["adj_ms"] if dom == "adj" else info["fwd_ms"]) / launches
-    # HBM bytes per launch of the dominant kernel: measured bytes per
-    # cell-update (rocprofv3 FETCH_SIZE / WRITE_SIZE passes, calibrated on a
-    # copy kernel) times this run's cell-updates per launch
     # kernel form the batch runs (autotuned unless --txr / RTM_TXR / RTM_TBLOCK fix it)
     form, txr = b.tblock, b.txr
     mw = b.march_w if form >= 2 else 0
     kname = ("k_" + dom) + {0: "", 1: "2", 2: "m", 3: ("m" if dom == "fwd" else "")}[form]
     cells_per_launch = (info["adj_cells"] / info["adj_launches"]) if dom == "adj" else \
         (info["fwd_cells"] / info["fwd_launches"])
-    # marching figures are measured per strip width (the 16-column ones
-    # unsuffixed); a width without its own figures takes the nearest measured
-    tr = None
-    if kname.endswith("m") and mw:
-        for w in sorted((16, 24, 26, 28, 32), key=lambda w: abs(w - mw)):
-            tr = load_traffic(args.variant, kname + ("" if w == 16 else f"_w{w}"))
-            if tr:
-                tr = dict(tr, strip_width=w)
-                break
-    else:
-        tr = load_traffic(args.variant, kname)
+    # HBM bytes per launch of the dominant kernel: measured bytes per
+    # cell-update (rocprofv3 FETCH_SIZE / WRITE_SIZE passes, calibrated on a
+    # copy kernel) times this run's cell-updates per launch; marching figures
+    # are kept per strip width, and a width without figures of its own
+    # reports null
+    tr = load_traffic(args.variant, kname + (f"_w{mw}" if kname.endswith("m") and mw else ""))
     traffic = round(tr["bytes_per_cell_update"] * cells_per_launch) if tr else None
+    same_as_single = None
+    stack_fp32 = None
+    data = None
+    if not args.no_validate:
+        same_as_single = shot_matches_single(lib, b, specs, cmig, wav, args.nt, dx, dt, abc, args.variant)
+        log(f"bench: shot 0 of the timed batch equals the single-shot tile path bit for bit: {same_as_single}")
+    if not args.no_compare and args.variant == "batched":
+        stack_fp32 = b.stack((args.nz, args.nx))
+        t1 = time.perf_counter()
+        data = [b.get_data(i) for i in range(len(specs))]
+        log(f"bench: observed data to host {time.perf_counter() - t1:.1f}s")
     b.close()
 
     img_errs = None
     if not args.no_validate:
         t1 = time.perf_counter()
-        img_errs = validate(lib, synth, args.variant, args.seed)
+        img_errs = validate(lib, synth, args.variant, args.seed, txr)
         log(f"bench: validation {time.perf_counter() - t1:.1f}s, image rel-L2 vs FP64 {img_errs}")
+
+    compare = None
+    if data is not None:
+        # the same survey and observed data through the plain FP32 GPU form
+        # (one shot per launch, no graphs) and through the BF16 form
+        # (snapshots every 4th level, compensated images), one timed
+        # migration each
+        import oracle as O
+
+        compare = {}
+        for v in ("plain", "bf16"):
+            # the plain form replays no graph: the autotuner's short run has
+            # loaded its kernels, so it needs no warm-up migration
+            bv, _, _, _, sv, _ = run_variant(lib, synth, args, v, 1, 0 if v == "plain" else 1, data=data)
+            st = bv.stack((args.nz, args.nx))
+            ent = {"shots_per_hour": round(3600.0 * args.shots / sv, 1), "seconds": round(sv, 2),
+                   "mcell_updates_per_s": round(phys / sv / 1e6, 1),
+                   "stack_rel_l2_vs_batched_fp32": [float(f"{O.rel_l2(a, r):.3e}") for a, r in zip(st, stack_fp32)],
+                   "kernel_form": bv.tblock, "tile_width": bv.txr}
+            if not args.no_validate:
+                ent["single_shot_bitwise"] = shot_matches_single(lib, bv, specs, cmig, wav, args.nt, dx, dt, abc, v)
+                if v == "bf16":
+                    ev = validate(lib, synth, v, args.seed, bv.txr)
+                    ent["subrun_rel_l2_vs_fp64"] = [[float(f"{x:.3e}") for x in e] for e in ev]
+            bv.close()
+            compare[v] = ent
+            log(f"bench: {v}: {ent}")
+        del data
+        compare["batched"] = {"shots_per_hour": round(shots_per_hour, 1), "seconds": round(sec_step, 2)}
+        compare["bf16_over_plain"] = round(compare["bf16"]["shots_per_hour"] / compare["plain"]["shots_per_hour"], 3)
+        compare["batched_over_plain"] = round(shots_per_hour / compare["plain"]["shots_per_hour"], 3)
+
     cpu = None
     if not args.no_cpu:
-        nth = min(16, os.cpu_count() or 1)
+        hi = host_info()
+        nth = hi["threads"]
         t1 = time.perf_counter()
         mc, secs, _ = cpu_baseline(lib, synth, args.seed, dx, dt, fp, abc, args.cpu_steps, nth)
         mc1, secs1, _ = cpu_baseline(lib, synth, args.seed, dx, dt, fp, abc, max(4, args.cpu_steps // 4), 1)
         log(f"bench: cpu baseline {time.perf_counter() - t1:.1f}s")
         cpu = {"value": round(mc, 3), "unit": "Mcell-updates/s", "cores": nth, "kind": "port",
-               "sample": f"oracle FP32 RTM (forward+adjoint+imaging) of one 800x900 shot (890x990 padded), "
-                         f"{args.cpu_steps} levels, {nth} threads, {secs:.1f}s",
-               "single_thread": {"value": round(mc1, 3), "cores": 1, "seconds": round(secs1, 2)}}
+               "sample": f"oracle FP32 RTM (forward+adjoint+imaging, reference operation order, -O3, no FMA, "
+                         f"OpenMP over x) of one 800x900 shot (890x990 padded), {args.cpu_steps} levels, "
+                         f"{nth} threads, {secs:.1f}s",
+               "single_thread": {"value": round(mc1, 3), "cores": 1, "seconds": round(secs1, 2),
+                                 "levels": max(4, args.cpu_steps // 4)},
+               "host": hi}
     max_err = max(max(e) for e in img_errs) if img_errs else None
     out = {
         "metric": "Mcell-updates/s & shots/hour, 2D elastic RTM at <=1e-3 image rel-L2; % HBM BW",
@@ -407,12 +584,13 @@ def main():
         "shots_per_hour": round(shots_per_hour, 1),
         "mcells_computational": round(comp / (ms_step * 1e-3) / 1e6, 1),
         "image_rel_l2_vs_fp64": max_err,
-        "valid": (max_err is not None and max_err <= 1e-3),
+        "timed_shot_equals_validated_path": same_as_single,
+        "valid": bool(max_err is not None and max_err <= 1e-3 and same_as_single),
+        "variants": compare,
         "roofline": {"bound": "hbm", "kernel": kname, "achieved": round(achieved, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": traffic,
                      "traffic_per_cell_update": tr["bytes_per_cell_update"] if traffic else None,
-                     "traffic_strip_width": tr.get("strip_width") if traffic else None,
                      "ms_per_launch": round(ms_launch, 5),
                      "cells_per_launch": round(cells_per_launch),
                      "algo_bytes_per_cell": fb if dom == "fwd" else abytes,

@@ -5,6 +5,7 @@ this file so that it travels with the repository snapshot.
 """
 from __future__ import annotations
 
+import concurrent.futures
 import glob
 import os
 import shutil
@@ -29,14 +30,18 @@ def build(verbose: bool = False) -> str:
     srcs = sorted(glob.glob(os.path.join(CSRC, "*.hip")))
     objdir = os.path.join(HERE, "_build")
     os.makedirs(objdir, exist_ok=True)
-    objs = []
-    for src in srcs:
-        obj = os.path.join(objdir, os.path.basename(src) + ".o")
-        cmd = [HIPCC, *HIP_FLAGS, "-c", src, "-o", obj]
-        if verbose:
-            print(" ".join(cmd), flush=True)
-        subprocess.run(cmd, check=True)
-        objs.append(obj)
+    objs = [os.path.join(objdir, os.path.basename(src) + ".o") for src in srcs]
+    cmds = [[HIPCC, *HIP_FLAGS, "-c", src, "-o", obj] for src, obj in zip(srcs, objs)]
+    # one hipcc per source, in parallel (each is single-threaded)
+    jobs = max(1, min(len(cmds), int(os.environ.get("MAX_JOBS", os.cpu_count() or 1))))
+    with concurrent.futures.ThreadPoolExecutor(jobs) as pool:
+        for cmd in cmds:
+            if verbose:
+                print(" ".join(cmd), flush=True)
+        for r in pool.map(lambda c: subprocess.run(c, capture_output=True, text=True), cmds):
+            sys.stderr.write(r.stderr)
+            if r.returncode != 0:
+                raise subprocess.CalledProcessError(r.returncode, r.args, r.stdout, r.stderr)
     tmp = LIB + ".tmp"
     cmd = [HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp, *objs]
     if verbose:

@@ -52,6 +52,13 @@ static int op3(int nt, int nz, int nx, float* const* hist, const float* const* c
   }
   for (int f = 0; f < 5; ++f)
     if (!hist[f] || !coefs[f]) { set_error("null array argument"); return E_ARG; }
+  if (5LL * nz * nx >= (1LL << 31)) {
+    // cells inside a level and the five coefficient arrays are addressed with
+    // 32-bit offsets; levels and fields with 64-bit ones (a history field
+    // holds nt*nz*nx elements, past 2^31 at 390 x 690 x 2000)
+    set_error("grid %d x %d: 5*nz*nx must stay below 2^31", nz, nx);
+    return E_ARG;
+  }
   if (nt < 3) return OK;  // no level below nt-2 to update
   STCHK(ensure_device());
   // The caller's layout, unpadded: each field's history moves in one

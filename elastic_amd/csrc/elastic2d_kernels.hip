@@ -151,8 +151,10 @@ __global__ __launch_bounds__(256) void k_fwd(const ShotDev* __restrict__ shots,
   const TileRef tr = tiles[xcd ? xcd_tile(blockIdx.x, ntiles) : (int)blockIdx.x];
   const ShotDev& S = shots[tr.shot];
   const int nz = S.nz, nx = S.nx, ldz = S.ldz;
-  // element offsets of one shot fit in 32 bits (5 fields x n2 < 2^31)
-  const int fs = (int)S.fstride, n2 = (int)S.n2;
+  // offsets inside one field are 32-bit (5 * n2 < 2^31, checked by the
+  // hosts); field offsets are 64-bit: a drop-in history field is nt * n2
+  const long long fs = S.fstride;
+  const int n2 = (int)S.n2;
   float* const out = lev_ptr(S.lev, S.lev_stride, S.lev_mod, S.lev_off, L);
   const float* const x2 = lev_ptr(S.lev, S.lev_stride, S.lev_mod, S.lev_off, L - 2);
   const float* const y1 = lev_ptr(S.lev, S.lev_stride, S.lev_mod, S.lev_off, L - 1);
@@ -384,7 +386,8 @@ __global__ __launch_bounds__(256) void k_fwd2(const ShotDev* __restrict__ shots,
   const TileRef tr = tiles[xcd ? xcd_tile(blockIdx.x, ntiles) : (int)blockIdx.x];
   const ShotDev& S = shots[tr.shot];
   const int nz = S.nz, nx = S.nx, ldz = S.ldz;
-  const int fs = (int)S.fstride, n2 = (int)S.n2;
+  const long long fs = S.fstride;
+  const int n2 = (int)S.n2;
   float* const out0 = lev_ptr(S.lev, S.lev_stride, S.lev_mod, S.lev_off, L);
   float* const out1 = lev_ptr(S.lev, S.lev_stride, S.lev_mod, S.lev_off, L + 1);
   const float* const x2 = lev_ptr(S.lev, S.lev_stride, S.lev_mod, S.lev_off, L - 2);
@@ -405,7 +408,7 @@ __global__ __launch_bounds__(256) void k_fwd2(const ShotDev* __restrict__ shots,
     const bool in = c < T::N1 && gx >= 0 && gx < nx && gz >= 0 && gz < nz;
     const int gi = gz + ldz * gx;
 #pragma unroll
-    for (int f = 0; f < 5; ++f) st[k][f] = in ? y1[f * fs + gi] : 0.f;
+    for (int f = 0; f < 5; ++f) st[k][f] = in ? (y1 + f * fs)[gi] : 0.f;
   }
   float ex[NIT0][5], ec[NIT0][5];
 #pragma unroll
@@ -417,7 +420,7 @@ __global__ __launch_bounds__(256) void k_fwd2(const ShotDev* __restrict__ shots,
     const int gi = gz + ldz * gx;
 #pragma unroll
     for (int f = 0; f < 5; ++f) {
-      ex[k][f] = in ? x2[f * fs + gi] : 0.f;
+      ex[k][f] = in ? (x2 + f * fs)[gi] : 0.f;
       ec[k][f] = in ? coef[f * n2 + gi] : 0.f;
     }
   }
@@ -560,8 +563,8 @@ __global__ __launch_bounds__(256) void k_fwd2(const ShotDev* __restrict__ shots,
     } else {
 #pragma unroll
       for (int f = 0; f < 5; ++f) {
-        out0[f * fs + gi] = cur[f];
-        out1[f * fs + gi] = v[f];
+        (out0 + f * fs)[gi] = cur[f];
+        (out1 + f * fs)[gi] = v[f];
       }
     }
     if ((flags & (F_REC | F_RECSUB)) && gz == S.zr && gx >= S.rx0 && gx < S.rx0 + S.nrx) {
@@ -774,7 +777,8 @@ __global__ __launch_bounds__(256) void k_adj(const ShotDev* __restrict__ shots,
   const TileRef tr = tiles[xcd ? xcd_tile(blockIdx.x, ntiles) : (int)blockIdx.x];
   const ShotDev& S = shots[tr.shot];
   const int nz = S.nz, nx = S.nx, ldz = S.ldz;
-  const int fs = (int)S.fstride, n2 = (int)S.n2;
+  const long long fs = S.fstride;
+  const int n2 = (int)S.n2;
   float* const out = lev_ptr(S.lev, S.lev_stride, S.lev_mod, S.lev_off, L);
   const float* const x2 = lev_ptr(S.lev, S.lev_stride, S.lev_mod, S.lev_off, L + 2);
   const float* const y1 = lev_ptr(S.lev, S.lev_stride, S.lev_mod, S.lev_off, L + 1);
@@ -1118,7 +1122,8 @@ __global__ __launch_bounds__(256) void k_adj2(const ShotDev* __restrict__ shots,
   const TileRef tr = tiles[xcd ? xcd_tile(blockIdx.x, ntiles) : (int)blockIdx.x];
   const ShotDev& S = shots[tr.shot];
   const int nz = S.nz, nx = S.nx, ldz = S.ldz;
-  const int fs = (int)S.fstride, n2 = (int)S.n2;
+  const long long fs = S.fstride;
+  const int n2 = (int)S.n2;
   float* const out0 = lev_ptr(S.lev, S.lev_stride, S.lev_mod, S.lev_off, L);
   float* const out1 = lev_ptr(S.lev, S.lev_stride, S.lev_mod, S.lev_off, L - 1);
   const float* const x2 = lev_ptr(S.lev, S.lev_stride, S.lev_mod, S.lev_off, L + 2);
@@ -1141,8 +1146,8 @@ __global__ __launch_bounds__(256) void k_adj2(const ShotDev* __restrict__ shots,
     float p0 = 0.f, p1 = 0.f, p2 = 0.f, p3 = 0.f, p4 = 0.f;
     if (in) {
       const int gi = gz + ldz * gx;
-      const float vx = y1[gi], vz = y1[fs + gi], xx = y1[2 * fs + gi], zz = y1[3 * fs + gi],
-                  xz = y1[4 * fs + gi];
+      const float vx = y1[gi], vz = (y1 + fs)[gi], xx = (y1 + 2 * fs)[gi], zz = (y1 + 3 * fs)[gi],
+                  xz = (y1 + 4 * fs)[gi];
       const float m1x = coef[gi], m1z = coef[n2 + gi], m2 = coef[2 * n2 + gi],
                   m3 = coef[3 * n2 + gi], m2m3 = coef[4 * n2 + gi];
       p0 = in ? fmaf(m2m3, xx, m2 * zz) : 0.f;
@@ -1168,7 +1173,7 @@ __global__ __launch_bounds__(256) void k_adj2(const ShotDev* __restrict__ shots,
     const int gi = gz + ldz * gx;
 #pragma unroll
     for (int f = 0; f < 5; ++f) {
-      eb[k][f] = in ? x2[f * fs + gi] : 0.f;
+      eb[k][f] = in ? (x2 + f * fs)[gi] : 0.f;
       ec[k][f] = in ? coef[f * n2 + gi] : 0.f;
     }
     if (in && inj && gz == S.zr && gx >= S.rx0 && gx < S.rx0 + S.nrx) {
@@ -1187,7 +1192,7 @@ __global__ __launch_bounds__(256) void k_adj2(const ShotDev* __restrict__ shots,
     const bool ok = zok && gx < nx;
     const int gi = gz + ldz * gx;
 #pragma unroll
-    for (int f = 0; f < 5; ++f) b1[j][f] = ok ? y1[f * fs + gi] : 0.f;
+    for (int f = 0; f < 5; ++f) b1[j][f] = ok ? (y1 + f * fs)[gi] : 0.f;
     if (ok && inj && gz == S.zr && gx >= S.rx0 && gx < S.rx0 + S.nrx) {
       const float* r = S.rec + (long long)(L - 2) * S.nrx + (gx - S.rx0);
 #pragma unroll
@@ -1299,8 +1304,8 @@ __global__ __launch_bounds__(256) void k_adj2(const ShotDev* __restrict__ shots,
     } else {
 #pragma unroll
       for (int f = 0; f < 5; ++f) {
-        out0[f * fs + gi] = oa[j][f];
-        out1[f * fs + gi] = v[f];
+        (out0 + f * fs)[gi] = oa[j][f];
+        (out1 + f * fs)[gi] = v[f];
       }
     }
 #pragma unroll
@@ -1489,7 +1494,8 @@ template <int SNAP, bool BORN, int DIR, int NW>
 __device__ __forceinline__ void fwdm_strip(const ShotDev& S, const TileRef& tr, int L, int flags, int xw,
                                            float (*col)[4][MarchCol<NW>::MZ]) {
   const int nz = S.nz, nx = S.nx, ldz = S.ldz;
-  const int fs = (int)S.fstride, n2 = (int)S.n2;
+  const long long fs = S.fstride;
+  const int n2 = (int)S.n2;
   float* const out = lev_ptr(S.lev, S.lev_stride, S.lev_mod, S.lev_off, L);
   const float* const x2 = lev_ptr(S.lev, S.lev_stride, S.lev_mod, S.lev_off, L - 2);
   const float* const y1 = lev_ptr(S.lev, S.lev_stride, S.lev_mod, S.lev_off, L - 1);
@@ -1694,7 +1700,8 @@ template <int SNAP, int DIR, int NW>
 __device__ __forceinline__ void adjm_strip(const ShotDev& S, const TileRef& tr, int L, int flags, int xw,
                                            float (*col)[6][MarchCol<NW>::MZ]) {
   const int nz = S.nz, nx = S.nx, ldz = S.ldz;
-  const int fs = (int)S.fstride, n2 = (int)S.n2;
+  const long long fs = S.fstride;
+  const int n2 = (int)S.n2;
   float* const out = lev_ptr(S.lev, S.lev_stride, S.lev_mod, S.lev_off, L);
   const float* const x2 = lev_ptr(S.lev, S.lev_stride, S.lev_mod, S.lev_off, L + 2);
   const float* const y1 = lev_ptr(S.lev, S.lev_stride, S.lev_mod, S.lev_off, L + 1);

@@ -171,6 +171,12 @@ int Engine::setup(const EngineCfg& cfg, const std::vector<ShotSpec>& shots,
     Res& r = res_[s];
     r.ldz = device_ldz(p.nz);
     r.n2 = (long long)r.ldz * p.nx;
+    if (5 * r.n2 >= (1LL << 31)) {
+      // the step kernels address cells inside the five coefficient arrays
+      // with 32-bit offsets
+      set_error("shot %zu: %lld cells per field (5 coefficient arrays) exceed 2^31 / 5", s, r.n2);
+      return E_ARG;
+    }
     STCHK(r.coef.alloc(5 * r.n2 * sizeof(float), "coefficients"));
     STCHK(fill_masked_coefs(coef5[s], p.nz, p.nx, r.ldz, r.coef.as<float>()));
     if (cfg_.abc > 0) {

@@ -87,10 +87,18 @@ def shot(nz, nx, nt, dx, dt, xs, zs, zr, rx0=0, nrx=None, abc=0, adj_abc=0, snap
     return RtmShot(nz, nx, nt, dx, dt, xs, zs, zr, rx0, nrx, abc, adj_abc, snap_every, snap_prec, comp_image)
 
 
-def rtm_op3(hist, coefs, zrec=0):
-    """Drop-in adjoint rtm_op3_c_ semantics (status-returning form) on copies."""
+def rtm_op3(hist, coefs, zrec=0, inplace=False):
+    """Drop-in adjoint rtm_op3_c_ semantics (status-returning form), on
+    copies of the histories, or in place on Fortran-ordered float32 arrays
+    (views along the level axis included) with inplace=True."""
     lib = load()
-    h = [_f32F(a).copy(order="F") for a in hist]
+    if inplace:
+        for a in hist:
+            if a.dtype != np.float32 or not a.flags.f_contiguous:
+                raise ValueError("in-place histories must be Fortran-contiguous float32")
+        h = list(hist)
+    else:
+        h = [_f32F(a).copy(order="F") for a in hist]
     c = [_f32F(a) for a in coefs]
     nz, nx, nt = h[0].shape
     _chk(lib.rtm_op3_hip(C.c_int(nt), C.c_int(nz), C.c_int(nx), C.c_int(zrec), *[_p(a) for a in h], *[_p(a) for a in c]))

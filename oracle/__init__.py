@@ -230,6 +230,21 @@ def image_S(vx, vz, vx1, vz1, dx, dt, dtype=np.float64):
     return S
 
 
+def image_adj_level(adj5, S6, w=1.0, dm=None, dtype=np.float64):
+    """One level of the Lame imaging condition, adjoint branch
+    (W/image-mod-rtm.f90:154-162), accumulated into dm = (drho, dlambda,
+    dmu) (zeros when None) with weight w. adj5: the five adjoint fields of
+    the level; S6: S1..S6 of the level. Returns the three updated arrays."""
+    a = [F(x, dtype) for x in adj5]
+    nz, nx = a[0].shape
+    S = np.concatenate([F(x, dtype).ravel(order="F") for x in S6])
+    out = [np.zeros((nz, nx), dtype, order="F") if dm is None else np.array(F(x, dtype), order="F", copy=True)
+           for x in (dm if dm is not None else range(3))]
+    getattr(lib(), "orc_image_adj_level_" + _suffix(dtype))(
+        C.c_int64(nz * nx), *[_p(x) for x in a], _p(S), _scal(dtype)(w), *[_p(x) for x in out])
+    return out
+
+
 def rtm(p: Params, wavelet, coefs, rec, dtype=np.float64):
     """Full single-shot RTM -> (drho, dlambda, dmu), each (nz, nx) Fortran order."""
     c = [F(a, dtype) for a in coefs]

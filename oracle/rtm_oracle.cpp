@@ -778,6 +778,13 @@ static void born(const Params& p, const T* wavelet, const T* m1_x, const T* m1_z
                                     T* S6) {                                                       \
     orc::image_S<T>(nz, nx, dx, dt, vx, vz, vx1, vz1, S1, S2, S3, S4, S5, S6);                     \
   }                                                                                                \
+  ORC_EXPORT void orc_image_adj_level_##SFX(int64_t n2, const T* vx, const T* vz, const T* xx,      \
+                                            const T* zz, const T* xz, const T* S, T w, T* dm1,     \
+                                            T* dm2, T* dm3) {                                      \
+    const T* a[5] = {vx, vz, xx, zz, xz};                                                          \
+    orc::image_adj_level<T>(n2, a, S, S + n2, S + 2 * n2, S + 3 * n2, S + 4 * n2, S + 5 * n2, w,   \
+                            dm1, dm2, dm3);                                                        \
+  }                                                                                                \
   ORC_EXPORT void orc_rtm_##SFX(const orc::Params* p, const T* wav, const T* m1x, const T* m1z,    \
                                 const T* m2, const T* m3, const T* m2m3, const T* rec, T* dm1,     \
                                 T* dm2, T* dm3) {                                                  \

@@ -43,3 +43,19 @@ def test_rtm_op3_c_abi_signature_matches_reference():
     assert len(args) == 14
     assert all(a.startswith("int ") for a in args[:4])
     assert all(a.startswith("float *") for a in args[4:])
+
+
+def test_rtm_op3_rejects_grids_past_32_bit_cell_offsets():
+    """Cells inside a level and inside the five coefficient arrays are
+    addressed with 32-bit offsets; the entry point refuses 5*nz*nx >= 2^31
+    with RTM_E_ARG before it touches a device (no compute call). Level and
+    field offsets are 64-bit: the config-0 drop-in history 390 x 690 x 2000
+    puts sigma_xz at element 4*nt*nz*nx, past 2^31, and is accepted."""
+    assert 4 * 2000 * 390 * 690 >= 2 ** 31 > 5 * 390 * 690
+    so = lib.load()
+    dummy = (ctypes.c_float * 4)()
+    ptrs = [ctypes.cast(dummy, ctypes.c_void_p)] * 10
+    nz, nx = 20000, 21475  # 5*nz*nx = 2,147,500,000 >= 2^31
+    rc = so.rtm_op3_hip(ctypes.c_int(8), ctypes.c_int(nz), ctypes.c_int(nx), ctypes.c_int(0), *ptrs)
+    assert rc == -1
+    assert "2^31" in so.rtm_hip_last_error().decode()

@@ -64,6 +64,54 @@ def test_rtm_op3_void_symbol_and_config0_grid(hip):
         assert _max_level_err(out[f], ref[f]) <= WAVE_TOL, f
 
 
+def _big_history(nz, nx, nt, zrec, seed):
+    """Seeded drop-in input: complete top two levels, receiver rows below."""
+    rng = np.random.default_rng(seed)
+    hist = []
+    for _ in range(5):
+        h = np.zeros((nz, nx, nt), np.float32, order="F")
+        h[:, :, nt - 2:] = rng.uniform(-1, 1, (nz, nx, 2))
+        h[zrec, :, : nt - 2] = rng.uniform(-1e-2, 1e-2, (nx, nt - 2))
+        hist.append(h)
+    return hist
+
+
+@pytest.mark.parametrize("form", ["tiles", "marching"])
+def test_rtm_op3_history_past_2_31_elements(hip, monkeypatch, form):
+    """Config-0 padded grid 390 x 690 with its 2000 levels: the sigma_xz
+    history starts at element 4*nt*nz*nx = 2,152,800,000 > 2^31, so field
+    offsets must be 64-bit. The top 12 levels are checked against the FP64
+    oracle run on the top 14 levels (the recursion of a level reads only the
+    two above it); the whole chain against the same history advanced in two
+    chunks whose offsets stay below 2^31 (levels 998..1999, then 0..999),
+    which must agree bit for bit."""
+    if form == "marching":
+        monkeypatch.setenv("RTM_TBLOCK", "2")
+        monkeypatch.setenv("RTM_MARCH_W", "26")
+    else:
+        monkeypatch.setenv("RTM_TBLOCK", "0")
+    c = synth.config0()
+    nz, nx, nt, zrec = c["geo"].nz, c["geo"].nx, c["nt"], c["geo"].zr
+    assert (nz, nx, nt) == (390, 690, 2000) and 4 * nt * nz * nx >= 2 ** 31
+    m = c["mig"]
+    coefs = [np.asfortranarray(a) for a in O.model_op(m.rho, m.lam, m.mu, c["dx"], c["dt"], np.float32)]
+    print(f"op3 {form}: {nz}x{nx}x{nt}, one call", flush=True)
+    full = _big_history(nz, nx, nt, zrec, 11)
+    top_in = [h[:, :, nt - 14:].copy(order="F") for h in full]  # a copy: full is updated in place
+    hip.rtm_op3(full, coefs, zrec, inplace=True)
+    ref = O.rtm_op3_c(top_in, coefs, "gather", np.float64, nthreads=8)
+    for f in range(5):
+        assert _max_level_err(full[f][:, :, nt - 14:], ref[f]) <= WAVE_TOL, f
+    del top_in, ref
+    print(f"op3 {form}: two chunks", flush=True)
+    chunk = _big_history(nz, nx, nt, zrec, 11)
+    hip.rtm_op3([h[:, :, 998:] for h in chunk], coefs, zrec, inplace=True)
+    hip.rtm_op3([h[:, :, :1000] for h in chunk], coefs, zrec, inplace=True)
+    for f in range(5):
+        assert np.array_equal(full[f], chunk[f]), f
+        assert np.isfinite(full[f][:, :, 0]).all() and np.abs(full[f][:, :, 0]).max() > 0
+
+
 def test_rtm_op3_trivial_nt(hip):
     hist, coefs = _adjoint_inputs(1, 20, 20, 2)
     out = hip.rtm_op3(hist, coefs)
