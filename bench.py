@@ -154,17 +154,18 @@ def validate(lib, synth, variant, seed, txr):
     return errs
 
 
-def shot_matches_single(lib, b, specs, cmig, wav, nt, dx, dt, abc, variant, k=0):
-    """Shot k of a timed batch (its kernel form and strip widths) against the
-    same shot run alone through the single-shot entry point on the tile
-    kernels of the batch's tile width: the images must agree bit for bit."""
+def shot_matches_single(lib, sample, txr, specs, cmig, wav, nt, dx, dt, abc, variant, k=0):
+    """Shot k of a timed batch (its kernel form and strip widths; sample =
+    its observed data and images, taken before the batch released its device
+    memory) against the same shot run alone through the single-shot entry
+    point on the tile kernels of the batch's tile width txr: the images must
+    agree bit for bit."""
     v = VARIANTS[variant]
     sp = specs[k]
-    rec = b.get_data(k)
-    img_b = b.image(k)
+    rec, img_b = sample
     s = lib.shot(sp["nz"], sp["nx"], nt, dx, dt, sp["xs"], sp["zs"], sp["zr"], sp["rx0"], sp["nrx"], abc, adj_abc=1,
                  snap_every=v["snap_every"], snap_prec=lib.SNAP_PREC[v["snap_prec"]], comp_image=v["comp_image"])
-    with env_pinned(RTM_TBLOCK=0, RTM_TXR=b.txr, RTM_GRAPH=0):
+    with env_pinned(RTM_TBLOCK=0, RTM_TXR=txr, RTM_GRAPH=0):
         img_s = lib.migrate(s, cmig[k], wav, rec)
     return all(np.array_equal(a, c) for a, c in zip(img_b, img_s)) and all(np.isfinite(a).all() for a in img_b)
 
@@ -497,15 +498,16 @@ def main():
     same_as_single = None
     stack_fp32 = None
     data = None
-    if not args.no_validate:
-        same_as_single = shot_matches_single(lib, b, specs, cmig, wav, args.nt, dx, dt, abc, args.variant)
-        log(f"bench: shot 0 of the timed batch equals the single-shot tile path bit for bit: {same_as_single}")
+    sample = (b.get_data(0), b.image(0))
     if not args.no_compare and args.variant == "batched":
         stack_fp32 = b.stack((args.nz, args.nx))
         t1 = time.perf_counter()
         data = [b.get_data(i) for i in range(len(specs))]
         log(f"bench: observed data to host {time.perf_counter() - t1:.1f}s")
-    b.close()
+    b.close()  # the single-shot check needs the device memory the batch held
+    if not args.no_validate:
+        same_as_single = shot_matches_single(lib, sample, txr, specs, cmig, wav, args.nt, dx, dt, abc, args.variant)
+        log(f"bench: shot 0 of the timed batch equals the single-shot tile path bit for bit: {same_as_single}")
 
     img_errs = None
     if not args.no_validate:
@@ -531,12 +533,14 @@ def main():
                    "mcell_updates_per_s": round(phys / sv / 1e6, 1),
                    "stack_rel_l2_vs_batched_fp32": [float(f"{O.rel_l2(a, r):.3e}") for a, r in zip(st, stack_fp32)],
                    "kernel_form": bv.tblock, "tile_width": bv.txr}
-            if not args.no_validate:
-                ent["single_shot_bitwise"] = shot_matches_single(lib, bv, specs, cmig, wav, args.nt, dx, dt, abc, v)
-                if v == "bf16":
-                    ev = validate(lib, synth, v, args.seed, bv.txr)
-                    ent["subrun_rel_l2_vs_fp64"] = [[float(f"{x:.3e}") for x in e] for e in ev]
+            vtxr, vsample = bv.txr, (data[0], bv.image(0))
             bv.close()
+            if not args.no_validate:
+                ent["single_shot_bitwise"] = shot_matches_single(lib, vsample, vtxr, specs, cmig, wav, args.nt, dx, dt,
+                                                                 abc, v)
+                if v == "bf16":
+                    ev = validate(lib, synth, v, args.seed, vtxr)
+                    ent["subrun_rel_l2_vs_fp64"] = [[float(f"{x:.3e}") for x in e] for e in ev]
             compare[v] = ent
             log(f"bench: {v}: {ent}")
         del data

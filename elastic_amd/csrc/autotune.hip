@@ -40,11 +40,14 @@ static std::string tune_key(const EngineCfg& c, const std::vector<ShotSpec>& sp)
   int dev = 0;
   (void)hipGetDevice(&dev);
   (void)hipGetDeviceProperties(&prop, dev);
-  char buf[256];
-  // shapes bucketed to 64 cells, mean cells per shot to 2^16
-  std::snprintf(buf, sizeof buf, "%s/nz%d/nx%d/c%lld/n%d/f%d/j%d/k%d/a%d/g%d", prop.gcnArchName, (nzmax + 63) / 64,
-                (nxmax + 63) / 64, (cells / (long long)sp.size()) >> 16, (int)std::min<size_t>(sp.size(), 8),
-                c.snap_fmt, c.snap_every, c.comp, c.adj_abc, c.use_graph);
+  char buf[320];
+  // shapes bucketed to 64 cells, mean cells per shot to 2^16; the marching
+  // strip height and width selection (fitting on or off, a fixed width)
+  const char* mw_env = getenv("RTM_MARCH_W");
+  std::snprintf(buf, sizeof buf, "%s/nz%d/nx%d/c%lld/n%d/f%d/j%d/k%d/a%d/g%d/h%d/fit%d/w%d", prop.gcnArchName,
+                (nzmax + 63) / 64, (nxmax + 63) / 64, (cells / (long long)sp.size()) >> 16,
+                (int)std::min<size_t>(sp.size(), 8), c.snap_fmt, c.snap_every, c.comp, c.adj_abc, c.use_graph,
+                c.march_nw, march_fit() ? 1 : 0, mw_env ? std::atoi(mw_env) : 0);
   for (char* p = buf; *p; ++p)
     if (*p == ' ' || *p == ':') *p = '_';
   return buf;
@@ -123,13 +126,14 @@ int autotune_txr(const EngineCfg& cfg, const std::vector<ShotSpec>& shots, const
   int cand[8][2] = {{1, 0}, {2, 0}, {4, 0}, {8, 0}, {1, 1}, {2, 1}, {4, 1}, {2, 2}};
   double t[8];
   int arg = -1;
-  // marching strip widths tried (RTM_MARCH_W fixes one)
+  // marching strip widths tried (RTM_MARCH_W fixes one). A group whose
+  // strips fit one round of resident workgroups at some width gets that
+  // width from the engine whatever the candidate, so then only it is timed.
   constexpr int kMW = 4;
   const int mws[kMW] = {16, 24, 28, 32};
-  // (groups whose strips fit one round at some width get that width from
-  // the engine whatever the candidate; the candidates matter for the others)
-  const int nmw = getenv("RTM_MARCH_W") ? 1 : kMW;
-  int mw_run = cfg.march_w, mw_best = cfg.march_w;
+  const int wfit = march_fit() ? fitted_march_w(sp, c.march_nw, march_resident(c, false)) : 0;
+  const int nmw = getenv("RTM_MARCH_W") || wfit > 0 ? 1 : kMW;
+  int mw_run = wfit > 0 ? wfit : cfg.march_w, mw_best = mw_run;
   auto time_form = [&](int txr, int form, double* ms) -> int {
     c.txr = txr;
     c.tblock = form;
@@ -151,7 +155,7 @@ int autotune_txr(const EngineCfg& cfg, const std::vector<ShotSpec>& shots, const
       double tw[kMW] = {-1, -1, -1, -1};
       int fw[kMW] = {2, 2, 2, 2};
       for (int m = 0; m < nmw; ++m) {
-        mw_run = nmw == 1 ? cfg.march_w : mws[m];
+        if (nmw > 1) mw_run = mws[m];
         for (int form = 2; form <= 3; ++form) {
           double tm = 0;
           STCHK(time_form(2, form, &tm));
@@ -166,7 +170,7 @@ int autotune_txr(const EngineCfg& cfg, const std::vector<ShotSpec>& shots, const
       for (int m = 0; m < nmw; ++m)
         if (tw[m] < tw[mi]) mi = m;
       cand[7][1] = fw[mi];
-      mw_best = nmw == 1 ? cfg.march_w : mws[mi];
+      mw_best = nmw == 1 ? mw_run : mws[mi];
     }
     if (arg < 0 || t[i] < t[arg]) arg = i;
   }

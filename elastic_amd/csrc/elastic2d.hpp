@@ -78,7 +78,10 @@ enum : int {
   A_NEXT = 16, // level L+1 exists (S1, S2 nonzero)
   A_COMP = 32, // compensated image accumulation
   A_NTS = 64,  // non-temporal stores of the new level
-  A_IMG2 = 128 // two-level step: also image level L-1
+  A_IMG2 = 128, // two-level step: also image level L-1
+  A_KSRC = 256  // with A_K: the R/Src sponge. The ring holds K-damped levels
+                // T(L) = K R(L); R(L) = inj + K T(L+2) + Op^T T(L+1); the
+                // level imaged is K T(L) for L >= 3, T(2), R(1)
 };
 
 // Workgroups are dispatched round-robin over the 8 XCDs; renumber them so

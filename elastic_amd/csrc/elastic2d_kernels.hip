@@ -832,7 +832,7 @@ __global__ __launch_bounds__(256) void k_adj(const ShotDev* __restrict__ shots,
     if (flags & A_XIN) {
 #pragma unroll
       for (int f = 0; f < 5; ++f) b[j][f] = gld(out + f * fs, ok ? gi : 0) + b[j][f];
-    } else if (ok && row && gx >= S.rx0 && gx < S.rx0 + S.nrx) {
+    } else if (!(flags & A_KSRC) && ok && row && gx >= S.rx0 && gx < S.rx0 + S.nrx) {
       const float* r = S.rec + (long long)(L - 1) * S.nrx + (gx - S.rx0);
       const long long rs = (long long)S.nt * S.nrx;
 #pragma unroll
@@ -899,6 +899,18 @@ __global__ __launch_bounds__(256) void k_adj(const ShotDev* __restrict__ shots,
     for (int f = 0; f < 5; ++f) o[j][f] = 0.f;
     if (!ok) continue;
     const int gi = gz + ldz * gx;
+    if (flags & A_KSRC) {
+      // R/Src sponge: the injection is added to the damped level L+2
+      const float kxx = gld(S.kx, gx);
+      const bool inj = row && gx >= S.rx0 && gx < S.rx0 + S.nrx;
+      const float* r = S.rec + (long long)(L - 1) * S.nrx + (inj ? gx - S.rx0 : 0);
+      const long long rs = (long long)S.nt * S.nrx;
+#pragma unroll
+      for (int f = 0; f < 5; ++f) {
+        b[j][f] = (b[j][f] * kz) * kxx;
+        if (inj) b[j][f] = gld(r + f * rs, 0) + b[j][f];
+      }
+    }
     float v0 = b[j][0] - sm<SX>(sh[0], p);
     v0 -= sp<1>(sh[2], p);
     float v1 = b[j][1] - sm<1>(sh[1], p);
@@ -908,8 +920,12 @@ __global__ __launch_bounds__(256) void k_adj(const ShotDev* __restrict__ shots,
     float v4 = b[j][4] - sm<1>(sh[3], p);
     v4 -= sm<SX>(sh[4], p);
     float v[5] = {v0, v1, v2, v3, v4};
-    if (flags & A_K) {
-      const float kx = gld(S.kx, gx);
+    // with the R/Src sponge, level 1 (read by no later step) keeps its
+    // undamped value, which is also the one imaged
+    const bool ksrc = flags & A_KSRC;
+    float kx = 1.f;
+    if ((flags & A_K) && (!ksrc || L >= 2)) {
+      kx = gld(S.kx, gx);
 #pragma unroll
       for (int f = 0; f < 5; ++f) v[f] = (v[f] * kz) * kx;
     }
@@ -921,7 +937,7 @@ __global__ __launch_bounds__(256) void k_adj(const ShotDev* __restrict__ shots,
       for (int f = 0; f < 5; ++f) gst(out + f * fs, gi, v[f]);
     }
 #pragma unroll
-    for (int f = 0; f < 5; ++f) o[j][f] = v[f];
+    for (int f = 0; f < 5; ++f) o[j][f] = ksrc && L >= 3 ? (v[f] * kz) * kx : v[f];
   }
 
   if (!imaging) return;
@@ -1862,10 +1878,12 @@ __device__ __forceinline__ void adjm_strip(const ShotDev& S, const TileRef& tr, 
       const int p = m.lane + 5;
       const int gx = x;
       const int gi = gz + ldz * gx;
+      const float kx = (flags & A_K) ? gld(S.kx, gx) : 1.f;
       float bb[5];
 #pragma unroll
       for (int f = 0; f < 5; ++f) {
         bb[f] = cur.b[f];
+        if (flags & A_KSRC) bb[f] = (bb[f] * kz) * kx;
         if (xin || cur_inj) bb[f] = cur.add[f] + bb[f];
       }
       float v0 = bb[0] - smq<DIR>(q0);
@@ -1877,11 +1895,16 @@ __device__ __forceinline__ void adjm_strip(const ShotDev& S, const TileRef& tr, 
       float v4 = bb[4] - sm<1>(col[b][2], p);
       v4 -= smq<DIR>(q4);
       float v[5] = {v0, v1, v2, v3, v4};
-      if (flags & A_K) {
-        const float kx = gld(S.kx, gx);
+      // with the R/Src sponge, level 1 (read by no later step) keeps its
+      // undamped value, which is also the one imaged
+      const bool ksrc = flags & A_KSRC;
+      if ((flags & A_K) && (!ksrc || L >= 2)) {
 #pragma unroll
         for (int f = 0; f < 5; ++f) v[f] = (v[f] * kz) * kx;
       }
+      float vi[5];  // the level's imaged values
+#pragma unroll
+      for (int f = 0; f < 5; ++f) vi[f] = ksrc && L >= 3 ? (v[f] * kz) * kx : v[f];
       if (flags & A_NTS) {
 #pragma unroll
         for (int f = 0; f < 5; ++f) gst_nt(out + f * fs, gi, v[f]);
@@ -1913,8 +1936,8 @@ __device__ __forceinline__ void adjm_strip(const ShotDev& S, const TileRef& tr, 
           s6 = fmaf(smq<DIR>(s1q), inv_dx, s6);
         }
         const float s3 = s4 + s5;
-        const float t[3] = {w * fmaf(v[0], s1, v[1] * s2), -w * fmaf(v[2], s3, v[3] * s3),
-                            -w * fmaf(v[4], s6, fmaf(v[3] * 2.f, s5, v[2] * 2.f * s4))};
+        const float t[3] = {w * fmaf(vi[0], s1, vi[1] * s2), -w * fmaf(vi[2], s3, vi[3] * s3),
+                            -w * fmaf(vi[4], s6, fmaf(vi[3] * 2.f, s5, vi[2] * 2.f * s4))};
         if (comp) {
 #pragma unroll
           for (int a = 0; a < 3; ++a) {

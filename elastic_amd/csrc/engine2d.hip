@@ -72,8 +72,8 @@ void sponge_factors(int n, int abc, std::vector<float>& k) {
 // Each coefficient array is zero outside the 0-based index range of the
 // operators that use it (W/rtmlib-mod.f90, 1-based: Ax/Bx ix 6..nx-5, iz
 // 5..nz-5; Az/Bz ix 5..nx-5, iz 6..nz-5; D/E/F/G 5..n-5; H/J 6..n-5).
-static const int kMaskIx0[5] = {5, 4, 4, 5, 4};
-static const int kMaskIz0[5] = {4, 5, 4, 5, 4};
+const int kMaskIx0[5] = {5, 4, 4, 5, 4};
+const int kMaskIz0[5] = {4, 5, 4, 5, 4};
 
 int fill_masked_coefs(const float* coef5, int nz, int nx, int ldz, float* dev) {
   const size_t n2 = (size_t)ldz * nx;
@@ -122,6 +122,10 @@ int Engine::setup(const EngineCfg& cfg, const std::vector<ShotSpec>& shots,
   }
   if (cfg_.nt < 1 || cfg_.snap_every < 1 || !(cfg_.dx > 0) || !(cfg_.dt > 0)) {
     set_error("engine setup: nt, snap_every must be >= 1 and dx, dt > 0");
+    return E_ARG;
+  }
+  if (cfg_.adj_abc < 0 || cfg_.adj_abc > 2) {
+    set_error("adjoint sponge mode %d not in {0 none, 1 exact adjoint, 2 R/Src}", cfg_.adj_abc);
     return E_ARG;
   }
   if (cfg_.txr != 1 && cfg_.txr != 2 && cfg_.txr != 4 && cfg_.txr != 8) {
@@ -266,13 +270,41 @@ int Engine::alloc_working() {
   return work_.alloc(arena_bytes_, "group working set");
 }
 
+// Workgroups of the marching adjoint (fwd = false) or forward resident at
+// once on the current device; RTM_MARCH_RESIDENT="adj,fwd" caps them (a test
+// hook: small grids then get fitted widths above 16, different per kernel).
+int march_resident(const EngineCfg& c, bool fwd) {
+  int r = fwd ? fwdm_resident_blocks(c.march_nw, c.snap_fmt) : adjm_resident_blocks(c.march_nw, c.snap_fmt);
+  if (const char* e = getenv("RTM_MARCH_RESIDENT")) {
+    int ca = 0, cf = 0;
+    if (std::sscanf(e, "%d,%d", &ca, &cf) == 2) {
+      const int cap = fwd ? cf : ca;
+      if (cap > 0 && (r <= 0 || cap < r)) r = cap;
+    }
+  }
+  return r;
+}
+
+// The narrowest even strip width from 16 to 64 whose strips of the given
+// shots fit one round of `resident` workgroups (0 when none does).
+int fitted_march_w(const std::vector<ShotSpec>& shots, int march_nw, int resident) {
+  if (resident <= 0) return 0;
+  const int sz = TILE_Z * march_nw;
+  for (int c = 16; c <= 64; c += 2) {
+    long long n = 0;
+    for (const ShotSpec& p : shots) n += (long long)((p.nz + sz - 1) / sz) * ((p.nx + c - 1) / c);
+    if (n <= resident) return c;
+  }
+  return 0;
+}
+
 // Shots are packed in order into groups whose working sets fit the arena
 // (and at most max_group shots); one tile table spans all shots of a group.
 int Engine::build_groups() {
   groups_.clear();
   march_fit_ = march_fit();
-  resident_ = march_fit_ ? adjm_resident_blocks(cfg_.march_nw, cfg_.snap_fmt) : 0;
-  fresident_ = march_fit_ ? fwdm_resident_blocks(cfg_.march_nw, cfg_.snap_fmt) : 0;
+  resident_ = march_fit_ ? march_resident(cfg_, false) : 0;
+  fresident_ = march_fit_ ? march_resident(cfg_, true) : 0;
   const int ns = (int)specs_.size();
   std::vector<TileRef> tiles, strips;
   const int tx = tile_x(cfg_.txr);
@@ -303,16 +335,11 @@ int Engine::build_groups() {
       // resident marching-adjoint workgroups (a launch that spills a few
       // workgroups into a second round costs most of a round)
       const int sz = TILE_Z * cfg_.march_nw;
-      auto count = [&](int w) {
-        long long n = 0;
-        for (int si : g.shots) n += (long long)((specs_[si].nz + sz - 1) / sz) * ((specs_[si].nx + w - 1) / w);
-        return n;
-      };
+      std::vector<ShotSpec> gs;
+      for (int si : g.shots) gs.push_back(specs_[si]);
       auto fit = [&](int resident) {
-        if (march_fit_ && resident > 0)
-          for (int c = 16; c <= 64; c += 2)
-            if (count(c) <= resident) return c;
-        return cfg_.march_w;
+        const int w = march_fit_ ? fitted_march_w(gs, cfg_.march_nw, resident) : 0;
+        return w > 0 ? w : cfg_.march_w;
       };
       auto add = [&](int w) {
         for (int slot = 0; slot < (int)g.shots.size(); ++slot) {
@@ -484,11 +511,13 @@ int Engine::enqueue_adjoint(int g, hipStream_t st) {
     STCHK(launch_fill(r.img.as<float>(), 3 * r.n2, 0.f, st));
     if (cfg_.comp) STCHK(launch_fill(r.imgc.as<float>(), 3 * r.n2, 0.f, st));
   }
-  const bool pairs = cfg_.tblock == 1 && cfg_.txr <= 4;
+  // the two-level adjoint has no R/Src sponge form: one level per launch
+  const bool pairs = cfg_.tblock == 1 && cfg_.txr <= 4 && cfg_.adj_abc != 2;
   for (int L = cfg_.nt; L >= 1;) {
     const bool two = pairs && L - 1 >= 1;
     int fl = A_INJ;
     if (cfg_.adj_abc && cfg_.abc > 0) fl |= A_K;
+    if (cfg_.adj_abc == 2 && cfg_.abc > 0) fl |= A_KSRC;
     if (L % cfg_.snap_every == 0) fl |= A_IMG;
     if (two && (L - 1) % cfg_.snap_every == 0) fl |= A_IMG2;
     if (L < cfg_.nt) fl |= A_NEXT;
@@ -520,7 +549,7 @@ long long Engine::fwd_launch_count(int mode) const {
   return n;
 }
 long long Engine::adj_launch_count() const {
-  const bool pairs = cfg_.tblock == 1 && cfg_.txr <= 4;
+  const bool pairs = cfg_.tblock == 1 && cfg_.txr <= 4 && cfg_.adj_abc != 2;
   long long n = 0;
   for (int L = cfg_.nt; L >= 1; ++n) L -= (pairs && L - 1 >= 1) ? 2 : 1;
   return n;

@@ -13,6 +13,11 @@
 
 namespace erm {
 
+// First 0-based column and row of each coefficient's operator range (the
+// last ones are nx-6, nz-6); cells outside are zeroed.
+extern const int kMaskIx0[5];
+extern const int kMaskIz0[5];
+
 struct ShotSpec {
   int nz, nx;                     // computational grid
   int xs, zs, zr, rx0, nrx;       // source cell, receiver row and columns
@@ -58,6 +63,11 @@ struct Timing {
 // The level ring of a shot's working set holds four levels, so that a
 // two-level forward step writes levels other than the two it reads.
 constexpr int kEngineRing = 4;
+
+// Strip-width fitting of the marching forms (see Engine::build_groups).
+bool march_fit();
+int march_resident(const EngineCfg& c, bool fwd);
+int fitted_march_w(const std::vector<ShotSpec>& shots, int march_nw, int resident);
 
 class Engine {
  public:

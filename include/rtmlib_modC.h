@@ -85,7 +85,10 @@ int rtm_final_c_(rtm_ctx *ctx);
  * compile-time): */
 enum {
   RTM_OPT_ADJ_ABC = 1,    /* 0: adjoint without sponge (W/ drop-in, default);
-                             1: exact adjoint of the sponged forward */
+                             1: exact adjoint of the sponged forward;
+                             2: the R/Src adjoint, K applied in place to the
+                                stored levels it+1 and it+2 before every step
+                                (R/Src/rtm-mod-adj.f90:418-430) */
   RTM_OPT_SNAP_EVERY = 2, /* image every j-th level, weight j (default 1) */
   RTM_OPT_SNAP_PREC = 3,  /* source snapshots: 0 fp32 (default), 1 bf16, 2 fp16 */
   RTM_OPT_COMP_IMAGE = 4, /* 1: compensated (Neumaier) FP32 image sums */
@@ -106,7 +109,7 @@ typedef struct {
   int xs, zs;         /* explosive source cell */
   int zr, rx0, nrx;   /* receiver row, first column, count */
   int abc;            /* sponge width (<=0: none) */
-  int adj_abc;        /* 0: adjoint without sponge, 1: exact adjoint sponge */
+  int adj_abc;        /* 0: adjoint without sponge, 1: exact adjoint sponge, 2: R/Src sponge */
   int snap_every;     /* imaging decimation j >= 1 */
   int snap_prec;      /* 0 fp32, 1 bf16, 2 fp16 (per-snapshot max-abs scale) */
   int comp_image;     /* compensated image accumulation */
@@ -178,7 +181,7 @@ typedef struct {
 typedef struct {
   int nt;
   float dx, dt;
-  int abc, adj_abc;
+  int abc, adj_abc;   /* adj_abc: 0 none, 1 exact adjoint sponge, 2 R/Src sponge */
   int snap_every;   /* imaging decimation j */
   int snap_prec;    /* 0 fp32, 1 bf16, 2 fp16 */
   int comp_image;   /* compensated image sums */

@@ -255,6 +255,29 @@ def rtm(p: Params, wavelet, coefs, rec, dtype=np.float64):
     return tuple(out)
 
 
+def rtm_rsrc(p: Params, wavelet, coefs, rec, dtype=np.float64):
+    """RTM with the R/Src adjoint (K applied in place to the stored levels
+    it+1 and it+2 before every step, R/Src/rtm-mod-adj.f90:365-475), full
+    histories, no decimation -> (drho, dlambda, dmu)."""
+    c = [F(a, dtype) for a in coefs]
+    w = np.ascontiguousarray(wavelet, dtype)
+    r = np.ascontiguousarray(rec, dtype)
+    out = [np.zeros((p.nz, p.nx), dtype, order="F") for _ in range(3)]
+    getattr(lib(), "orc_rtm_rsrc_" + _suffix(dtype))(C.byref(p), _p(w), *[_p(a) for a in c], _p(r),
+                                                     *[_p(a) for a in out])
+    return tuple(out)
+
+
+def rsrc_adjoint(p: Params, coefs, rec, dtype=np.float64):
+    """The R/Src adjoint alone: the five stored adjoint histories (each
+    (nz, nx, nt), Fortran order) after the whole loop."""
+    c = [F(a, dtype) for a in coefs]
+    r = np.ascontiguousarray(rec, dtype)
+    h = np.zeros((5, p.nt, p.nx, p.nz), dtype)
+    getattr(lib(), "orc_rsrc_adjoint_" + _suffix(dtype))(C.byref(p), *[_p(a) for a in c], _p(r), _p(h))
+    return [np.asfortranarray(h[f].transpose(2, 1, 0)) for f in range(5)]
+
+
 def born(p: Params, wavelet, coefs, dm, dtype=np.float64):
     """Born modelling (W/rtm-mod-adj.f90:273-375) -> receivers (5, nrx, nt)."""
     c = [F(a, dtype) for a in coefs]

@@ -667,6 +667,101 @@ static void rtm(const Params& p, const T* wavelet, const T* m1_x, const T* m1_z,
   }
 }
 
+// RTM with the R/Src adjoint (R/Src/rtm-mod-adj.f90:365-475): the sponge K
+// is applied, in place, to the stored adjoint levels it+1 and it+2 before
+// every step, so each level is damped once before it feeds the stencil and
+// again before it is added two levels down; the imaging condition then reads
+// the stored (damped) levels. Written literally over full (nz,nx,nt)
+// histories in the reference's call order, with the scatter adjoints of
+// W/rtmlib-mod.f90 (the operators R/Src calls):
+//   it = nt:    A(nt) row zr = data(nt)                           (:380-386)
+//   it = nt-1:  K A(nt); A(nt-1) row = data; A(nt-1) += Op^T A(nt) (:388-415)
+//   it < nt-1:  K A(it+1); K A(it+2); A(it) row = data;
+//               A(it) += A(it+2); A(it) += Op^T A(it+1)            (:418-462)
+// followed by image_op2 over it = 1..nt (:472).
+// The adjoint part: A holds 5 full histories, field f of level L at
+// A + (f*nt + L-1)*n2, zero on entry.
+template <typename T>
+static void rsrc_adjoint(const Params& p, const T* m1_x, const T* m1_z, const T* m2, const T* m3,
+                         const T* m2m3, const T* rec, T* A) {
+  const int nz = p.nz, nx = p.nx, nt = p.nt;
+  const int64_t n2 = (int64_t)nz * nx;
+  std::vector<T> filt(std::max(p.abc + 5, 1));
+  if (p.abc >= 0) abc_filter<T>(p.abc, filt.data());
+  auto lev = [&](int f, int L) { return A + ((int64_t)f * nt + (L - 1)) * n2; };
+  auto inject = [&](int f, int L) {
+    for (int ix = 0; ix < p.nrx; ++ix)
+      lev(f, L)[I2(nz, p.zr, p.rx0 + ix)] = rec[(int64_t)f * nt * p.nrx + (L - 1) + (int64_t)nt * ix];
+  };
+  auto damp = [&](int L) {
+    for (int f = 0; f < 5; ++f) abc_K(nz, nx, p.abc, filt.data(), lev(f, L));
+  };
+  auto add2 = [&](int f, int L) {
+    T* a = lev(f, L);
+    const T* b = lev(f, L + 2);
+    for (int64_t i = 0; i < n2; ++i) a[i] = a[i] + b[i];
+  };
+  // one level L <- data row (+ A(L+2)) + Op^T A(L+1), per field in the
+  // reference's order: Vx (D, F, H), Vz (E, G, J), sxx (Ax), szz (Bz),
+  // sxz (Bx, Az)
+  auto level = [&](int L, bool plus2) {
+    inject(0, L);
+    if (plus2) add2(0, L);
+    adj_op_scatter<T>(4, nz, nx, m2m3, lev(0, L), lev(2, L + 1));
+    adj_op_scatter<T>(6, nz, nx, m2, lev(0, L), lev(3, L + 1));
+    adj_op_scatter<T>(8, nz, nx, m3, lev(0, L), lev(4, L + 1));
+    inject(1, L);
+    if (plus2) add2(1, L);
+    adj_op_scatter<T>(5, nz, nx, m2, lev(1, L), lev(2, L + 1));
+    adj_op_scatter<T>(7, nz, nx, m2m3, lev(1, L), lev(3, L + 1));
+    adj_op_scatter<T>(9, nz, nx, m3, lev(1, L), lev(4, L + 1));
+    inject(2, L);
+    if (plus2) add2(2, L);
+    adj_op_scatter<T>(0, nz, nx, m1_x, lev(2, L), lev(0, L + 1));
+    inject(3, L);
+    if (plus2) add2(3, L);
+    adj_op_scatter<T>(3, nz, nx, m1_z, lev(3, L), lev(1, L + 1));
+    inject(4, L);
+    if (plus2) add2(4, L);
+    adj_op_scatter<T>(1, nz, nx, m1_x, lev(4, L), lev(0, L + 1));
+    adj_op_scatter<T>(2, nz, nx, m1_z, lev(4, L), lev(1, L + 1));
+  };
+  for (int f = 0; f < 5; ++f) inject(f, nt);
+  if (nt >= 2) {
+    damp(nt);
+    level(nt - 1, false);
+  }
+  for (int it = nt - 2; it >= 1; --it) {
+    damp(it + 1);
+    damp(it + 2);
+    level(it, true);
+  }
+}
+
+template <typename T>
+static void rtm_rsrc(const Params& p, const T* wavelet, const T* m1_x, const T* m1_z, const T* m2,
+                     const T* m3, const T* m2m3, const T* rec, T* dm1, T* dm2, T* dm3) {
+  const int nz = p.nz, nx = p.nx, nt = p.nt;
+  const int64_t n2 = (int64_t)nz * nx;
+  std::vector<T> hvx(n2 * nt), hvz(n2 * nt);
+  forward<T>(p, wavelet, m1_x, m1_z, m2, m3, m2m3, nullptr, hvx.data(), hvz.data(), nullptr);
+  std::vector<T> A((size_t)5 * n2 * nt, T(0));
+  rsrc_adjoint<T>(p, m1_x, m1_z, m2, m3, m2m3, rec, A.data());
+  auto lev = [&](int f, int L) { return A.data() + ((int64_t)f * nt + (L - 1)) * n2; };
+  for (int64_t i = 0; i < n2; ++i) dm1[i] = dm2[i] = dm3[i] = T(0);
+  std::vector<T> S(6 * n2);
+  for (int it = 1; it <= nt; ++it) {
+    const T* vx1 = it < nt ? hvx.data() + n2 * it : nullptr;
+    const T* vz1 = it < nt ? hvz.data() + n2 * it : nullptr;
+    image_S<T>(nz, nx, T(p.dx), T(p.dt), hvx.data() + n2 * (it - 1), hvz.data() + n2 * (it - 1), vx1, vz1,
+               S.data(), S.data() + n2, S.data() + 2 * n2, S.data() + 3 * n2, S.data() + 4 * n2,
+               S.data() + 5 * n2);
+    const T* a[5] = {lev(0, it), lev(1, it), lev(2, it), lev(3, it), lev(4, it)};
+    image_adj_level<T>(n2, a, S.data(), S.data() + n2, S.data() + 2 * n2, S.data() + 3 * n2,
+                       S.data() + 4 * n2, S.data() + 5 * n2, T(1), dm1, dm2, dm3);
+  }
+}
+
 // Born modelling (rtm_op2 with adj=.false.), W/rtm-mod-adj.f90:273-375: the
 // background forward gives S1..S6 at every level; image_op2 (forward branch)
 // turns dm into scattering sources; the scattered wavefield is propagated
@@ -789,6 +884,16 @@ static void born(const Params& p, const T* wavelet, const T* m1_x, const T* m1_z
                                 const T* m2, const T* m3, const T* m2m3, const T* rec, T* dm1,     \
                                 T* dm2, T* dm3) {                                                  \
     orc::rtm<T>(*p, wav, m1x, m1z, m2, m3, m2m3, rec, dm1, dm2, dm3);                              \
+  }                                                                                                \
+  ORC_EXPORT void orc_rsrc_adjoint_##SFX(const orc::Params* p, const T* m1x, const T* m1z,          \
+                                         const T* m2, const T* m3, const T* m2m3, const T* rec,    \
+                                         T* hist) {                                                \
+    orc::rsrc_adjoint<T>(*p, m1x, m1z, m2, m3, m2m3, rec, hist);                                   \
+  }                                                                                                \
+  ORC_EXPORT void orc_rtm_rsrc_##SFX(const orc::Params* p, const T* wav, const T* m1x,              \
+                                     const T* m1z, const T* m2, const T* m3, const T* m2m3,        \
+                                     const T* rec, T* dm1, T* dm2, T* dm3) {                       \
+    orc::rtm_rsrc<T>(*p, wav, m1x, m1z, m2, m3, m2m3, rec, dm1, dm2, dm3);                         \
   }                                                                                                \
   ORC_EXPORT void orc_born_##SFX(const orc::Params* p, const T* wav, const T* m1x, const T* m1z,   \
                                  const T* m2, const T* m3, const T* m2m3, const T* dm1,            \

@@ -185,6 +185,46 @@ def test_marching_form_equals_tiles(hip, monkeypatch, prec, j, mw, nw):
                 assert np.array_equal(x, y), tb
 
 
+def _strips(specs, w, nw=2):
+    sz = 64 * nw
+    return sum(((s["nz"] + sz - 1) // sz) * ((s["nx"] + w - 1) // w) for s in specs)
+
+
+def _fitted(specs, resident, nw=2):
+    return next(c for c in range(16, 65, 2) if _strips(specs, c, nw) <= resident)
+
+
+@pytest.mark.parametrize("prec,j", [("fp32", 1), ("bf16", 4)])
+def test_marching_fitted_widths_above_16_differ_per_kernel(hip, monkeypatch, prec, j):
+    """As on the 64-shot survey, the fitted strip widths are above 16 and
+    differ between the adjoint and the forward (whose strips then come from a
+    second table): resident-workgroup caps (RTM_MARCH_RESIDENT) at the strip
+    counts of widths 24 and 20 force that on this small survey. Data and
+    images are bit for bit those of the tile kernels."""
+    specs, ctrue, cmig, wav = _survey(hip)
+    ca, cf = _strips(specs, 24), _strips(specs, 20)
+    wa, wf = _fitted(specs, ca), _fitted(specs, cf)
+    assert wa > 16 and wf > 16 and wa != wf
+    monkeypatch.setenv("RTM_MARCH_RESIDENT", f"{ca},{cf}")
+    monkeypatch.delenv("RTM_MARCH_W", raising=False)
+    monkeypatch.delenv("RTM_MARCH_FIT", raising=False)
+    monkeypatch.setenv("RTM_MARCH_NW", "2")
+    out = {}
+    for tb in ("0", "2"):
+        monkeypatch.setenv("RTM_TBLOCK", tb)
+        b = _batch(hip, specs, ctrue, cmig, wav, txr=2, snap_every=j, snap_prec=prec, comp_image=int(prec != "fp32"))
+        if tb == "2":
+            assert b.march_w == wa
+        b.migrate()
+        out[tb] = ([b.get_data(i) for i in range(len(specs))], [b.image(i) for i in range(len(specs))])
+        b.close()
+    for a, c in zip(out["0"][0], out["2"][0]):
+        assert np.array_equal(a, c)
+    for a, c in zip(out["0"][1], out["2"][1]):
+        for x, y in zip(a, c):
+            assert np.array_equal(x, y)
+
+
 def test_marching_form_needs_width_two(hip, monkeypatch):
     specs, ctrue, cmig, wav = _survey(hip, nshots=1)
     monkeypatch.setenv("RTM_TBLOCK", "2")

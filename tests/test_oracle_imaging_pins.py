@@ -15,14 +15,21 @@ cannot pin (it uses the same S1..S6 in both directions):
   and Bz (W/rtmlib-mod.f90:229-237, :344-352), and with m1 = 0 the stresses
   at the source follow sigma(L) = K(sigma(L-2) + dt*s(L-1)) in closed form.
 
+It also pins the oracle's restatement of the R/Src adjoint with the sponge
+(R/Src/rtm-mod-adj.f90:365-475, ``rtm_rsrc`` / ``rsrc_adjoint``): without a
+sponge it is the drop-in path's linear map, and with every operator zero the
+powers of K on each stored level are counted off the loop.
+
 Each pin is shown to reject a planted slip of the kind it guards against (a
 wrong S6 tap shape, a missing 1/dx, a shrunken range, a dropped factor 2, a
-one-level source shift, K applied at level 2, shifted taps).
+one-level source shift, K applied at level 2, shifted taps, K once per
+level).
 """
 import numpy as np
 import pytest
 
 import oracle as O
+from elastic_amd import synth
 
 DX, DT = 10.0, 1e-3
 NZ, NX = 23, 27
@@ -342,3 +349,89 @@ def test_forward_first_three_levels_by_hand():
     shifted = {d + 1: t for d, t in taps.items()}
     assert not _check_level3(h, w[0], w[1], m1x, m1z, shifted)
     assert not _check_level3(h, w[0], w[2], m1x, m1z, taps)
+
+
+# --------------------------------------------------------------------------- R/Src adjoint
+def _small_shot(abc, nt=60, seed=5):
+    nzp, nxp, dt = 24, 34, 1e-3
+    vp = synth.layered_vp(nzp, nxp, seed, vp_top=1800.0, vp_bot=2600.0, nlayers=2, anom_sigma=4.0)
+    pad = abc + 5 if abc > 0 else 8
+    m = synth.elastic_from_vp(synth.pad_edge(vp, pad))
+    coefs = O.model_op(m.rho, m.lam, m.mu, DX, dt, np.float64)
+    nz, nx = nzp + 2 * pad, nxp + 2 * pad
+    p = O.make_params(nz, nx, nt, DX, dt, pad + nxp // 2, pad + 3, pad + 2, pad, nxp, abc)
+    w = O.ricker(nt, dt, 40.0, nsource=30)
+    rec = np.random.default_rng(seed).uniform(-1, 1, (5, nxp, nt))
+    return p, w, coefs, rec
+
+
+def test_rsrc_adjoint_without_sponge_equals_the_drop_in_path():
+    """With no sponge the R/Src loop (scatter adjoints, full histories) and
+    the gather-form recursion of the drop-in path are the same linear map."""
+    p, w, coefs, rec = _small_shot(0)
+    a = O.rtm_rsrc(p, w, coefs, rec)
+    b = O.rtm(p, w, coefs, rec)
+    for x, y in zip(a, b):
+        assert np.abs(y).max() > 0
+        assert O.rel_l2(x, y) <= 1e-12
+
+
+def test_rsrc_adjoint_with_sponge_differs_from_the_exact_adjoint():
+    p, w, coefs, rec = _small_shot(8)
+    p.adj_abc = 1
+    a = O.rtm_rsrc(p, w, coefs, rec)
+    b = O.rtm(p, w, coefs, rec)
+    assert min(O.rel_l2(x, y) for x, y in zip(a, b)) > 1e-8  # far above FP64 rounding
+
+
+def _rsrc_powers(nt, l0):
+    """K powers of the stored levels when every operator is zero and data
+    d sits at level l0 only, counted off R/Src/rtm-mod-adj.f90:380-462:
+    level L receives A(L+2) after both of that level's K (computed(L) =
+    K^2 computed(L+2)); a stored level L is damped twice (L >= 3), once
+    (L = 2) or never (L = 1)."""
+    count = {L: (2 if L >= 3 else 1 if L == 2 else 0) for L in range(1, nt + 1)}
+    comp = {}
+    L, e = l0, 0
+    while L >= 1:
+        comp[L] = e
+        e += count[L]
+        L -= 2
+    return {L: comp[L] + count[L] for L in comp}
+
+
+def _check_rsrc_powers(hist, nt, zr, xr, kf, powers, d=1.0):
+    for f in range(5):
+        for L in range(1, nt + 1):
+            lv = hist[f][:, :, L - 1]
+            want = d * kf ** powers[L] if L in powers else 0.0
+            if abs(lv[zr, xr] - want) > 1e-14:
+                return False
+            rest = lv.copy()
+            rest[zr, xr] = 0.0
+            if np.any(rest != 0):
+                return False
+    return True
+
+
+@pytest.mark.parametrize("l0", [12, 11])
+def test_rsrc_sponge_powers_closed_form(l0):
+    nt, abc, nz, nx, zr = 12, 8, 30, 32, 2
+    p = O.make_params(nz, nx, nt, DX, DT, 15, 15, zr, 0, nx, abc)
+    coefs = _coefs(0.0, 0.0, 0.0, 0.0, 0.0, nz, nx)
+    xr = 3  # a receiver column inside the sponge, near the corner
+    rec = np.zeros((5, nx, nt))
+    rec[:, xr, l0 - 1] = 1.0
+    kf = float(O.abc_K(np.ones((nz, nx)), abc)[zr, xr])
+    assert kf < 0.95
+    h = O.rsrc_adjoint(p, coefs, rec)
+    good = _rsrc_powers(nt, l0)
+    assert _check_rsrc_powers(h, nt, zr, xr, kf, good)
+    # planted slips: K once per level (as the exact adjoint applies it), and
+    # level 2 damped twice like the others
+    once = {L: (good[L] + 1) // 2 for L in good}
+    assert not _check_rsrc_powers(h, nt, zr, xr, kf, once)
+    if 2 in good:
+        two = dict(good)
+        two[2] += 1
+        assert not _check_rsrc_powers(h, nt, zr, xr, kf, two)
