@@ -9,6 +9,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <vector>
 
 #include "../../include/rtmlib_modC.h"
@@ -135,6 +136,143 @@ static int op3(int nt, int nz, int nx, float* const* hist, const float* const* c
   t.adj_launches = nt - 2;
   t.adj_cells = (double)nz * nx * (nt - 2);
   store_timing(t);
+  return OK;
+}
+
+// ---------------------------------------------------------------------------
+// Allocation-free drop-in adjoint (rtm_op3_hip_noalloc). All device memory it
+// uses is the file-scope array `scratch` (each GPU the library runs on holds
+// its own copy until the process ends): a ring of three levels of the five
+// fields, the masked coefficients, the shot descriptor and the tile table.
+// The histories stay in host memory and pass through the ring one level at a
+// time. Every call on a GPU shares that GPU's array, so calls that overlap on
+// one GPU race; the array keeps the last call's contents, and each call
+// overwrites every part of it that it reads.
+constexpr size_t kScratchBytes = size_t(128) << 20;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static unsigned char* g_scratch_dev[kMaxDevices] = {};
+
+// Device address of the current device's scratch. The array's name is a
+// host-side stand-in; its address on a device comes from
+// hipGetSymbolAddress with that device current, once per device.
+static int scratch_base(unsigned char** out) {
+  int dev = 0;
+  HIPCHK(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDevices) {
+    set_error("device %d beyond the %d devices the scratch table covers", dev, kMaxDevices);
+    return E_ARG;
+  }
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_dev[dev]) {
+    void* p = nullptr;
+    HIPCHK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    g_scratch_dev[dev] = static_cast<unsigned char*>(p);
+  }
+  *out = g_scratch_dev[dev];
+  return OK;
+}
+
+struct MaskOrigin {
+  int ix0[5], iz0[5];
+};
+
+// Zero each (unpadded, ldz = nz) coefficient array outside its operator's
+// range, as fill_masked_coefs does on the host.
+__global__ void k_mask_coefs(float* __restrict__ coef, int nz, int nx, MaskOrigin m) {
+  const long long n2 = (long long)nz * nx;
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < 5 * n2;
+       i += (long long)gridDim.x * blockDim.x) {
+    const int c = (int)(i / n2);
+    const long long j = i - c * n2;
+    const int ix = (int)(j / nz), iz = (int)(j - (long long)ix * nz);
+    if (ix < m.ix0[c] || ix > nx - 6 || iz < m.iz0[c] || iz > nz - 6) coef[i] = 0.f;
+  }
+}
+
+// Tile (or strip) table of one shot: entry bz + nbz*bx covers rows bz*tz,
+// columns bx*tx (the order op3 builds on the host).
+__global__ void k_tile_table(TileRef* __restrict__ t, int nbz, int nbx, int tz, int tx) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < nbz * nbx) {
+    const int bx = i / nbz, bz = i - bx * nbz;
+    t[i] = TileRef{0, bz * tz, bx * tx, i};
+  }
+}
+
+static int op3_noalloc(int nt, int nz, int nx, float* const* hist, const float* const* coefs) {
+  if (nz < 12 || nx < 12) {
+    set_error("grid %d x %d too small: the 10th-order operators need nz, nx >= 12", nz, nx);
+    return E_ARG;
+  }
+  for (int f = 0; f < 5; ++f)
+    if (!hist[f] || !coefs[f]) { set_error("null array argument"); return E_ARG; }
+  if (5LL * nz * nx >= (1LL << 31)) {
+    set_error("grid %d x %d: 5*nz*nx must stay below 2^31", nz, nx);
+    return E_ARG;
+  }
+  const bool march = default_tblock() == 2;
+  const int txr = default_txr(), mw = default_march_w(), mnw = default_march_nw();
+  if (march && (mw < 2 || (mw & 1))) { set_error("RTM_MARCH_W=%d is not a positive even number", mw); return E_ARG; }
+  if (!march && txr != 1 && txr != 2 && txr != 4 && txr != 8) { set_error("unsupported tile width %d", txr); return E_ARG; }
+  const int tx = march ? mw : tile_x(txr), tz = march ? TILE_Z * mnw : TILE_Z;
+  const int nbz = (nz + tz - 1) / tz, nbx = (nx + tx - 1) / tx, ntiles = nbz * nbx;
+  const size_t n2 = (size_t)nz * nx;
+  auto up = [](size_t b) { return (b + 255) / 256 * 256; };
+  const size_t off_desc = up(20 * n2 * sizeof(float));
+  const size_t off_tiles = up(off_desc + sizeof(ShotDev));
+  const size_t need = off_tiles + (size_t)ntiles * sizeof(TileRef);
+  if (need > kScratchBytes) {
+    set_error("rtm_op3_hip_noalloc: a %d x %d grid needs %zu bytes of scratch, above its %zu bytes "
+              "(20 * nz * nx floats plus tables: nz * nx up to about 1.67e6)", nz, nx, need, kScratchBytes);
+    return E_ARG;
+  }
+  if (nt < 3) return OK;  // no level below nt-2 to update
+  STCHK(ensure_device());
+  unsigned char* base = nullptr;
+  STCHK(scratch_base(&base));
+  float* const ring = reinterpret_cast<float*>(base);
+  float* const cf = ring + 15 * n2;
+  ShotDev* const dd = reinterpret_cast<ShotDev*>(base + off_desc);
+  TileRef* const tt = reinterpret_cast<TileRef*>(base + off_tiles);
+  const hipStream_t st = nullptr;  // the null stream: nothing to create
+  for (int c = 0; c < 5; ++c)
+    HIPCHK(hipMemcpyAsync(cf + c * n2, coefs[c], n2 * sizeof(float), hipMemcpyHostToDevice, st));
+  MaskOrigin mo;
+  for (int c = 0; c < 5; ++c) { mo.ix0[c] = kMaskIx0[c]; mo.iz0[c] = kMaskIz0[c]; }
+  k_mask_coefs<<<1024, 256, 0, st>>>(cf, nz, nx, mo);
+  HIPCHK(hipGetLastError());
+  k_tile_table<<<(ntiles + 255) / 256, 256, 0, st>>>(tt, nbz, nbx, tz, tx);
+  HIPCHK(hipGetLastError());
+  // ring of three levels: field f of level L at ring + (L % 3)*5*n2 + f*n2
+  ShotDev d{};
+  d.nz = nz; d.nx = nx; d.ldz = nz; d.nt = nt; d.n2 = (long long)n2;
+  d.lev = ring; d.lev_stride = 5 * (long long)n2; d.fstride = (long long)n2; d.lev_mod = 3; d.lev_off = 0;
+  d.coef = cf;
+  HIPCHK(hipMemcpyAsync(dd, &d, sizeof(ShotDev), hipMemcpyHostToDevice, st));
+  auto slot = [&](int L) { return ring + (size_t)(L % 3) * 5 * n2; };
+  auto level_in = [&](int L) -> int {
+    for (int f = 0; f < 5; ++f)
+      HIPCHK(hipMemcpyAsync(slot(L) + f * n2, hist[f] + (size_t)L * n2, n2 * sizeof(float), hipMemcpyHostToDevice,
+                            st));
+    return OK;
+  };
+  STCHK(level_in(nt - 1));
+  STCHK(level_in(nt - 2));
+  for (int it = nt - 3; it >= 0; --it) {
+    // the slot of level it held level it+3, last read by the step of it+1
+    STCHK(level_in(it));
+    if (march)
+      STCHK(launch_adjm(dd, tt, ntiles, it, A_XIN, mw, mnw, SNAP_HIST, default_xcd(), st));
+    else
+      STCHK(launch_adj(dd, tt, ntiles, it, A_XIN, txr, SNAP_HIST, default_xcd(), st));
+    for (int f = 0; f < 5; ++f)
+      HIPCHK(hipMemcpyAsync(hist[f] + (size_t)it * n2, slot(it) + f * n2, n2 * sizeof(float), hipMemcpyDeviceToHost,
+                            st));
+  }
+  HIPCHK(hipStreamSynchronize(st));
   return OK;
 }
 
@@ -343,6 +481,16 @@ RTM_EXPORT int rtm_op3_hip(int nt, int nz, int nx, int zrec, float* Vx0, float* 
   float* h[5] = {Vx0, Vz0, sigmaxx0, sigmazz0, sigmaxz0};
   const float* c[5] = {m1_x, m1_z, aux_m2_c, aux_m3_c, aux_m2m3_c};
   return op3(nt, nz, nx, h, c);
+}
+
+RTM_EXPORT int rtm_op3_hip_noalloc(int nt, int nz, int nx, int zrec, float* Vx0, float* Vz0,
+                                   float* sigmaxx0, float* sigmazz0, float* sigmaxz0, const float* m1_x,
+                                   const float* m1_z, const float* aux_m2_c, const float* aux_m3_c,
+                                   const float* aux_m2m3_c) {
+  (void)zrec;
+  float* h[5] = {Vx0, Vz0, sigmaxx0, sigmazz0, sigmaxz0};
+  const float* c[5] = {m1_x, m1_z, aux_m2_c, aux_m3_c, aux_m2m3_c};
+  return op3_noalloc(nt, nz, nx, h, c);
 }
 
 RTM_EXPORT void rtm_op3_c_(int nt, int nz, int nx, int zrec, float* Vx0, float* Vz0,

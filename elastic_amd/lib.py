@@ -105,6 +105,17 @@ def rtm_op3(hist, coefs, zrec=0, inplace=False):
     return h
 
 
+def rtm_op3_noalloc(hist, coefs, zrec=0):
+    """rtm_op3_hip_noalloc (the allocation-free drop-in) on copies."""
+    lib = load()
+    h = [_f32F(a).copy(order="F") for a in hist]
+    c = [_f32F(a) for a in coefs]
+    nz, nx, nt = h[0].shape
+    _chk(lib.rtm_op3_hip_noalloc(C.c_int(nt), C.c_int(nz), C.c_int(nx), C.c_int(zrec), *[_p(a) for a in h],
+                                 *[_p(a) for a in c]))
+    return h
+
+
 def rtm_op3_c_void(hist, coefs, zrec=0):
     """Call the void drop-in symbol rtm_op3_c_ itself (in place on copies)."""
     lib = load()

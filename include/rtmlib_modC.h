@@ -55,6 +55,18 @@ int rtm_op3_hip(int nt, int nz, int nx, int zrec, float *Vx0, float *Vz0, float 
                 float *sigmazz0, float *sigmaxz0, const float *m1_x, const float *m1_z,
                 const float *aux_m2_c, const float *aux_m3_c, const float *aux_m2m3_c);
 
+/* Allocation-free form of rtm_op3_hip: same arguments, semantics and status
+ * codes, bit-for-bit the same results. No counterpart in the reference; it
+ * is for callers that cannot allocate device memory (serving loops,
+ * harnesses that forbid hipMalloc). It calls no hipMalloc / hipFree: its
+ * device memory is one per-GPU library array of 128 MiB, so nz*nx is limited
+ * to about 1.67e6 cells (RTM_E_ARG beyond). The histories stay in host
+ * memory and pass through a 3-level device ring, one level at a time. Calls
+ * that overlap on one GPU share that array and race. */
+int rtm_op3_hip_noalloc(int nt, int nz, int nx, int zrec, float *Vx0, float *Vz0, float *sigmaxx0,
+                        float *sigmazz0, float *sigmaxz0, const float *m1_x, const float *m1_z,
+                        const float *aux_m2_c, const float *aux_m3_c, const float *aux_m2m3_c);
+
 /* ------------------------------------------------------------------------
  * Host mirror of the Fortran operator module rtm_mod_adj
  * (W/rtm-mod-adj.f90). rtm_init_c_ mirrors rtm_init (:43-117): Fortran
