@@ -416,6 +416,32 @@ def run_variant(lib, synth, args, variant, steps, warmup, data=None, graph=True)
     return b, specs, cmig, wav, (time.perf_counter() - ts) / max(steps, 1), info
 
 
+def learned(args):
+    """The learned absorbing layer on the 400 x 800 configuration: training
+    on wide-sponge (120-point) teachers of seeded random layered models,
+    then boundary-reflection energy, receiver error and RTM image error on
+    held-out models for the 8-point band sponge alone, with the network, and
+    for the 40-point sponge, all against the wide-sponge runs; forward cost
+    per level with and without the network. Prints its own JSON line and
+    saves the trained parameters (--labc-out)."""
+    from elastic_amd import learned as LRN
+
+    t0 = time.perf_counter()
+    out, net = LRN.experiment(nz=args.n3l[0], nx=args.n3l[1], nt=args.nt3l, steps=tuple(args.labc_steps),
+                              levels_per_shot=args.labc_levels, log=log)
+    if args.labc_out:
+        np.save(args.labc_out, net.params)
+    net.close()
+    line = {"metric": "learned absorbing layer: reflection and image rel-L2 vs a 120-point sponge", "value":
+            out["image_rel_l2_vs_wide"]["band_sponge_net"], "unit": "rel-L2", "n_gpus": 1, "steps": 1,
+            "ms_per_step": round((time.perf_counter() - t0) * 1e3, 1), "higher_is_better": False, "dtype": "fp32",
+            "data": "synthetic (seeded random layered models, Vp 1500-4500 m/s, 3-10 layers, Ricker 8-25 Hz)",
+            "config": {"workload": f"learned layer, {args.n3l[0]}x{args.n3l[1]} grid, {args.nt3l} levels, 8-cell band "
+                                   "+ 8-point band sponge vs 40-point sponge, teachers with a 120-point sponge"},
+            "results": out}
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -436,10 +462,16 @@ def main():
     ap.add_argument("--no-graph", action="store_true",
                     help="launch every level from the host instead of replaying HIP graphs (same kernels; "
                          "for tracing tools that do not follow graph launches)")
-    ap.add_argument("--workload", default="elastic2d", choices=("elastic2d", "acoustic3d", "elastic3d", "probe"),
+    ap.add_argument("--workload", default="elastic2d",
+                    choices=("elastic2d", "acoustic3d", "elastic3d", "probe", "learned"),
                     help="acoustic3d: the 3D acoustic subdomain; elastic3d: the 9-field 3D elastic adjoint "
-                         "step; probe: the streaming-kernel bandwidth probe; all instead of the headline 2D "
-                         "workload")
+                         "step; probe: the streaming-kernel bandwidth probe; learned: training and evaluation "
+                         "of the learned absorbing layer; all instead of the headline 2D workload")
+    ap.add_argument("--n3l", type=int, nargs=2, default=(400, 800))
+    ap.add_argument("--nt3l", type=int, default=2500)
+    ap.add_argument("--labc-steps", type=int, nargs=3, default=(4000, 2000, 2000))
+    ap.add_argument("--labc-levels", type=int, default=200)
+    ap.add_argument("--labc-out", default="")
     ap.add_argument("--n3d", type=int, default=512)
     ap.add_argument("--nt3d", type=int, default=1500)
     ap.add_argument("--n3e", type=int, default=200)
@@ -465,6 +497,9 @@ def main():
         return
     if args.workload == "probe":
         probe(args)
+        return
+    if args.workload == "learned":
+        learned(args)
         return
     dx, dt, fp, abc = 10.0, 8e-4, 15.0, 40
     log(f"bench: {args.shots}-shot survey {args.nz}x{args.nx}, nt={args.nt}, variant={args.variant}")

@@ -58,6 +58,13 @@ struct TileRef {
   int shot, z0, x0, tile;
 };
 
+// Level L of a shot's storage (ring or drop-in history).
+__device__ __forceinline__ float* lev_ptr(float* lev, long long stride, int mod, int off, int L) {
+  int slot = (L + off) % mod;
+  if (slot < 0) slot += mod;  // levels before the first one map into the ring too
+  return lev + (long long)slot * stride;
+}
+
 // Forward flags
 enum : int {
   F_REC = 1,     // record row zr into rec (store)

@@ -455,6 +455,10 @@ struct rtm_batch {
   std::vector<int> nz, nx, nrx;
 };
 
+namespace erm {
+Engine* batch_engine(rtm_batch* b) { return b ? &b->eng : nullptr; }
+}  // namespace erm
+
 RTM_EXPORT const char* rtm_hip_last_error(void) { return erm::last_error(); }
 
 RTM_EXPORT int rtm_hip_init(int device) {

@@ -96,14 +96,6 @@ struct SnapQ {
   __device__ static float dec(unsigned short h) { return SNAP == SNAP_F16 ? dq_f16(h) : dq_bf16(h); }
 };
 
-struct LevelPtr {
-  float* p;
-};
-__device__ __forceinline__ float* lev_ptr(float* lev, long long stride, int mod, int off, int L) {
-  int slot = (L + off) % mod;
-  if (slot < 0) slot += mod;  // levels before the first one map into the ring too
-  return lev + (long long)slot * stride;
-}
 
 // Global-memory accesses through address-space-1 pointers. The descriptor's
 // pointers are generic; read through them the compiler emits FLAT

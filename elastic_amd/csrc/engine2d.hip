@@ -379,7 +379,35 @@ int Engine::build_groups() {
     HIPCHK(hipMemcpy(d_strips_.p, strips.data(), strips.size() * sizeof(TileRef), hipMemcpyHostToDevice));
   }
   for (int g = 0; g < (int)groups_.size(); ++g) STCHK(fill_desc(g, false, false));
+  return build_lcells();
+}
+
+// Band and rim cells of every group's shots (learned layer), one table.
+int Engine::build_lcells() {
+  d_lcells_.release();
+  if (!cfg_.labc_w) return OK;
+  std::vector<LCell> all;
+  for (Group& g : groups_) {
+    std::vector<std::pair<int, int>> grids;
+    for (int s : g.shots) grids.push_back({specs_[s].nz, specs_[s].nx});
+    std::vector<LCell> cells;
+    labc_cell_table(grids, cells, &g.nlband, &g.nlrim);
+    if (g.nlband == 0) {
+      set_error("learned layer: a grid of this group is too small for the 8-cell band");
+      return E_ARG;
+    }
+    g.lcell0 = (int)all.size();
+    all.insert(all.end(), cells.begin(), cells.end());
+  }
+  STCHK(d_lcells_.alloc(all.size() * sizeof(LCell), "learned-layer cells"));
+  HIPCHK(hipMemcpy(d_lcells_.p, all.data(), all.size() * sizeof(LCell), hipMemcpyHostToDevice));
   return OK;
+}
+
+int Engine::set_learned(const float* w) {
+  destroy_graphs();  // their forward loops change
+  cfg_.labc_w = w;
+  return build_lcells();
 }
 
 // Descriptors of group g (variant 0: migration coefficients; 1: true
@@ -448,9 +476,14 @@ int Engine::enqueue_forward(int g, int mode, hipStream_t st) {
   }
   const bool born = mode == 5;
   const int first = born ? 1 : 2;
+  // the learned layer corrects every level after its plain update (the
+  // source runs: imaging, data synthesis, modelling)
+  const bool learn = cfg_.labc_w && mode <= 3;
+  const LCell* lcells = learn ? d_lcells_.as<LCell>() + gr.lcell0 : nullptr;
   // levels L >= 3 go two per launch (temporal blocking) where the kernel
-  // supports the tile width; Born scattering sources are per level
-  const bool pairs = cfg_.tblock == 1 && !born && cfg_.txr <= 4;
+  // supports the tile width; Born scattering sources and the learned layer
+  // are per level
+  const bool pairs = cfg_.tblock == 1 && !born && !learn && cfg_.txr <= 4;
   for (int L = first; L <= cfg_.nt;) {
     const bool two = pairs && L >= 3 && L + 1 <= cfg_.nt;
     int fl = 0;
@@ -478,6 +511,9 @@ int Engine::enqueue_forward(int g, int mode, hipStream_t st) {
       STCHK(launch_fwdm(desc, strips, gr.fnstrips, L, fl, gr.fmarch_w, cfg_.march_nw, snap, cfg_.xcd, st));
     else
       STCHK(launch_fwd(desc, tiles, gr.ntiles, L, fl, cfg_.txr, snap, cfg_.xcd, st));
+    if (learn && L >= 3)
+      STCHK(launch_labc_infer(cfg_.labc_w, desc, lcells, gr.nlband, gr.nlrim, L,
+                              mode == 0 && cfg_.snap_fmt == SNAP_HIST, st));
     for (int Lm = L; Lm <= L + (two ? 1 : 0); ++Lm) {
       if (mode == 3 && cfg_.movie && cfg_.movie_every > 0 && Lm % cfg_.movie_every == 0 &&
           Lm / cfg_.movie_every <= cfg_.movie_n) {
@@ -543,7 +579,7 @@ int Engine::enqueue_adjoint(int g, hipStream_t st) {
 // loops of enqueue_forward / enqueue_adjoint).
 long long Engine::fwd_launch_count(int mode) const {
   const bool born = mode == 5;
-  const bool pairs = cfg_.tblock == 1 && !born && cfg_.txr <= 4;
+  const bool pairs = cfg_.tblock == 1 && !born && !(cfg_.labc_w && mode <= 3) && cfg_.txr <= 4;
   long long n = 0;
   for (int L = born ? 1 : 2; L <= cfg_.nt; ++n) L += (pairs && L >= 3 && L + 1 <= cfg_.nt) ? 2 : 1;
   return n;

@@ -7,6 +7,7 @@
 // adjoint loops of a group can be captured once in HIP graphs and replayed.
 #pragma once
 
+#include <utility>
 #include <vector>
 
 #include "elastic2d.hpp"
@@ -52,6 +53,11 @@ struct EngineCfg {
   int movie_every = 0, movie_n = 0;
   int movie_z0 = 0, movie_x0 = 0, movie_nz = 0, movie_nx = 0;
   float* movie = nullptr;
+  // Learned absorbing layer: BF16-rounded network parameters on the device
+  // (null = none). The forward runs then apply the network's correction to
+  // the band cells of every level L >= 3 after its plain update (and clear
+  // the rim), one level per launch; the adjoint is left as configured.
+  const float* labc_w = nullptr;
 };
 
 struct Timing {
@@ -63,6 +69,22 @@ struct Timing {
 // The level ring of a shot's working set holds four levels, so that a
 // two-level forward step writes levels other than the two it reads.
 constexpr int kEngineRing = 4;
+
+// Learned absorbing layer (learned_abc.hip): cell table entry of the
+// inference launch -- a band cell (ed = edge | depth << 2) or a rim cell
+// (ed = -1) of grid `shot`.
+struct LCell {
+  int shot, iz, ix, ed;
+};
+// Band cells, then rim cells, of the grids (nz, nx) listed.
+void labc_cell_table(const std::vector<std::pair<int, int>>& grids, std::vector<LCell>& cells, int* nband,
+                     int* nrim);
+// After the plain update of level L: band cells += learned correction from
+// levels L-1 and L-2, rim cells = 0 (w: the BF16-rounded parameters).
+// With hist, the correction of Vx, Vz also goes into the fp32 source
+// history of level L (SNAP_HIST), which the step wrote before it.
+int launch_labc_infer(const float* w, const ShotDev* shots, const LCell* cells, int nband, int nrim, int L,
+                      bool hist, hipStream_t st);
 
 // Strip-width fitting of the marching forms (see Engine::build_groups).
 bool march_fit();
@@ -89,6 +111,9 @@ class Engine {
   int get_image(int shot, float* img3);          // 3*(nz,nx)
   int get_stack(float* img3);                    // 3*(stack_nz,stack_nx)
   int get_last_levels(int shot, float* last10);  // levels nt-1, nt of the last run
+  // Use (w = BF16-rounded parameters on the device) or drop (null) the
+  // learned layer in the forward runs.
+  int set_learned(const float* w);
   const Timing& timing() const { return tm_; }
   // strip width of the first group's marching launches: the adjoint's for
   // form 2, the forward's for form 3 (cfg march_w when the form does not
@@ -123,8 +148,10 @@ class Engine {
     int fstrip0 = 0, fnstrips = 0;  // slice of the strip table for the marching forward
     int fmarch_w = 0;               // its strip width
     int desc0 = 0;              // first descriptor
+    int lcell0 = 0, nlband = 0, nlrim = 0;  // slice of the learned layer's cell table
     hipGraphExec_t gfwd = nullptr, gadj = nullptr, gsyn = nullptr;
   };
+  int build_lcells();
   int alloc_working();
   int build_groups();
   int fill_desc(int g, bool true_model, bool born_mode);
@@ -145,13 +172,16 @@ class Engine {
   int fresident_ = 0;       // resident marching-forward workgroups
   long long n2max_ = 0;
   size_t arena_bytes_ = 0;
-  DevBuf src_amp_, work_, d_desc_, d_tiles_, d_strips_, stack_, born_buf_;
+  DevBuf src_amp_, work_, d_desc_, d_tiles_, d_strips_, stack_, born_buf_, d_lcells_;
   std::vector<ShotDev> h_desc_;
   hipStream_t st_ = nullptr;
   hipEvent_t ev_[4] = {};
   int last_group_ = -1;
   Timing tm_;
 };
+
+// The engine of a batch handle (elastic2d_host.hip).
+Engine* batch_engine(struct rtm_batch* b);
 
 int fill_masked_coefs(const float* coef5, int nz, int nx, int ldz, float* dev);
 void sponge_factors(int n, int abc, std::vector<float>& k);

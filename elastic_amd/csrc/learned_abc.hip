@@ -348,50 +348,156 @@ __global__ void k_sample(int* idx, int batch, int n, unsigned seed) {
   idx[i] = (int)(x % (unsigned)n);
 }
 
-// Inference on a grid: corrections delta[cell][5] (physical units) from
-// levels L-1 and L-2. A block evaluates 8 band cells with 32 lanes each:
-// the lanes gather the cell's features into LDS (max-abs scale reduced over
-// the 32 lanes), then lane j computes hidden unit j of each layer; weights
-// are read through the caches (the same 36 KB for every block).
-constexpr int ICELLS = 8;
-__global__ __launch_bounds__(256) void k_infer(const float* __restrict__ w, Lev m1, Lev m2, Mat mat, Band b,
-                                               float* __restrict__ delta) {
-  __shared__ float sx[ICELLS][KP];
-  __shared__ float sh1[ICELLS][H];
-  __shared__ float sh2[ICELLS][H];
-  const int lc = threadIdx.x >> 5, j = threadIdx.x & 31;
-  const int c = blockIdx.x * ICELLS + lc;
-  const bool live = c < b.total;  // uniform over the cell's 32 lanes
-  int iz = 0, ix = 0, edge = 0, depth = 0;
-  if (live) cell_coords(b, c, iz, ix, edge, depth);
+// ---------------------------------------------------------------------------
+// Inference on MFMA, for any number of grids at once (the shots of an engine
+// group, or one grid). The cell table lists band cells (ed = edge | depth <<
+// 2) and then rim cells (ed = -1) of the grids; level L of shot `shot` is
+// read and written through its descriptor. After the plain update of level
+// L, each band cell gets s * NN(features(L-1, L-2) / s) added and each rim
+// cell is cleared. A wave evaluates 16 cells: its lanes gather the
+// features (4 lanes per cell, max-abs over the cell's 230 wavefield values),
+// layer 1 (245 -> 32, 90% of the arithmetic) runs as 16 x 16 x 32 BF16
+// matrix products with FP32 accumulation (the features and weights are
+// BF16-rounded already, so the products are exact and only the summation
+// order differs from a scalar loop), layers 2 and 3 run in FP32 on the
+// vector ALUs from LDS. Weights are read once per block (64 cells).
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+constexpr int MC = 16;        // cells per wave (rows of the matrix product)
+constexpr int KX = 256;       // features padded to 8 K-steps of 32
+constexpr int KXP = KX + 8;   // LDS row pitch (bf16): rows 16 B apart in the banks
+
+__device__ __forceinline__ __bf16 to_bf16(float x) {  // x already bf16-representable
+  return __builtin_bit_cast(__bf16, (unsigned short)(__float_as_uint(x) >> 16));
+}
+
+struct GridView {
+  const float* l1;  // level L-1, field f at + f*fs
+  const float* l2;  // level L-2
+  long long fs;
+  int ldz, nz, nx;
+  __device__ __forceinline__ float get(const float* p, int f, int iz, int ix) const {
+    if (iz < 0 || iz >= nz || ix < 0 || ix >= nx) return 0.f;
+    const float v = p[f * fs + iz + ldz * ix];
+    return f >= 2 ? v * (1.f / Z0) : v;
+  }
+};
+
+__global__ __launch_bounds__(256) void k_infer_mfma(const float* __restrict__ w, const ShotDev* __restrict__ shots,
+                                                    const LCell* __restrict__ cells, int nband, int nrim, int L,
+                                                    int hist) {
+  const int nbb = (nband + 4 * MC - 1) / (4 * MC);
+  if ((int)blockIdx.x >= nbb) {  // rim blocks
+    const int i = ((int)blockIdx.x - nbb) * 256 + (int)threadIdx.x;
+    if (i < nrim) {
+      const LCell c = cells[nband + i];
+      const ShotDev& S = shots[c.shot];
+      float* o = lev_ptr(S.lev, S.lev_stride, S.lev_mod, S.lev_off, L);
+      for (int f = 0; f < NO; ++f) o[f * S.fstride + c.iz + (long long)S.ldz * c.ix] = 0.f;
+    }
+    return;
+  }
+  __shared__ __bf16 sW1[H][KXP];  // W1 transposed: [hidden][feature]
+  __shared__ float sW2[H * H], sW3[H * NO], sB[2 * H + NO];
+  __shared__ __bf16 sX[4][MC][KXP];
+  __shared__ float sH[4][MC][H];
+  const int t = threadIdx.x, wv = t >> 6, lane = t & 63;
+  for (int i = t; i < H * KX; i += 256) {
+    const int j = i / KX, k = i % KX;
+    sW1[j][k] = to_bf16(k < NF ? w[OW1 + k * H + j] : 0.f);
+  }
+  for (int i = t; i < H * H; i += 256) sW2[i] = w[OW2 + i];
+  for (int i = t; i < H * NO; i += 256) sW3[i] = w[OW3 + i];
+  for (int i = t; i < H; i += 256) {
+    sB[i] = w[OB1 + i];
+    sB[H + i] = w[OB2 + i];
+  }
+  if (t < NO) sB[2 * H + t] = w[OB3 + t];
+
+  // features: 4 lanes per cell
+  const int cl = lane >> 2, part = lane & 3;
+  const int c = (int)blockIdx.x * 4 * MC + wv * MC + cl;
+  const bool live = c < nband;  // uniform over the cell's 4 lanes
+  LCell cc = live ? cells[c] : LCell{0, 0, 0, 0};
+  const ShotDev& S = shots[cc.shot];
+  const int edge = cc.ed & 3, depth = cc.ed >> 2;
+  const GridView g{lev_ptr(S.lev, S.lev_stride, S.lev_mod, S.lev_off, L - 1),
+                   lev_ptr(S.lev, S.lev_stride, S.lev_mod, S.lev_off, L - 2), S.fstride, S.ldz, S.nz, S.nx};
+  const int dz = edge == 0 ? 1 : edge == 1 ? -1 : 0, dx = edge == 2 ? 1 : edge == 3 ? -1 : 0;
+  const int tz = edge >= 2 ? 1 : 0, tx = edge < 2 ? 1 : 0;
+  constexpr int NV = (NRAW + 3) / 4;  // values per lane
+  float rv[NV];
   float m = 0.f;
-  for (int k = j; k < NRAW; k += 32) {
-    const float v = live ? raw_feature(m1, m2, iz, ix, edge, depth, k) : 0.f;
-    sx[lc][k] = v;
+#pragma unroll
+  for (int u = 0; u < NV; ++u) {
+    const int k = part + 4 * u;
+    float v = 0.f;
+    if (live && k < 225) {
+      const int kn = k / 25, rem = k - 25 * kn, mm = rem / 5 - 2, f = rem - 5 * (rem / 5);
+      v = g.get(g.l1, f, cc.iz + kn * dz + mm * tz, cc.ix + kn * dx + mm * tx);
+    } else if (live && k < NRAW) {
+      v = g.get(g.l2, k - 225, cc.iz, cc.ix);
+    }
+    rv[u] = v;
     m = fmaxf(m, fabsf(v));
   }
-#pragma unroll
-  for (int o = 16; o > 0; o >>= 1) m = fmaxf(m, __shfl_xor(m, o, 32));
+  m = fmaxf(m, __shfl_xor(m, 1, 64));
+  m = fmaxf(m, __shfl_xor(m, 2, 64));
   const float inv = m > 1e-30f ? 1.f / m : 0.f;
+#pragma unroll
+  for (int u = 0; u < NV; ++u) {
+    const int k = part + 4 * u;
+    if (k < NRAW) sX[wv][cl][k] = to_bf16(bf16r(rv[u] * inv));
+  }
   float cp = 0.f, cs = 0.f, zp = 0.f;
-  if (live) mat.at(iz, ix, cp, cs, zp);
-  for (int k = j; k < KP; k += 32)
-    sx[lc][k] = bf16r(k < NRAW ? sx[lc][k] * inv : static_feature(edge, depth, cp, cs, zp, k));
+  if (live) Mat{S.coef, (int)S.n2, S.ldz}.at(cc.iz, cc.ix, cp, cs, zp);
+  for (int k = NRAW + part; k < KX; k += 4)
+    sX[wv][cl][k] = to_bf16(bf16r(live ? static_feature(edge, depth, cp, cs, zp, k) : 0.f));
   __syncthreads();
-  float a = w[OB1 + j];
-  for (int k = 0; k < NF; ++k) a = fmaf(sx[lc][k], w[OW1 + k * H + j], a);
-  sh1[lc][j] = gelu(a);
+
+  // layer 1 on MFMA: Z1[16 x 32] = X[16 x 256] W1[256 x 32], two 16-column tiles
+  const int r16 = lane & 15, kq = (lane >> 4) * 8;
+#pragma unroll
+  for (int nt = 0; nt < 2; ++nt) {
+    f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int kk = 0; kk < KX / 32; ++kk) {
+      const bf16x8 a = *reinterpret_cast<const bf16x8*>(&sX[wv][r16][kk * 32 + kq]);
+      const bf16x8 b = *reinterpret_cast<const bf16x8*>(&sW1[nt * 16 + r16][kk * 32 + kq]);
+      acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc, 0, 0, 0);
+    }
+    // this lane holds rows (lane / 16) * 4 + i of column nt * 16 + r16
+#pragma unroll
+    for (int i = 0; i < 4; ++i) sH[wv][(lane >> 4) * 4 + i][nt * 16 + r16] = gelu(acc[i] + sB[nt * 16 + r16]);
+  }
   __syncthreads();
-  float a2 = w[OB2 + j];
+  // layer 2 (FP32): hidden units part*8 .. part*8+7 of cell cl; layer 3
+  // partial sums over them, reduced over the cell's 4 lanes
+  float y[NO] = {0.f, 0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+  for (int u = 0; u < 8; ++u) {
+    const int j = part * 8 + u;
+    float a = sB[H + j];
 #pragma unroll 8
-  for (int k = 0; k < H; ++k) a2 = fmaf(sh1[lc][k], w[OW2 + k * H + j], a2);
-  sh2[lc][j] = gelu(a2);
-  __syncthreads();
-  if (live && j < NO) {
-    float y = w[OB3 + j];
-#pragma unroll 8
-    for (int k = 0; k < H; ++k) y = fmaf(sh2[lc][k], w[OW3 + k * NO + j], y);
-    delta[(size_t)c * NO + j] = m > 1e-30f ? y * m * (j >= 2 ? Z0 : 1.f) : 0.f;
+    for (int k = 0; k < H; ++k) a = fmaf(sH[wv][cl][k], sW2[k * H + j], a);
+    const float h2 = gelu(a);
+#pragma unroll
+    for (int o = 0; o < NO; ++o) y[o] = fmaf(h2, sW3[j * NO + o], y[o]);
+  }
+#pragma unroll
+  for (int o = 0; o < NO; ++o) {
+    y[o] += __shfl_xor(y[o], 1, 64);
+    y[o] += __shfl_xor(y[o], 2, 64);
+  }
+  if (live && m > 1e-30f) {
+    float* out = lev_ptr(S.lev, S.lev_stride, S.lev_mod, S.lev_off, L);
+    const long long gi = cc.iz + (long long)S.ldz * cc.ix;
+    float* const h = hist ? static_cast<float*>(S.snap) + (long long)(L - 1) * 2 * S.n2 : nullptr;
+    for (int o = part; o < NO; o += 4) {
+      const float d = (y[o] + sB[2 * H + o]) * m * (o >= 2 ? Z0 : 1.f);
+      out[o * S.fstride + gi] += d;
+      if (h && o < 2) h[o * S.n2 + gi] += d;
+    }
   }
 }
 
@@ -412,36 +518,41 @@ __global__ void k_noise(float* lev, int fs, int ldz, Band b, float amp, unsigned
   }
 }
 
-// level(L)[band cell] += delta[cell]
-__global__ void k_apply(float* lev, int fs, int ldz, Band b, const float* delta) {
-  const int c = blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= b.total) return;
-  int iz, ix, edge, depth;
-  cell_coords(b, c, iz, ix, edge, depth);
-  for (int f = 0; f < NO; ++f) lev[f * fs + iz + ldz * ix] += delta[(size_t)c * NO + f];
-}
-
-// Zero the 5-cell rim of one level: the band grid's stencil updates reach
-// row/column 4, which a learned-layer run keeps at zero like the windowed
-// training levels.
-__global__ void k_zero_rim(float* lev, int fs, int ldz, int nz, int nx) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  const int per = 2 * R * nx + 2 * R * nz;
-  if (i >= per) return;
-  int iz, ix;
-  if (i < 2 * R * nx) {
-    const int r = i / nx;
-    ix = i % nx;
-    iz = r < R ? r : nz - 2 * R + r;
-  } else {
-    const int j = i - 2 * R * nx, r = j / nz;
-    iz = j % nz;
-    ix = r < R ? r : nx - 2 * R + r;
-  }
-  for (int f = 0; f < NO; ++f) lev[f * fs + iz + ldz * ix] = 0.f;
-}
-
 }  // namespace labc
+
+// Band cells, then rim cells, of the given grids (shot = index in grids).
+void labc_cell_table(const std::vector<std::pair<int, int>>& grids, std::vector<LCell>& cells, int* nband,
+                     int* nrim) {
+  using namespace labc;
+  cells.clear();
+  for (int s = 0; s < (int)grids.size(); ++s) {
+    const Band b = band_of(grids[s].first, grids[s].second);
+    for (int c = 0; c < b.total; ++c) {
+      int iz, ix, edge, depth;
+      cell_coords(b, c, iz, ix, edge, depth);
+      cells.push_back(LCell{s, iz, ix, edge | (depth << 2)});
+    }
+  }
+  *nband = (int)cells.size();
+  for (int s = 0; s < (int)grids.size(); ++s) {
+    const int nz = grids[s].first, nx = grids[s].second;
+    for (int ix = 0; ix < nx; ++ix)
+      for (int iz = 0; iz < nz; ++iz)
+        if (iz < R || iz >= nz - R || ix < R || ix >= nx - R) cells.push_back(LCell{s, iz, ix, -1});
+  }
+  *nrim = (int)cells.size() - *nband;
+}
+
+int launch_labc_infer(const float* w, const ShotDev* shots, const LCell* cells, int nband, int nrim, int L,
+                      bool hist, hipStream_t st) {
+  using namespace labc;
+  const int blocks = (nband + 4 * MC - 1) / (4 * MC) + (nrim + 255) / 256;
+  if (blocks == 0) return OK;
+  hipLaunchKernelGGL(k_infer_mfma, dim3(blocks), dim3(256), 0, st, w, shots, cells, nband, nrim, L, (int)hist);
+  HIPCHK(hipGetLastError());
+  return OK;
+}
+
 }  // namespace erm
 
 using namespace erm;
@@ -637,12 +748,14 @@ struct LGrid {
             int band_nx = 0) {
     ldz = device_ldz(nz);
     n2 = (long long)ldz * nx;
-    STCHK(ring.alloc(10 * n2 * 4, "ring"));
+    // four levels, so that level L is written while L-1 and L-2 stay
+    // readable for the learned layer's inference after the step
+    STCHK(ring.alloc(20 * n2 * 4, "ring"));
     STCHK(coef.alloc(5 * n2 * 4, "coefficients"));
     STCHK(fill_masked_coefs(coef5, nz, nx, ldz, coef.as<float>()));
-    STCHK(launch_fill(ring.as<float>(), 10 * n2, 0.f, nullptr));
+    STCHK(launch_fill(ring.as<float>(), 20 * n2, 0.f, nullptr));
     h.nz = nz; h.nx = nx; h.ldz = ldz; h.nt = nt; h.n2 = n2;
-    h.lev = ring.as<float>(); h.lev_stride = 5 * n2; h.fstride = n2; h.lev_mod = 2; h.lev_off = 2;
+    h.lev = ring.as<float>(); h.lev_stride = 5 * n2; h.fstride = n2; h.lev_mod = 4; h.lev_off = 4;
     h.coef = coef.as<float>();
     if (abc > 0 || band_abc > 0) {
       std::vector<float> hz(nz, 1.f), hx(nx, 1.f);
@@ -682,7 +795,7 @@ struct LGrid {
     HIPCHK(hipMemcpy(tiles.p, t.data(), t.size() * sizeof(TileRef), hipMemcpyHostToDevice));
     return OK;
   }
-  float* level(int L) const { return ring.as<float>() + (size_t)(L % 2) * 5 * n2; }
+  float* level(int L) const { return ring.as<float>() + (size_t)((L + 4) % 4) * 5 * n2; }
   int step(int L, int flags, hipStream_t st) {
     return launch_fwd(desc.as<ShotDev>(), tiles.as<TileRef>(), ntiles, L, flags, 4, SNAP_HIST, 1, st);
   }
@@ -797,20 +910,32 @@ RTM_EXPORT int labc_eval(labc_net* n, labc_data* d, float* loss) {
   return OK;
 }
 
+RTM_EXPORT int rtm_batch_set_learned(rtm_batch* b, labc_net* n) {
+  Engine* e = batch_engine(b);
+  if (!e) { set_error("null batch"); return E_ARG; }
+  return e->set_learned(n ? n->w.as<float>() : nullptr);
+}
+
 RTM_EXPORT int labc_forward(labc_net* n, const rtm_shot_t* s, const float* coef5, const float* wavelet,
                             float* rec, float* last) {
   if (!s || !coef5 || !wavelet || (s->nrx > 0 && !rec)) { set_error("bad argument"); return E_ARG; }
   if (n && labc_cell_count(s->nz, s->nx) == 0) { set_error("grid too small for the band"); return E_ARG; }
   STCHK(ensure_device());
-  DevBuf recd, delta;
+  DevBuf recd;
   const int nr = std::max(s->nrx, 1);
   STCHK(recd.alloc((size_t)5 * s->nt * nr * 4, "receivers"));
   STCHK(launch_fill(recd.as<float>(), (size_t)5 * s->nt * nr, 0.f, nullptr));
   LGrid g;
   STCHK(g.setup(s->nz, s->nx, s->nt, s->dt, s->zs, s->xs, coef5, wavelet, s->abc, s->zr, s->rx0, s->nrx,
                 recd.as<float>()));
-  const Band bd = band_of(s->nz, s->nx);
-  if (n) STCHK(delta.alloc((size_t)bd.total * NO * 4, "corrections"));
+  DevBuf dcells;
+  int nband = 0, nrim = 0;
+  if (n) {
+    std::vector<LCell> cells;
+    labc_cell_table({{s->nz, s->nx}}, cells, &nband, &nrim);
+    STCHK(dcells.alloc(cells.size() * sizeof(LCell), "band cells"));
+    HIPCHK(hipMemcpy(dcells.p, cells.data(), cells.size() * sizeof(LCell), hipMemcpyHostToDevice));
+  }
   hipEvent_t e0, e1;
   HIPCHK(hipEventCreate(&e0));
   HIPCHK(hipEventCreate(&e1));
@@ -818,21 +943,10 @@ RTM_EXPORT int labc_forward(labc_net* n, const rtm_shot_t* s, const float* coef5
   for (int L = 2; L <= s->nt; ++L) {
     int fl = F_SRC | (s->nrx > 0 ? F_REC : 0);
     if (L >= 3 && s->abc > 0) fl |= F_K;  // with a network: the sponge and the layer together
-    const bool learn = n && L >= 3;
-    if (learn) {
-      hipLaunchKernelGGL(k_infer, dim3((bd.total + ICELLS - 1) / ICELLS), dim3(256), 0, nullptr, n->w.as<float>(),
-                         g.lev_view(L - 1), g.lev_view(L - 2), g.mat(), bd, delta.as<float>());
-      HIPCHK(hipGetLastError());
-    }
     STCHK(g.step(L, fl, nullptr));
-    if (learn) {
-      hipLaunchKernelGGL(k_apply, dim3((bd.total + 255) / 256), dim3(256), 0, nullptr, g.level(L), (int)g.n2,
-                         g.ldz, bd, delta.as<float>());
-      const int per = 2 * R * (s->nz + s->nx);
-      hipLaunchKernelGGL(k_zero_rim, dim3((per + 255) / 256), dim3(256), 0, nullptr, g.level(L), (int)g.n2, g.ldz,
-                         s->nz, s->nx);
-      HIPCHK(hipGetLastError());
-    }
+    if (n && L >= 3)
+      STCHK(launch_labc_infer(n->w.as<float>(), g.desc.as<ShotDev>(), dcells.as<LCell>(), nband, nrim, L, false,
+                              nullptr));
   }
   HIPCHK(hipEventRecord(e1, nullptr));
   HIPCHK(hipEventSynchronize(e1));

@@ -168,3 +168,157 @@ def propagate(net, s: _lib.RtmShot, coef5, wavelet, want_last=False):
     if want_last:
         return rec, unpack_fields(last), ms
     return rec, ms
+
+
+# --------------------------------------------------------------------------- experiment
+def random_layered_vp(nz, nx, rng, vmin=1500.0, vmax=4500.0, nmin=3, nmax=10):
+    """Seeded random layered Vp model: nmin..nmax interfaces at random depths
+    with small random dips, each layer's velocity uniform in [vmin, vmax]."""
+    n = int(rng.integers(nmin, nmax + 1))
+    z = np.arange(nz, dtype=np.float64)[:, None]
+    x = np.arange(nx, dtype=np.float64)[None, :]
+    depths = np.sort(rng.uniform(0.05 * nz, 0.95 * nz, n))
+    dips = rng.uniform(-0.08, 0.08, n)
+    vels = rng.uniform(vmin, vmax, n + 1)
+    vp = np.full((nz, nx), vels[0])
+    for d, s, v in zip(depths, dips, vels[1:]):
+        vp = np.where(z >= d + s * (x - nx / 2), v, vp)
+    return np.asfortranarray(vp)
+
+
+def _coefs(vp, pad, dx, dt):
+    from . import synth
+
+    m = synth.elastic_from_vp(synth.pad_edge(vp, pad))
+    return _lib.model_op(m.rho, m.lam, m.mu, dx, dt)
+
+
+def _rel(a, b):
+    den = float(np.sqrt(sum(np.sum(np.asarray(y, np.float64) ** 2) for y in b)))
+    num = float(np.sqrt(sum(np.sum((np.asarray(x, np.float64) - np.asarray(y, np.float64)) ** 2)
+                            for x, y in zip(a, b))))
+    return num / den if den > 0 else 0.0
+
+
+def experiment(nz=400, nx=800, dx=10.0, dt=1e-3, nt=2500, abc_wide=120, band_abc=8, train_models=5,
+               shots_per_model=2, levels_per_shot=200, test_models=2, steps=(4000, 2000, 2000), batch=8192,
+               noise=0.03, seed=20260301, log=None):
+    """The learned layer end to end on nz x nx grids (all on the GPU):
+
+    * training: `train_models` seeded random layered models (Vp 1500-4500
+      m/s, 3-10 layers) with `shots_per_model` shots each (Ricker 8-25 Hz,
+      random positions), teacher-forced pairs from runs with an `abc_wide`
+      sponge placed outside the band grid (`levels_per_shot` levels each:
+      train_models * shots_per_model * levels_per_shot snapshot pairs),
+      noise augmentation, Adam in three learning-rate stages;
+    * reflection energy: on held-out models, the physical-region wavefield
+      of the last two levels and the receiver data of an 8-cell band grid
+      with the band sponge alone, with the sponge and the network, and of a
+      40-point-sponge grid, each relative to the wide-sponge run;
+    * images: RTM (exact-adjoint sponge) of the held-out shots through the
+      batch engine with each boundary treatment, the observed data the wide
+      grid's (true minus smoothed model), relative L2 of the physical-region
+      images against the wide-sponge RTM;
+    * cost: device time per forward level with and without the layer.
+    Returns (metrics dict, trained Net)."""
+    from . import synth
+
+    say = log or (lambda *a: None)
+    rng = np.random.default_rng(seed)
+    pad_w = abc_wide + 5 + PAD
+    nzb, nxb = nz + 2 * PAD, nx + 2 * PAD
+    ncell = cell_count(nzb, nxb)
+    nshot_train = train_models * shots_per_model
+    d = Data(ncell * levels_per_shot * nshot_train)
+    t0 = _time()
+    for mi in range(train_models):
+        vp = random_layered_vp(nz, nx, rng)
+        cw, cs = _coefs(vp, pad_w, dx, dt), _coefs(vp, PAD, dx, dt)
+        for si in range(shots_per_model):
+            fp = float(rng.uniform(8.0, 25.0))
+            zs, xs = int(rng.integers(2, nz // 3)), int(rng.integers(0, nx))
+            wav = _lib.ricker(nt, dt, fp)
+            lv = np.linspace(150, nt, levels_per_shot).round().astype(int)
+            d.collect(nz, nx, pad_w, abc_wide, nt, dt, zs, xs, cw, cs, wav, lv, noise=noise,
+                      seed=int(seed + 31 * mi + si), abc_band=band_abc)
+    say(f"learned: {len(d)} training rows from {nshot_train * levels_per_shot} snapshot pairs, "
+        f"{_time() - t0:.1f}s")
+    net = Net(seed=int(seed % 100000))
+    l0 = net.eval(d)
+    t0 = _time()
+    for lr, n in zip((2e-3, 5e-4, 1e-4), steps):
+        net.train(d, n, batch=batch, lr=lr, seed=int(lr * 1e6) + seed)
+    l1 = net.eval(d)
+    train_s = _time() - t0
+    say(f"learned: loss {l0:.3e} -> {l1:.3e} in {sum(steps)} Adam steps, {train_s:.1f}s")
+    d.close()
+
+    # held-out models: reflection energy and images
+    refl = {"band_sponge": [], "band_sponge_net": [], "sponge40": []}
+    rec_err = {k: [] for k in refl}
+    img_err = {k: [] for k in refl}
+    ms_level = {}
+    for mi in range(test_models):
+        vp = random_layered_vp(nz, nx, rng)
+        vps = synth.smooth(vp, 10.0)
+        shots = [(int(rng.integers(2, nz // 3)), int(rng.integers(nx // 8, nx - nx // 8)), float(rng.uniform(8, 25)))
+                 for _ in range(shots_per_model)]
+        for zs, xs, fp in shots:
+            wav = _lib.ricker(nt, dt, fp)
+            zr = 2
+            # forward runs of the true model: the wide reference and the three boundaries
+            sw = _lib.shot(nz + 2 * pad_w, nx + 2 * pad_w, nt, dx, dt, pad_w + xs, pad_w + zs, pad_w + zr, pad_w, nx,
+                           abc_wide)
+            rw, lw = _lib.forward(sw, _coefs(vp, pad_w, dx, dt), wav, want_last=True)
+            ref = [a[pad_w:pad_w + nz, pad_w:pad_w + nx] for a in lw]
+            sb = _lib.shot(nzb, nxb, nt, dx, dt, PAD + xs, PAD + zs, PAD + zr, PAD, nx, band_abc)
+            cb = _coefs(vp, PAD, dx, dt)
+            for key, n in (("band_sponge", None), ("band_sponge_net", net)):
+                r, last, ms = propagate(n, sb, cb, wav, want_last=True)
+                refl[key].append(_rel([a[PAD:PAD + nz, PAD:PAD + nx] for a in last], ref))
+                rec_err[key].append(_rel(r, rw))
+                ms_level.setdefault(key, []).append(ms / (nt - 1))
+            p40 = 45
+            s40 = _lib.shot(nz + 2 * p40, nx + 2 * p40, nt, dx, dt, p40 + xs, p40 + zs, p40 + zr, p40, nx, 40)
+            r40, l40 = _lib.forward(s40, _coefs(vp, p40, dx, dt), wav, want_last=True)
+            refl["sponge40"].append(_rel([a[p40:p40 + nz, p40:p40 + nx] for a in l40], ref))
+            rec_err["sponge40"].append(_rel(r40, rw))
+            # RTM: observed data of the wide grid (true minus smooth model)
+            rws = _lib.forward(sw, _coefs(vps, pad_w, dx, dt), wav)
+            dobs = rw - rws
+            imgs = {}
+            for key, pad, abc, use_net in (("wide", pad_w, abc_wide, False), ("sponge40", p40, 40, False),
+                                           ("band_sponge", PAD, band_abc, False),
+                                           ("band_sponge_net", PAD, band_abc, True)):
+                spec = dict(nz=nz + 2 * pad, nx=nx + 2 * pad, xs=pad + xs, zs=pad + zs, zr=pad + zr, rx0=pad, nrx=nx,
+                            img_z0=pad, img_x0=pad, img_nz=nz, img_nx=nx, stack_z0=0, stack_x0=0)
+                b = _lib.Batch(nt, dx, dt, [spec], [_coefs(vps, pad, dx, dt)], wav, abc=abc, adj_abc=1, txr=2)
+                if use_net:
+                    b.set_learned(net)
+                b.set_data(0, dobs)
+                b.migrate()
+                img = b.image(0)
+                imgs[key] = [a[pad:pad + nz, pad:pad + nx] for a in img]
+                b.close()
+            for key in img_err:
+                img_err[key].append(_rel(imgs[key], imgs["wide"]))
+            say(f"learned: model {mi} shot ({zs},{xs}) {fp:.1f} Hz: reflection "
+                f"{[round(refl[k][-1], 4) for k in refl]}, image {[round(img_err[k][-1], 4) for k in img_err]}")
+
+    def mean(v):
+        return float(np.mean(v)) if v else None
+
+    out = {"training": {"snapshot_pairs": nshot_train * levels_per_shot, "rows": ncell * levels_per_shot * nshot_train,
+                        "loss_initial": l0, "loss_final": l1, "adam_steps": int(sum(steps)), "seconds": train_s},
+           "reflection_rel_l2": {k: mean(v) for k, v in refl.items()},
+           "receiver_rel_l2": {k: mean(v) for k, v in rec_err.items()},
+           "image_rel_l2_vs_wide": {k: mean(v) for k, v in img_err.items()},
+           "per_case": {"reflection": refl, "image": img_err},
+           "ms_per_forward_level": {k: mean(v) for k, v in ms_level.items()}}
+    return out, net
+
+
+def _time():
+    import time
+
+    return time.perf_counter()

@@ -350,6 +350,13 @@ class Batch:
         keys = ["fwd_ms", "fwd_launches", "adj_ms", "adj_launches", "total_ms", "fwd_cells", "adj_cells", "group_size"]
         return dict(zip(keys, out.tolist()))
 
+    def set_learned(self, net):
+        """Use the learned absorbing layer `net` (elastic_amd.learned.Net) in
+        this batch's forward runs, or drop it with None; the network must stay
+        alive while the batch runs."""
+        self._net = net
+        _chk(load().rtm_batch_set_learned(self._h, net.h if net is not None else None))
+
     def close(self):
         if self._h:
             load().rtm_batch_destroy(self._h)

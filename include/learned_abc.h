@@ -112,6 +112,17 @@ int labc_forward(labc_net *net, const rtm_shot_t *s, const float *coef5, const f
 /* Device time of the last labc_forward time loop (ms). */
 double labc_last_forward_ms(void);
 
+/* The learned layer in a batch (no counterpart in the reference): with a
+ * network, every forward run of the batch (data synthesis, the source
+ * wavefield of the imaging) corrects the band cells of each level L >= 3
+ * after its plain update and clears the 5-cell rim, in one launch per
+ * level over the group's shots, the network's first layer on BF16 matrix
+ * cores. The shots' grids should be padded by LABC_RIM + LABC_WIDTH cells
+ * and the batch's sponge width (cfg abc) be the band sponge the network was
+ * trained with; the adjoint keeps the batch's adjoint sponge option. NULL
+ * drops the layer. The network must outlive the batch's runs. */
+int rtm_batch_set_learned(rtm_batch *b, labc_net *net);
+
 #ifdef __cplusplus
 }
 #endif

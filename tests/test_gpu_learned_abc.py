@@ -238,3 +238,61 @@ def test_learned_layer_gain_over_same_width_sponge(L, hip):
         ratios.append(errs[1] / errs[0])
     net.close()
     assert np.mean(ratios) < 0.85, ratios
+
+
+def test_batch_engine_learned_layer_equals_single_grid_path(L, hip):
+    """The batch engine with the learned layer (per-level MFMA inference over
+    all shots of a group, uneven grids) gives the single-grid propagation's
+    receivers bit for bit: synthesized data = labc_forward(true) -
+    labc_forward(migration model), on every shot."""
+    nt, dx, dt = 300, 10.0, 1e-3
+    pad = L.PAD
+    rng = np.random.default_rng(21)
+    params = (rng.standard_normal(LA.NPARAM) * 0.05).astype(np.float32)
+    net = L.Net(seed=1)
+    net.params = params
+    specs, ct, cm = [], [], []
+    for k, (nzp, nxp) in enumerate(((40, 70), (52, 61), (40, 90))):
+        vt = synth.layered_vp(nzp, nxp, 30 + k, vp_top=1800.0, vp_bot=3000.0, nlayers=3, anom_sigma=5.0)
+        vm = synth.smooth(vt, 3.0)
+        specs.append(dict(nz=nzp + 2 * pad, nx=nxp + 2 * pad, xs=pad + nxp // 2, zs=pad + 3, zr=pad + 2, rx0=pad,
+                          nrx=nxp, img_z0=pad, img_x0=pad, img_nz=nzp, img_nx=nxp, stack_z0=0, stack_x0=0))
+        for v, lst in ((vt, ct), (vm, cm)):
+            m = synth.elastic_from_vp(synth.pad_edge(v, pad))
+            lst.append(hip.model_op(m.rho, m.lam, m.mu, dx, dt))
+    w = hip.ricker(nt, dt, 25.0, nsource=60)
+    b = hip.Batch(nt, dx, dt, specs, cm, w, abc=8, adj_abc=1, txr=2, use_graph=1)
+    b.set_true_model(ct)
+    b.set_learned(net)
+    b.synthesize()
+    for i, sp in enumerate(specs):
+        s = hip.shot(sp["nz"], sp["nx"], nt, dx, dt, sp["xs"], sp["zs"], sp["zr"], sp["rx0"], sp["nrx"], abc=8)
+        rt, _ = L.propagate(net, s, ct[i], w)
+        rm, _ = L.propagate(net, s, cm[i], w)
+        assert np.array_equal(b.get_data(i), rt - rm), i
+        plain, _ = L.propagate(None, s, ct[i], w)
+        assert O.rel_l2(rt, plain) > 1e-4  # the layer acted
+    b.migrate()
+    with_net = b.image(0)
+    b.set_learned(None)
+    b.migrate()
+    without = b.image(0)
+    b.close()
+    net.close()
+    for a, c in zip(with_net, without):
+        assert np.isfinite(a).all() and np.abs(a).max() > 0
+    assert max(O.rel_l2(a, c) for a, c in zip(with_net, without)) > 1e-6
+
+
+def test_learned_experiment_small(L, hip):
+    """The end-to-end experiment (training on wide-sponge teachers, held-out
+    reflection and RTM image errors for the three boundaries) at a small
+    size: every metric is finite and the training lowers the loss."""
+    out, net = L.experiment(nz=60, nx=120, nt=500, train_models=2, shots_per_model=1, levels_per_shot=40,
+                            test_models=1, steps=(300, 150, 150), batch=4096, seed=7)
+    net.close()
+    print(out["reflection_rel_l2"], out["image_rel_l2_vs_wide"], out["ms_per_forward_level"])
+    assert out["training"]["loss_final"] < out["training"]["loss_initial"]
+    for grp in ("reflection_rel_l2", "receiver_rel_l2", "image_rel_l2_vs_wide"):
+        for k, v in out[grp].items():
+            assert v is not None and np.isfinite(v), (grp, k)
