@@ -128,10 +128,10 @@ int launch_fwdm(const ShotDev* shots, const TileRef* strips, int nstrips, int L,
                 int snap_fmt, int xcd, hipStream_t st);
 int launch_adjm(const ShotDev* shots, const TileRef* strips, int nstrips, int L, int flags, int xw, int nw,
                 int snap_fmt, int xcd, hipStream_t st);
-// Workgroups of the marching adjoint (nw waves, snapshot format) resident on
-// the current device at once: per-CU occupancy times the CU count (0 if
-// unknown).
-int adjm_resident_blocks(int nw, int snap_fmt);
+// Workgroups of the marching adjoint (nw waves, snapshot format, compensated
+// image sums or not) resident on the current device at once: per-CU
+// occupancy times the CU count (0 if unknown).
+int adjm_resident_blocks(int nw, int snap_fmt, bool comp);
 // The same for the marching forward (without Born sources).
 int fwdm_resident_blocks(int nw, int snap_fmt);
 // Snapshot of the last level (no later step writes it): dV = 0.

@@ -1704,7 +1704,10 @@ __global__ __launch_bounds__(64 * NW) void k_fwdm(const ShotDev* __restrict__ sh
 // P0 = m2m3 sxx + m2 szz, P1 = m2 sxx + m2m3 szz, P2 = m3 sxz, P3 = m1x Vx,
 // P4 = m1z Vz of level L+1 are formed once per column as it enters the
 // queues; imaging reads the source Vx, Vz the same way.
-template <int SNAP, int DIR, int NW>
+// XIN: the drop-in form (the level's own content added first, no imaging
+// or injection), a separate instance: the engine's instance carries neither
+// its loads nor their registers.
+template <int SNAP, int DIR, int NW, bool XIN, bool COMP>
 __device__ __forceinline__ void adjm_strip(const ShotDev& S, const TileRef& tr, int L, int flags, int xw,
                                            float (*col)[6][MarchCol<NW>::MZ]) {
   const int nz = S.nz, nx = S.nx, ldz = S.ldz;
@@ -1718,10 +1721,11 @@ __device__ __forceinline__ void adjm_strip(const ShotDev& S, const TileRef& tr, 
   const int ncol = xe - xb, xs = DIR > 0 ? xb : xe - 1;  // columns, first column of the march
   const MarchLane<NW> m(z0, nz);
   const int gz = m.gz;
-  const bool imaging = flags & A_IMG;
+  // the drop-in form has no imaging, injection or sponge
+  const bool imaging = !XIN && (flags & A_IMG);
   const bool next = flags & A_NEXT;
-  const bool comp = flags & A_COMP;
-  const bool xin = flags & A_XIN;
+  constexpr bool comp = !XIN && COMP;  // compensated image sums (A_COMP), an instance of its own
+  constexpr bool xin = XIN;
   const SrcView<SNAP> sv(S, L);
   // 64-row blocks of the 16-bit snapshot scales: the wave's own rows (clamped
   // for a wave wholly below the grid) and the halo rows above and below the
@@ -1768,7 +1772,7 @@ __device__ __forceinline__ void adjm_strip(const ShotDev& S, const TileRef& tr, 
       s0q[k] = s1q[k] = 0.f;
     }
   }
-  const bool row = (flags & A_INJ) && gz == S.zr;
+  const bool row = !XIN && (flags & A_INJ) && gz == S.zr;
   // Operands of a column, loaded with clamped indices and masked where they
   // are taken: own cell (level L+2, the level's own content for the drop-in
   // form, the injected receivers), halo rows, imaging operands.
@@ -1825,7 +1829,9 @@ __device__ __forceinline__ void adjm_strip(const ShotDev& S, const TileRef& tr, 
   const bool zin4 = gz >= 4 && gz <= nz - 6, zin5 = gz >= 5 && gz <= nz - 6;
   const float w = S.img_w, inv_dx = S.inv_dx, inv_dt = S.inv_dt;
 
-  for (int st = 0; st < ncol; ++st) {
+  // two columns per trip, their column operands in two buffers that swap
+  // roles (no copy of one into the other)
+  auto column = [&](int st, Col& cur, Col& nxt) __attribute__((always_inline)) {
     const int x = xs + DIR * st;
     const bool more = st + 1 < ncol;
     const int xq = x + 6 * DIR;  // column entering the queues
@@ -1841,7 +1847,6 @@ __device__ __forceinline__ void adjm_strip(const ShotDev& S, const TileRef& tr, 
       nsc0 = sv.scale_u(0, zbw, cx(xq));
       nsc1 = sv.scale_u(1, zbw, cx(xq));
     }
-    Col nxt;
     load_col(x + DIR, more, nxt);
 
     const int b = st & 1;
@@ -1955,38 +1960,50 @@ __device__ __forceinline__ void adjm_strip(const ShotDev& S, const TileRef& tr, 
       push(s0q, sv.value_s(ns0, nsc0));
       push(s1q, sv.value_s(ns1, nsc1));
     }
-    cur = nxt;
     cur_h = m.hok && more;
     cur_inj = row && more && x + DIR >= S.rx0 && x + DIR < S.rx0 + S.nrx;
+  };
+  Col alt;
+  for (int st = 0; st < ncol; st += 2) {
+    column(st, cur, alt);
+    if (st + 1 < ncol) column(st + 1, alt, cur);
   }
 }
 
-template <int SNAP, int NW>
+template <int SNAP, int NW, bool XIN, bool COMP>
 __device__ __forceinline__ void adjm_kernel(const ShotDev* __restrict__ shots, const TileRef* __restrict__ strips,
                                             int nstrips, int L, int flags, int xw, int xcd) {
   __shared__ float col[2][6][MarchCol<NW>::MZ];  // P1, P2, P3, P4, source Vx, source Vz of the centre column
   const TileRef tr = strips[xcd ? xcd_tile(blockIdx.x, nstrips) : (int)blockIdx.x];
   // neighbouring strips march in opposite directions (see k_fwdm)
   if (((tr.x0 / xw) & 1) == 0)
-    adjm_strip<SNAP, 1, NW>(shots[tr.shot], tr, L, flags, xw, col);
+    adjm_strip<SNAP, 1, NW, XIN, COMP>(shots[tr.shot], tr, L, flags, xw, col);
   else
-    adjm_strip<SNAP, -1, NW>(shots[tr.shot], tr, L, flags, xw, col);
+    adjm_strip<SNAP, -1, NW, XIN, COMP>(shots[tr.shot], tr, L, flags, xw, col);
 }
 // fp32 snapshot formats: held to 168 VGPRs (3 waves per SIMD; a few spills,
 // measured 9% faster than 2 waves without them)
-template <int SNAP, int NW>
+template <int SNAP, int NW, bool COMP>
 __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(3))) void k_adjm(
     const ShotDev* __restrict__ shots, const TileRef* __restrict__ strips, int nstrips, int L, int flags, int xw,
     int xcd) {
-  adjm_kernel<SNAP, NW>(shots, strips, nstrips, L, flags, xw, xcd);
+  adjm_kernel<SNAP, NW, false, COMP>(shots, strips, nstrips, L, flags, xw, xcd);
 }
-// 16-bit snapshot formats: their decode needs more registers; limiting them
-// to 168 spills 30 and more, measured slower than 2 waves per SIMD
-template <int SNAP, int NW>
-__global__ __launch_bounds__(64 * NW) void k_adjm16(const ShotDev* __restrict__ shots,
+// the drop-in form (fp32 histories, A_XIN)
+template <int NW>
+__global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(3))) void k_adjm_xin(
+    const ShotDev* __restrict__ shots, const TileRef* __restrict__ strips, int nstrips, int L, int flags, int xw,
+    int xcd) {
+  adjm_kernel<SNAP_HIST, NW, true, false>(shots, strips, nstrips, L, flags, xw, xcd);
+}
+// 16-bit snapshot formats: their decode needs more registers; held to 168
+// (3 waves per SIMD, at most 3 spills), measured 7% faster on the BF16
+// 64-shot workload than 2 waves per SIMD without spills
+template <int SNAP, int NW, bool COMP>
+__global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(3))) void k_adjm16(const ShotDev* __restrict__ shots,
                                                    const TileRef* __restrict__ strips, int nstrips, int L, int flags,
                                                    int xw, int xcd) {
-  adjm_kernel<SNAP, NW>(shots, strips, nstrips, L, flags, xw, xcd);
+  adjm_kernel<SNAP, NW, false, COMP>(shots, strips, nstrips, L, flags, xw, xcd);
 }
 
 __global__ void k_fill(float* out, size_t n, float v) {
@@ -2189,25 +2206,39 @@ template <int NW>
 static void adjm_dispatch(int snap, dim3 g, hipStream_t st, const ShotDev* s, const TileRef* t, int n, int L,
                           int flags, int xw, int xcd) {
   const dim3 b(TILE_Z * NW);
+  if (flags & A_XIN) {  // drop-in form (fp32 histories, checked by the launcher)
+    hipLaunchKernelGGL((k_adjm_xin<NW>), g, b, 0, st, s, t, n, L, flags, xw, xcd);
+    return;
+  }
+  if (flags & A_COMP) {
+    switch (snap) {
+      case SNAP_HIST: hipLaunchKernelGGL((k_adjm<SNAP_HIST, NW, true>), g, b, 0, st, s, t, n, L, flags, xw, xcd); break;
+      case SNAP_F32: hipLaunchKernelGGL((k_adjm<SNAP_F32, NW, true>), g, b, 0, st, s, t, n, L, flags, xw, xcd); break;
+      case SNAP_BF16: hipLaunchKernelGGL((k_adjm16<SNAP_BF16, NW, true>), g, b, 0, st, s, t, n, L, flags, xw, xcd); break;
+      case SNAP_F16: hipLaunchKernelGGL((k_adjm16<SNAP_F16, NW, true>), g, b, 0, st, s, t, n, L, flags, xw, xcd); break;
+      default: break;
+    }
+    return;
+  }
   switch (snap) {
-    case SNAP_HIST: hipLaunchKernelGGL((k_adjm<SNAP_HIST, NW>), g, b, 0, st, s, t, n, L, flags, xw, xcd); break;
-    case SNAP_F32: hipLaunchKernelGGL((k_adjm<SNAP_F32, NW>), g, b, 0, st, s, t, n, L, flags, xw, xcd); break;
-    case SNAP_BF16: hipLaunchKernelGGL((k_adjm16<SNAP_BF16, NW>), g, b, 0, st, s, t, n, L, flags, xw, xcd); break;
-    case SNAP_F16: hipLaunchKernelGGL((k_adjm16<SNAP_F16, NW>), g, b, 0, st, s, t, n, L, flags, xw, xcd); break;
+    case SNAP_HIST: hipLaunchKernelGGL((k_adjm<SNAP_HIST, NW, false>), g, b, 0, st, s, t, n, L, flags, xw, xcd); break;
+    case SNAP_F32: hipLaunchKernelGGL((k_adjm<SNAP_F32, NW, false>), g, b, 0, st, s, t, n, L, flags, xw, xcd); break;
+    case SNAP_BF16: hipLaunchKernelGGL((k_adjm16<SNAP_BF16, NW, false>), g, b, 0, st, s, t, n, L, flags, xw, xcd); break;
+    case SNAP_F16: hipLaunchKernelGGL((k_adjm16<SNAP_F16, NW, false>), g, b, 0, st, s, t, n, L, flags, xw, xcd); break;
     default: break;
   }
 }
 
-template <int NW>
+template <int NW, bool COMP>
 static int adjm_occupancy(int snap) {
   int n = 0;
   const int b = TILE_Z * NW;
   hipError_t e = hipSuccess;
   switch (snap) {
-    case SNAP_HIST: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_adjm<SNAP_HIST, NW>, b, 0); break;
-    case SNAP_F32: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_adjm<SNAP_F32, NW>, b, 0); break;
-    case SNAP_BF16: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_adjm16<SNAP_BF16, NW>, b, 0); break;
-    default: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_adjm16<SNAP_F16, NW>, b, 0); break;
+    case SNAP_HIST: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_adjm<SNAP_HIST, NW, COMP>, b, 0); break;
+    case SNAP_F32: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_adjm<SNAP_F32, NW, COMP>, b, 0); break;
+    case SNAP_BF16: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_adjm16<SNAP_BF16, NW, COMP>, b, 0); break;
+    default: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_adjm16<SNAP_F16, NW, COMP>, b, 0); break;
   }
   return e == hipSuccess ? n : 0;
 }
@@ -2239,8 +2270,11 @@ int fwdm_resident_blocks(int nw, int snap) {
   return per * device_cus();
 }
 
-int adjm_resident_blocks(int nw, int snap) {
-  const int per = nw == 1 ? adjm_occupancy<1>(snap) : nw == 2 ? adjm_occupancy<2>(snap) : adjm_occupancy<4>(snap);
+int adjm_resident_blocks(int nw, int snap, bool comp) {
+  const int per = comp ? (nw == 1 ? adjm_occupancy<1, true>(snap) : nw == 2 ? adjm_occupancy<2, true>(snap)
+                                                                          : adjm_occupancy<4, true>(snap))
+                       : (nw == 1 ? adjm_occupancy<1, false>(snap) : nw == 2 ? adjm_occupancy<2, false>(snap)
+                                                                           : adjm_occupancy<4, false>(snap));
   return per * device_cus();
 }
 
@@ -2271,6 +2305,10 @@ int launch_adjm(const ShotDev* shots, const TileRef* strips, int nstrips, int L,
                 int xcd, hipStream_t st) {
   if (nstrips <= 0) return OK;
   STCHK(march_args(nstrips, xw, nw, snap));
+  if ((flags & A_XIN) && (snap != SNAP_HIST || (flags & (A_IMG | A_INJ | A_K)))) {
+    set_error("drop-in adjoint form: fp32 histories without imaging, injection or sponge");
+    return E_ARG;
+  }
   const dim3 g(nstrips);
   switch (nw) {
     case 1: adjm_dispatch<1>(snap, g, st, shots, strips, nstrips, L, flags, xw, xcd); break;

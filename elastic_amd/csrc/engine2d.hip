@@ -274,7 +274,8 @@ int Engine::alloc_working() {
 // once on the current device; RTM_MARCH_RESIDENT="adj,fwd" caps them (a test
 // hook: small grids then get fitted widths above 16, different per kernel).
 int march_resident(const EngineCfg& c, bool fwd) {
-  int r = fwd ? fwdm_resident_blocks(c.march_nw, c.snap_fmt) : adjm_resident_blocks(c.march_nw, c.snap_fmt);
+  int r = fwd ? fwdm_resident_blocks(c.march_nw, c.snap_fmt)
+              : adjm_resident_blocks(c.march_nw, c.snap_fmt, c.comp != 0);
   if (const char* e = getenv("RTM_MARCH_RESIDENT")) {
     int ca = 0, cf = 0;
     if (std::sscanf(e, "%d,%d", &ca, &cf) == 2) {
@@ -404,7 +405,7 @@ int Engine::build_lcells() {
   return OK;
 }
 
-int Engine::set_learned(const float* w) {
+int Engine::set_learned(const unsigned char* w) {
   destroy_graphs();  // their forward loops change
   cfg_.labc_w = w;
   return build_lcells();

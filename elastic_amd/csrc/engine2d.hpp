@@ -53,11 +53,11 @@ struct EngineCfg {
   int movie_every = 0, movie_n = 0;
   int movie_z0 = 0, movie_x0 = 0, movie_nz = 0, movie_nx = 0;
   float* movie = nullptr;
-  // Learned absorbing layer: BF16-rounded network parameters on the device
-  // (null = none). The forward runs then apply the network's correction to
+  // Learned absorbing layer: the network's packed inference weights on the
+  // device (null = none). The forward runs then apply the network's correction to
   // the band cells of every level L >= 3 after its plain update (and clear
   // the rim), one level per launch; the adjoint is left as configured.
-  const float* labc_w = nullptr;
+  const unsigned char* labc_w = nullptr;
 };
 
 struct Timing {
@@ -80,10 +80,11 @@ struct LCell {
 void labc_cell_table(const std::vector<std::pair<int, int>>& grids, std::vector<LCell>& cells, int* nband,
                      int* nrim);
 // After the plain update of level L: band cells += learned correction from
-// levels L-1 and L-2, rim cells = 0 (w: the BF16-rounded parameters).
+// levels L-1 and L-2, rim cells = 0 (w: the network's packed inference
+// weights).
 // With hist, the correction of Vx, Vz also goes into the fp32 source
 // history of level L (SNAP_HIST), which the step wrote before it.
-int launch_labc_infer(const float* w, const ShotDev* shots, const LCell* cells, int nband, int nrim, int L,
+int launch_labc_infer(const unsigned char* w, const ShotDev* shots, const LCell* cells, int nband, int nrim, int L,
                       bool hist, hipStream_t st);
 
 // Strip-width fitting of the marching forms (see Engine::build_groups).
@@ -111,9 +112,9 @@ class Engine {
   int get_image(int shot, float* img3);          // 3*(nz,nx)
   int get_stack(float* img3);                    // 3*(stack_nz,stack_nx)
   int get_last_levels(int shot, float* last10);  // levels nt-1, nt of the last run
-  // Use (w = BF16-rounded parameters on the device) or drop (null) the
-  // learned layer in the forward runs.
-  int set_learned(const float* w);
+  // Use (w = the network's packed inference weights on the device) or drop
+  // (null) the learned layer in the forward runs.
+  int set_learned(const unsigned char* w);
   const Timing& timing() const { return tm_; }
   // strip width of the first group's marching launches: the adjoint's for
   // form 2, the forward's for form 3 (cfg march_w when the form does not

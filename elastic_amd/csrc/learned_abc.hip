@@ -383,7 +383,33 @@ struct GridView {
   }
 };
 
-__global__ __launch_bounds__(256) void k_infer_mfma(const float* __restrict__ w, const ShotDev* __restrict__ shots,
+// Inference weights as the kernel keeps them in LDS, packed once per weight
+// update: W1 transposed to [hidden][feature] in BF16 (rows KXP long), then
+// W2, W3 and the three biases in FP32. The kernel copies the block with
+// 16-byte loads.
+constexpr int PW1 = H * KXP * 2;             // bytes of the BF16 W1 block
+constexpr int PF = H * H + H * NO + 2 * H + NO;  // floats after it
+constexpr int PBYTES = (PW1 + PF * 4 + 15) / 16 * 16;
+
+__global__ void k_pack_infer(const float* __restrict__ w, unsigned char* __restrict__ packed) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  __bf16* w1 = reinterpret_cast<__bf16*>(packed);
+  float* f = reinterpret_cast<float*>(packed + PW1);
+  if (i < H * KXP) {
+    const int j = i / KXP, k = i % KXP;
+    w1[i] = to_bf16(k < NF ? w[OW1 + k * H + j] : 0.f);
+  }
+  if (i < H * H) f[i] = w[OW2 + i];
+  if (i < H * NO) f[H * H + i] = w[OW3 + i];
+  if (i < H) {
+    f[H * H + H * NO + i] = w[OB1 + i];
+    f[H * H + H * NO + H + i] = w[OB2 + i];
+  }
+  if (i < NO) f[H * H + H * NO + 2 * H + i] = w[OB3 + i];
+}
+
+__global__ __launch_bounds__(256) void k_infer_mfma(const unsigned char* __restrict__ packed,
+                                                    const ShotDev* __restrict__ shots,
                                                     const LCell* __restrict__ cells, int nband, int nrim, int L,
                                                     int hist) {
   const int nbb = (nband + 4 * MC - 1) / (4 * MC);
@@ -397,22 +423,16 @@ __global__ __launch_bounds__(256) void k_infer_mfma(const float* __restrict__ w,
     }
     return;
   }
-  __shared__ __bf16 sW1[H][KXP];  // W1 transposed: [hidden][feature]
-  __shared__ float sW2[H * H], sW3[H * NO], sB[2 * H + NO];
+  __shared__ __align__(16) unsigned char sWp[PBYTES];  // the packed weights
   __shared__ __bf16 sX[4][MC][KXP];
   __shared__ float sH[4][MC][H];
   const int t = threadIdx.x, wv = t >> 6, lane = t & 63;
-  for (int i = t; i < H * KX; i += 256) {
-    const int j = i / KX, k = i % KX;
-    sW1[j][k] = to_bf16(k < NF ? w[OW1 + k * H + j] : 0.f);
-  }
-  for (int i = t; i < H * H; i += 256) sW2[i] = w[OW2 + i];
-  for (int i = t; i < H * NO; i += 256) sW3[i] = w[OW3 + i];
-  for (int i = t; i < H; i += 256) {
-    sB[i] = w[OB1 + i];
-    sB[H + i] = w[OB2 + i];
-  }
-  if (t < NO) sB[2 * H + t] = w[OB3 + t];
+  for (int i = t; i < PBYTES / 16; i += 256)
+    reinterpret_cast<uint4*>(sWp)[i] = reinterpret_cast<const uint4*>(packed)[i];
+  const __bf16(*sW1)[KXP] = reinterpret_cast<const __bf16(*)[KXP]>(sWp);
+  const float* sW2 = reinterpret_cast<const float*>(sWp + PW1);
+  const float* sW3 = sW2 + H * H;
+  const float* sB = sW3 + H * NO;
 
   // features: 4 lanes per cell
   const int cl = lane >> 2, part = lane & 3;
@@ -543,7 +563,7 @@ void labc_cell_table(const std::vector<std::pair<int, int>>& grids, std::vector<
   *nrim = (int)cells.size() - *nband;
 }
 
-int launch_labc_infer(const float* w, const ShotDev* shots, const LCell* cells, int nband, int nrim, int L,
+int launch_labc_infer(const unsigned char* w, const ShotDev* shots, const LCell* cells, int nband, int nrim, int L,
                       bool hist, hipStream_t st) {
   using namespace labc;
   const int blocks = (nband + 4 * MC - 1) / (4 * MC) + (nrim + 255) / 256;
@@ -563,7 +583,7 @@ static double g_last_ms = 0.0;
 RTM_EXPORT double labc_last_forward_ms(void) { return g_last_ms; }
 
 struct labc_net {
-  DevBuf master, w, m, v, grad, loss;
+  DevBuf master, w, m, v, grad, loss, packed;  // packed: the inference layout of w
   int t = 0;
 };
 
@@ -584,6 +604,8 @@ static int launch_mlp(labc_net* net, const float* feat, const int* idx, int n, c
 static int refresh_weights(labc_net* n, hipStream_t st) {
   hipLaunchKernelGGL(k_round_params, dim3((NP + 255) / 256), dim3(256), 0, st, n->master.as<float>(),
                      n->w.as<float>());
+  hipLaunchKernelGGL(k_pack_infer, dim3((H * KXP + 255) / 256), dim3(256), 0, st, n->w.as<float>(),
+                     n->packed.as<unsigned char>());
   HIPCHK(hipGetLastError());
   return OK;
 }
@@ -609,6 +631,7 @@ RTM_EXPORT int labc_net_create(labc_net** out, unsigned seed) {
   STCHK(n->v.alloc(NP * 4, "net"));
   STCHK(n->grad.alloc(NP * 4, "net"));
   STCHK(n->loss.alloc(4, "net"));
+  STCHK(n->packed.alloc(PBYTES, "net"));
   HIPCHK(hipMemcpy(n->master.p, p.data(), NP * 4, hipMemcpyHostToDevice));
   HIPCHK(hipMemset(n->m.p, 0, NP * 4));
   HIPCHK(hipMemset(n->v.p, 0, NP * 4));
@@ -913,7 +936,7 @@ RTM_EXPORT int labc_eval(labc_net* n, labc_data* d, float* loss) {
 RTM_EXPORT int rtm_batch_set_learned(rtm_batch* b, labc_net* n) {
   Engine* e = batch_engine(b);
   if (!e) { set_error("null batch"); return E_ARG; }
-  return e->set_learned(n ? n->w.as<float>() : nullptr);
+  return e->set_learned(n ? n->packed.as<unsigned char>() : nullptr);
 }
 
 RTM_EXPORT int labc_forward(labc_net* n, const rtm_shot_t* s, const float* coef5, const float* wavelet,
@@ -945,7 +968,7 @@ RTM_EXPORT int labc_forward(labc_net* n, const rtm_shot_t* s, const float* coef5
     if (L >= 3 && s->abc > 0) fl |= F_K;  // with a network: the sponge and the layer together
     STCHK(g.step(L, fl, nullptr));
     if (n && L >= 3)
-      STCHK(launch_labc_infer(n->w.as<float>(), g.desc.as<ShotDev>(), dcells.as<LCell>(), nband, nrim, L, false,
+      STCHK(launch_labc_infer(n->packed.as<unsigned char>(), g.desc.as<ShotDev>(), dcells.as<LCell>(), nband, nrim, L, false,
                               nullptr));
   }
   HIPCHK(hipEventRecord(e1, nullptr));
