@@ -66,7 +66,11 @@ def algo_bytes(variant):
     v = VARIANTS[variant]
     j = v["snap_every"]
     if v["snap_prec"] == "fp32" and j == 1:
-        fwd, adj = 80 + 8, 80 + 8 + 24  # Vx, Vz history write / read; 3 images RMW
+        # forward: Vx, Vz history written (8 B). Adjoint: the history read at
+        # levels L and L+1 (S1 = (V(L+1) - V(L)) / dt, 16 B; a kernel that
+        # advances one level per launch reads each history level twice) and
+        # 3 images read and written (24 B)
+        fwd, adj = 80 + 8, 80 + 16 + 24
     else:
         b = 2 if v["snap_prec"] in ("bf16", "fp16") else 4
         img = 48 if v["comp_image"] else 24

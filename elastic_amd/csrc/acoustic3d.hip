@@ -88,15 +88,22 @@ __device__ __forceinline__ T g1(const T* p, int i) {
 }
 
 // The march along y is software-pipelined: the global loads of plane y+1
-// (its new queue entry, its LDS halo, and the own-cell p_prev, c, snapshot
-// and image operands) are issued before plane y is computed, so their
-// latency hides behind the arithmetic of the current plane. The plane tile
-// is double-buffered in LDS with one barrier per plane whose fences cover LDS
+// (its new queue entry, and the own-cell p_prev, c, snapshot and image
+// operands) are issued before plane y is computed, so their latency hides
+// behind the arithmetic of the current plane. The plane tile is
+// double-buffered in LDS with one barrier per plane whose fences cover LDS
 // only, so those loads stay in flight across it; they are issued with
 // clamped indices and masked where the values are taken.
+// The halo cells of a plane are loaded with the queue entry of the same
+// plane, five planes before it is computed: that is when the neighbouring
+// tiles load those cells as their own queue entries, so the second read of
+// each line is an L2 hit rather than a second fetch. Each thread keeps its
+// halo cells of the five planes ahead in a ring in LDS (slot = plane % 5)
+// that only it reads.
 template <bool IMG>
 __global__ __launch_bounds__(256) void k_ac3_step(const Ac3Args a) {
   __shared__ float sh[2][ALX * ALZ];
+  __shared__ float hring[5][ANH][256];
   const int nbz = (a.n1 - 10 + AZ - 1) / AZ;
   const int nbx = (a.n2 - 10 + AX - 1) / AX;
   // z-fastest tile order, renumbered so each XCD gets a contiguous range:
@@ -168,9 +175,20 @@ __global__ __launch_bounds__(256) void k_ac3_step(const Ac3Args a) {
         im[j] = g1(static_cast<const float*>(a.img), o);
       }
     }
+    // halo cells of plane y+5, with its queue entry
 #pragma unroll
-    for (int k = 0; k < ANH; ++k) h[k] = g1(a.p, hoff[k] >= 0 ? hoff[k] + py : 0);
+    for (int k = 0; k < ANH; ++k) h[k] = g1(a.p, hoff[k] >= 0 ? hoff[k] + py + AH * s3 : 0);
   };
+  // halo cells of the first five planes into the ring
+#pragma unroll
+  for (int j = 0; j < AH; ++j) {
+    const int py = (y0 + j) * s3;
+#pragma unroll
+    for (int k = 0; k < ANH; ++k) {
+      const float v = g1(a.p, hoff[k] >= 0 ? hoff[k] + py : 0);
+      hring[(y0 + j) % 5][k][tid] = hoff[k] >= 0 ? v : 0.f;
+    }
+  }
   if (y0 < y1) load_plane(y0, hv, qn, ppv, cv, snv, imv);
   float mx = 0.f;
   for (int y = y0; y < y1; ++y) {
@@ -180,9 +198,13 @@ __global__ __launch_bounds__(256) void k_ac3_step(const Ac3Args a) {
     for (int j = 0; j < ATXR; ++j) q[10][j] = coff[j] >= 0 ? qn[j] : 0.f;
 #pragma unroll
     for (int j = 0; j < ATXR; ++j) sp[(ty * ATXR + j + AH) * ALZ + tz + AH] = q[5][j];
+    // this plane's halo cells from the ring; the slot then takes plane y+5's
+    const int slot = y % 5;
 #pragma unroll
-    for (int k = 0; k < ANH; ++k)
-      if (hlds[k] >= 0) sp[hlds[k]] = hoff[k] >= 0 ? hv[k] : 0.f;
+    for (int k = 0; k < ANH; ++k) {
+      if (hlds[k] >= 0) sp[hlds[k]] = hring[slot][k][tid];
+      hring[slot][k][tid] = hoff[k] >= 0 ? hv[k] : 0.f;
+    }
     // the tile of plane y is complete; plane y-1's buffer is free again
     // once every wave has passed this barrier
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");

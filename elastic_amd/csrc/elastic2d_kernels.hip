@@ -1500,7 +1500,7 @@ struct MarchLane {
 // One strip of the forward step, marching towards +x (DIR 1) or -x (DIR -1).
 template <int SNAP, bool BORN, int DIR, int NW>
 __device__ __forceinline__ void fwdm_strip(const ShotDev& S, const TileRef& tr, int L, int flags, int xw,
-                                           float (*col)[4][MarchCol<NW>::MZ]) {
+                                           float (*col)[4][MarchCol<NW>::MZ], float (*hring)[3][10]) {
   const int nz = S.nz, nx = S.nx, ldz = S.ldz;
   const long long fs = S.fstride;
   const int n2 = (int)S.n2;
@@ -1515,32 +1515,47 @@ __device__ __forceinline__ void fwdm_strip(const ShotDev& S, const TileRef& tr, 
   const int gz = m.gz;
   const bool born = BORN;  // Born sources (F_BORN) in their own instance: 10 fewer live registers
 
-  // queues: sigma_xx (S-x), sigma_xz (S+x), Vx (S+x), Vz (S-x)
+  // queues: sigma_xx (S-x), sigma_xz (S+x), Vx (S+x), Vz (S-x). The
+  // priming columns are loaded in increasing x on both march directions, so
+  // the columns a strip shares with its neighbour (which starts at the same
+  // end, marching the other way) are requested by both at about the same time.
   float qa[11], qb[11], qc[11], qd[11];
 #pragma unroll
-  for (int k = 0; k < 11; ++k) {
+  for (int i = 0; i < 11; ++i) {
+    const int k = DIR > 0 ? i : 10 - i;
     const int gx = xs + DIR * (k - 5);
     const bool in = m.zok && gx >= 0 && gx < nx;
     qa[k] = gldm(y1 + 2 * fs, gz + ldz * gx, in);
     qb[k] = gldm(y1 + 4 * fs, gz + ldz * gx, in);
     qc[k] = gldm(y1, gz + ldz * gx, in);
     qd[k] = gldm(y1 + fs, gz + ldz * gx, in);
+    // Halo rows of sigma_xz, Vx, Vz: lanes 0..9 of the first wave load their
+    // halo row of a column when the column enters the queues, the step at
+    // which the strips above and below load those rows as their own (so the
+    // second read is an L2 hit), and keep it in a 7-slot LDS ring (slot =
+    // step % 7) until the column reaches the centre six steps later; only
+    // the lane that writes an entry reads it. sigma_zz is not queued: its
+    // halo rows come with the centre column's own loads.
+    if (k >= 5 && m.lane < 10) {
+      const bool hin = m.hok && k - 5 < ncol;
+      const int ih = m.hz + ldz * gx;
+      hring[k - 5][0][m.lane] = gldm(y1 + 4 * fs, ih, hin);
+      hring[k - 5][1][m.lane] = gldm(y1, ih, hin);
+      hring[k - 5][2][m.lane] = gldm(y1 + fs, ih, hin);
+    }
   }
   // Operands of a column: centre sigma_zz, halo rows of the LDS fields, own
   // cell level L-2 (+ Born sources) and coefficients. Loads are issued with
   // clamped indices and masked where the values are taken, so that nothing
   // waits for them before the next column.
   struct Col {
-    float zz, h[4], o[5], c[5], bs[5];
+    float zz, h, o[5], c[5], bs[5];
   };
   auto load_col = [&](int gx, bool more, Col& v) {
     const int ic = m.zok && more ? gz + ldz * gx : 0;
     const int ih = m.hok && more ? m.hz + ldz * gx : 0;
     v.zz = gld(y1 + 3 * fs, ic);
-    v.h[0] = gld(y1 + 3 * fs, ih);
-    v.h[1] = gld(y1 + 4 * fs, ih);
-    v.h[2] = gld(y1, ih);
-    v.h[3] = gld(y1 + fs, ih);
+    v.h = gld(y1 + 3 * fs, ih);
 #pragma unroll
     for (int f = 0; f < 5; ++f) {
       v.o[f] = gld(x2 + f * fs, ic);
@@ -1561,6 +1576,10 @@ __device__ __forceinline__ void fwdm_strip(const ShotDev& S, const TileRef& tr, 
     const bool qok = m.zok && more && xq >= 0 && xq < nx;
     const int iq = qok ? gz + ldz * xq : 0;
     const float na = gld(y1 + 2 * fs, iq), nb = gld(y1 + 4 * fs, iq), nc = gld(y1, iq), nd = gld(y1 + fs, iq);
+    // halo rows of the column entering the queues (centre at step st + 6)
+    const bool hq = m.hok && st + 6 < ncol;
+    const int ihq = hq ? m.hz + ldz * xq : 0;
+    const float hb = gld(y1 + 4 * fs, ihq), hc = gld(y1, ihq), hd = gld(y1 + fs, ihq);
     Col nxt;
     load_col(x + DIR, more, nxt);
 
@@ -1570,8 +1589,9 @@ __device__ __forceinline__ void fwdm_strip(const ShotDev& S, const TileRef& tr, 
     col[b][2][m.lane + 5] = qc[5];
     col[b][3][m.lane + 5] = qd[5];
     if (m.lane < 10) {
+      col[b][0][m.hs] = cur_h ? cur.h : 0.f;
 #pragma unroll
-      for (int f = 0; f < 4; ++f) col[b][f][m.hs] = cur_h ? cur.h[f] : 0.f;
+      for (int f = 0; f < 3; ++f) col[b][f + 1][m.hs] = hring[st % 7][f][m.lane];
     }
     strip_lds_sync<NW>();
 
@@ -1681,23 +1701,32 @@ __device__ __forceinline__ void fwdm_strip(const ShotDev& S, const TileRef& tr, 
     push(qb, qok ? nb : 0.f);
     push(qc, qok ? nc : 0.f);
     push(qd, qok ? nd : 0.f);
+    if (m.lane < 10) {
+      float* r = &hring[(st + 6) % 7][0][m.lane];
+      r[0] = hq ? hb : 0.f;
+      r[10] = hq ? hc : 0.f;
+      r[20] = hq ? hd : 0.f;
+    }
     cur = nxt;
     cur_h = m.hok && more;
   }
 }
 
+// held to 128 VGPRs (4 waves per SIMD)
 template <int SNAP, bool BORN, int NW>
-__global__ __launch_bounds__(64 * NW) void k_fwdm(const ShotDev* __restrict__ shots, const TileRef* __restrict__ strips,
-                                                 int nstrips, int L, int flags, int xw, int xcd) {
+__global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(4))) void k_fwdm(
+    const ShotDev* __restrict__ shots, const TileRef* __restrict__ strips, int nstrips, int L, int flags, int xw,
+    int xcd) {
   __shared__ float col[2][4][MarchCol<NW>::MZ];  // sigma_zz, sigma_xz, Vx, Vz of the centre column
+  __shared__ float hring[7][3][10];              // sigma_xz, Vx, Vz of the halo rows of the next six columns
   const TileRef tr = strips[xcd ? xcd_tile(blockIdx.x, nstrips) : (int)blockIdx.x];
   // neighbouring strips march in opposite directions, so the columns they
   // share (a strip's priming columns) are read by both at the same end of
   // their marches, close in time, and come from L2 for the second
   if (((tr.x0 / xw) & 1) == 0)
-    fwdm_strip<SNAP, BORN, 1, NW>(shots[tr.shot], tr, L, flags, xw, col);
+    fwdm_strip<SNAP, BORN, 1, NW>(shots[tr.shot], tr, L, flags, xw, col, hring);
   else
-    fwdm_strip<SNAP, BORN, -1, NW>(shots[tr.shot], tr, L, flags, xw, col);
+    fwdm_strip<SNAP, BORN, -1, NW>(shots[tr.shot], tr, L, flags, xw, col, hring);
 }
 
 // Adjoint step (gather form, as k_adj) marching along x: the products
@@ -1709,7 +1738,7 @@ __global__ __launch_bounds__(64 * NW) void k_fwdm(const ShotDev* __restrict__ sh
 // its loads nor their registers.
 template <int SNAP, int DIR, int NW, bool XIN, bool COMP>
 __device__ __forceinline__ void adjm_strip(const ShotDev& S, const TileRef& tr, int L, int flags, int xw,
-                                           float (*col)[6][MarchCol<NW>::MZ]) {
+                                           float (*col)[6][MarchCol<NW>::MZ], float (*hring)[6][10]) {
   const int nz = S.nz, nx = S.nx, ldz = S.ldz;
   const long long fs = S.fstride;
   const int n2 = (int)S.n2;
@@ -1751,11 +1780,50 @@ __device__ __forceinline__ void adjm_strip(const ShotDev& S, const TileRef& tr, 
     w.m2m3 = gld(coef + 4 * n2, i);
   };
 
-  // queues: P0 (S-x), P2 (S+x), P3 (S+x), P4 (S-x); P1 at columns x .. x+5
+  // Halo rows: lanes 0..9 of the first wave load their halo row of a column
+  // when the column enters the queues, the step at which the strips above
+  // and below load those rows as their own (so the second read is an L2
+  // hit), and keep its four products and two source values in a 7-slot LDS
+  // ring (slot = step % 7) until the column reaches the centre six steps
+  // later. Only the lane that writes an entry reads it.
+  struct Halo {
+    Raw w;
+    typename SrcView<SNAP>::Raw s[2];
+    float sc[2][2];  // 16-bit scales of the source values: above, below
+  };
+  auto load_halo = [&](int gx, bool in, Halo& h) {
+    const int ih = in ? m.hz + ldz * gx : 0;
+    raw(ih, h.w);
+    if (imaging) {
+#pragma unroll
+      for (int a = 0; a < 2; ++a) {
+        sv.load_q(a, ih, in, h.s[a]);
+        h.sc[a][0] = sv.scale_u(a, zbt, cx(gx));
+        h.sc[a][1] = sv.scale_u(a, zbb, cx(gx));
+      }
+    }
+  };
+  auto put_halo = [&](int slot, bool in, const Halo& h) {
+    if (m.lane < 10) {
+      float* r = &hring[slot][0][m.lane];
+      r[0] = in ? fmaf(h.w.m2, h.w.xx, h.w.m2m3 * h.w.zz) : 0.f;
+      r[10] = in ? h.w.m3 * h.w.xz : 0.f;
+      r[20] = in ? h.w.m1x * h.w.vx : 0.f;
+      r[30] = in ? h.w.m1z * h.w.vz : 0.f;
+      if (imaging) {
+        r[40] = sv.value_s(h.s[0], m.lane < 5 ? h.sc[0][0] : h.sc[0][1]);
+        r[50] = sv.value_s(h.s[1], m.lane < 5 ? h.sc[1][0] : h.sc[1][1]);
+      }
+    }
+  };
+
+  // queues: P0 (S-x), P2 (S+x), P3 (S+x), P4 (S-x); P1 at columns x .. x+5.
+  // Priming columns in increasing x on both march directions (see k_fwdm).
   float q0[11], q2[11], q3[11], q4[11], d1[6];
   float s0q[11], s1q[11];  // source Vx (S+x), Vz (S-x)
 #pragma unroll
-  for (int k = 0; k < 11; ++k) {
+  for (int i = 0; i < 11; ++i) {
+    const int k = DIR > 0 ? i : 10 - i;
     const int gx = xs + DIR * (k - 5);
     const bool in = m.zok && gx >= 0 && gx < nx;
     Raw w;
@@ -1771,21 +1839,25 @@ __device__ __forceinline__ void adjm_strip(const ShotDev& S, const TileRef& tr, 
     } else {
       s0q[k] = s1q[k] = 0.f;
     }
+    if (k >= 5) {  // the strip's first six columns: halo rows into ring slots 0..5
+      const bool hin = m.hok && k - 5 < ncol;
+      Halo h;
+      load_halo(gx, hin, h);
+      put_halo(k - 5, hin, h);
+    }
   }
   const bool row = !XIN && (flags & A_INJ) && gz == S.zr;
   // Operands of a column, loaded with clamped indices and masked where they
   // are taken: own cell (level L+2, the level's own content for the drop-in
-  // form, the injected receivers), halo rows, imaging operands.
+  // form, the injected receivers), imaging operands.
   struct Col {
     float b[5], add[5];  // level L+2; the level's own content (drop-in form) or the injected receivers
-    Raw h;
-    typename SrcView<SNAP>::Raw hs[2], nt2[2];
-    float hsc[2][2], ntsc[2];  // their 16-bit scales (halo: above, below)
+    typename SrcView<SNAP>::Raw nt2[2];
+    float ntsc[2];  // their 16-bit scales
     float im[3], imc[3];
   };
   auto load_col = [&](int gx, bool more, Col& c) {
     const int ic = m.zok && more ? gz + ldz * gx : 0;
-    const int ih = m.hok && more ? m.hz + ldz * gx : 0;
     const bool inj = row && more && gx >= S.rx0 && gx < S.rx0 + S.nrx;
     const float* r = S.rec + (long long)(L - 1) * S.nrx + (inj ? gx - S.rx0 : 0);
     const long long rs = (long long)S.nt * S.nrx;
@@ -1797,14 +1869,7 @@ __device__ __forceinline__ void adjm_strip(const ShotDev& S, const TileRef& tr, 
       else
         c.add[f] = row ? gld(r + f * rs, 0) : 0.f;
     }
-    raw(ih, c.h);
     if (imaging) {
-#pragma unroll
-      for (int a = 0; a < 2; ++a) {
-        sv.load_q(a, ih, m.hok && more, c.hs[a]);
-        c.hsc[a][0] = sv.scale_u(a, zbt, cx(gx));
-        c.hsc[a][1] = sv.scale_u(a, zbb, cx(gx));
-      }
       if (next) {  // arrays 2, 3 exist only below the last level
         sv.load_q(2, ic, m.zok && more, c.nt2[0]);
         sv.load_q(3, ic, m.zok && more, c.nt2[1]);
@@ -1824,7 +1889,6 @@ __device__ __forceinline__ void adjm_strip(const ShotDev& S, const TileRef& tr, 
   Col cur;
   load_col(xs, true, cur);
   bool cur_inj = row && xs >= S.rx0 && xs < S.rx0 + S.nrx;
-  bool cur_h = m.hok;
   const float kz = ((flags & A_K) && m.zok) ? gld(S.kz, gz) : 1.f;
   const bool zin4 = gz >= 4 && gz <= nz - 6, zin5 = gz >= 5 && gz <= nz - 6;
   const float w = S.img_w, inv_dx = S.inv_dx, inv_dt = S.inv_dt;
@@ -1847,6 +1911,10 @@ __device__ __forceinline__ void adjm_strip(const ShotDev& S, const TileRef& tr, 
       nsc0 = sv.scale_u(0, zbw, cx(xq));
       nsc1 = sv.scale_u(1, zbw, cx(xq));
     }
+    // halo rows of the column entering the queues (centre at step st + 6)
+    const bool hq = m.hok && st + 6 < ncol;
+    Halo nh;
+    load_halo(xq, hq, nh);
     load_col(x + DIR, more, nxt);
 
     const int b = st & 1;
@@ -1859,14 +1927,12 @@ __device__ __forceinline__ void adjm_strip(const ShotDev& S, const TileRef& tr, 
       col[b][5][m.lane + 5] = s1q[5];
     }
     if (m.lane < 10) {
-      const Raw& h = cur.h;
-      col[b][0][m.hs] = cur_h ? fmaf(h.m2, h.xx, h.m2m3 * h.zz) : 0.f;
-      col[b][1][m.hs] = cur_h ? h.m3 * h.xz : 0.f;
-      col[b][2][m.hs] = cur_h ? h.m1x * h.vx : 0.f;
-      col[b][3][m.hs] = cur_h ? h.m1z * h.vz : 0.f;
+      const float* r = &hring[st % 7][0][m.lane];
+#pragma unroll
+      for (int f = 0; f < 4; ++f) col[b][f][m.hs] = r[10 * f];
       if (imaging) {
-        col[b][4][m.hs] = sv.value_s(cur.hs[0], m.lane < 5 ? cur.hsc[0][0] : cur.hsc[0][1]);
-        col[b][5][m.hs] = sv.value_s(cur.hs[1], m.lane < 5 ? cur.hsc[1][0] : cur.hsc[1][1]);
+        col[b][4][m.hs] = r[40];
+        col[b][5][m.hs] = r[50];
       }
     }
     strip_lds_sync<NW>();
@@ -1960,7 +2026,7 @@ __device__ __forceinline__ void adjm_strip(const ShotDev& S, const TileRef& tr, 
       push(s0q, sv.value_s(ns0, nsc0));
       push(s1q, sv.value_s(ns1, nsc1));
     }
-    cur_h = m.hok && more;
+    put_halo((st + 6) % 7, hq, nh);
     cur_inj = row && more && x + DIR >= S.rx0 && x + DIR < S.rx0 + S.nrx;
   };
   Col alt;
@@ -1974,12 +2040,13 @@ template <int SNAP, int NW, bool XIN, bool COMP>
 __device__ __forceinline__ void adjm_kernel(const ShotDev* __restrict__ shots, const TileRef* __restrict__ strips,
                                             int nstrips, int L, int flags, int xw, int xcd) {
   __shared__ float col[2][6][MarchCol<NW>::MZ];  // P1, P2, P3, P4, source Vx, source Vz of the centre column
+  __shared__ float hring[7][6][10];              // the same for the halo rows of the next six columns
   const TileRef tr = strips[xcd ? xcd_tile(blockIdx.x, nstrips) : (int)blockIdx.x];
   // neighbouring strips march in opposite directions (see k_fwdm)
   if (((tr.x0 / xw) & 1) == 0)
-    adjm_strip<SNAP, 1, NW, XIN, COMP>(shots[tr.shot], tr, L, flags, xw, col);
+    adjm_strip<SNAP, 1, NW, XIN, COMP>(shots[tr.shot], tr, L, flags, xw, col, hring);
   else
-    adjm_strip<SNAP, -1, NW, XIN, COMP>(shots[tr.shot], tr, L, flags, xw, col);
+    adjm_strip<SNAP, -1, NW, XIN, COMP>(shots[tr.shot], tr, L, flags, xw, col, hring);
 }
 // fp32 snapshot formats: held to 168 VGPRs (3 waves per SIMD; a few spills,
 // measured 9% faster than 2 waves without them)
