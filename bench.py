@@ -420,6 +420,40 @@ def run_variant(lib, synth, args, variant, steps, warmup, data=None, graph=True)
     return b, specs, cmig, wav, (time.perf_counter() - ts) / max(steps, 1), info
 
 
+def survey_with_layer(args, net):
+    """The batched BF16 survey (64 shots, 800 x 2400, 6000 levels, BF16
+    snapshots every 4th level, compensated images) timed with three
+    boundaries: the 40-point sponge (45 padding cells), the 8-point band
+    sponge alone and with the learned layer (13 cells; the network is the
+    one trained at 400 x 800, so this measures cost, not image quality).
+    One timed migration each, after the data synthesis."""
+    from elastic_amd import lib, synth
+
+    nz, nx, nshots, nt, dx, dt, fp = args.nz, args.nx, args.shots, args.nt, 10.0, 8e-4, 15.0
+    res = {}
+    for key, abc, use_net in (("sponge40", 40, False), ("band_sponge", 8, False), ("band_sponge_net", 8, True)):
+        b, specs, _, _, _ = build_batch(lib, synth, nz, nx, nshots, nt, dx, dt, fp, abc, "bf16", args.seed)
+        try:
+            if use_net:
+                b.set_learned(net)
+            b.synthesize()
+            lib.synchronize()
+            t0 = time.perf_counter()
+            b.migrate()
+            lib.synchronize()
+            sec = time.perf_counter() - t0
+            info = b.info()
+            cells = sum(s["nz"] * s["nx"] for s in specs) * 2.0 * nt
+            res[key] = {"seconds": round(sec, 2), "shots_per_hour": round(3600.0 * nshots / sec, 1),
+                        "padding_cells": abc + 5, "cell_updates": cells, "fwd_ms": round(info["fwd_ms"], 1),
+                        "adj_ms": round(info["adj_ms"], 1), "kernel_form": b.tblock}
+            log(f"learned survey {key}: {res[key]}")
+        finally:
+            b.close()
+    res["net_over_sponge40"] = round(res["band_sponge_net"]["shots_per_hour"] / res["sponge40"]["shots_per_hour"], 3)
+    return res
+
+
 def learned(args):
     """The learned absorbing layer on the 400 x 800 configuration: training
     on wide-sponge (120-point) teachers of seeded random layered models,
@@ -435,6 +469,7 @@ def learned(args):
                               levels_per_shot=args.labc_levels, log=log)
     if args.labc_out:
         np.save(args.labc_out, net.params)
+    out["survey_bf16"] = survey_with_layer(args, net)
     net.close()
     line = {"metric": "learned absorbing layer: reflection and image rel-L2 vs a 120-point sponge", "value":
             out["image_rel_l2_vs_wide"]["band_sponge_net"], "unit": "rel-L2", "n_gpus": 1, "steps": 1,

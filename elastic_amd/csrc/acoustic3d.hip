@@ -116,7 +116,9 @@ __global__ __launch_bounds__(256) void k_ac3_step(const Ac3Args a) {
   const int z0 = AH + bz * AZ, x0 = AH + bx * AX;
   const int y0 = AH + by * a.ychunk;
   const int y1 = min(y0 + a.ychunk, a.n3 - AH);
-  const int tz = threadIdx.x, ty = threadIdx.y;
+  // ty is wave-uniform (a wave is one row of 64 threads), so values indexed
+  // by it and by the plane come from scalar loads
+  const int tz = threadIdx.x, ty = __builtin_amdgcn_readfirstlane(threadIdx.y);
   const int tid = tz + AZ * ty;
   const int gz = z0 + tz;
   const bool zok = gz < a.n1 - AH;
@@ -191,6 +193,10 @@ __global__ __launch_bounds__(256) void k_ac3_step(const Ac3Args a) {
   }
   if (y0 < y1) load_plane(y0, hv, qn, ppv, cv, snv, imv);
   float mx = 0.f;
+  // sponge factors: kz of the thread's row (loop invariant), kx of the
+  // wave's columns and ky of the plane (wave-uniform, scalar loads)
+  const float kzv = (a.kz && zok) ? g1(a.kz, gz) : 1.f;
+  auto sc = [](const float* p, int i) { return ((const __attribute__((address_space(4))) float*)p)[i]; };
   for (int y = y0; y < y1; ++y) {
     const int bb = (y - y0) & 1;
     float* const sp = sh[bb];
@@ -215,8 +221,7 @@ __global__ __launch_bounds__(256) void k_ac3_step(const Ac3Args a) {
     load_plane(y + 1 < y1 ? y + 1 : y, hv2, qn, pp2, c2, sn2, im2);
     const int py = y * s3;
     if (zok) {
-      const float kzv = a.kz ? g1(a.kz, gz) : 1.f;
-      const float kyv = a.kz ? g1(a.ky, y) : 1.f;
+      const float kyv = a.kz ? sc(a.ky, y) : 1.f;
 #pragma unroll
       for (int j = 0; j < ATXR; ++j) {
         const int lxo = ty * ATXR + j;
@@ -230,7 +235,7 @@ __global__ __launch_bounds__(256) void k_ac3_step(const Ac3Args a) {
         lap += W4 * ((sp[p0 + 4] + sp[p0 - 4]) + (sp[p0 + 4 * ALZ] + sp[p0 - 4 * ALZ]) + (q[9][j] + q[1][j]));
         lap += W5 * ((sp[p0 + 5] + sp[p0 - 5]) + (sp[p0 + 5 * ALZ] + sp[p0 - 5 * ALZ]) + (q[10][j] + q[0][j]));
         const int gi = coff[j] + py;
-        const float kk = a.kz ? kzv * g1(a.kx, gx) * kyv : 1.f;
+        const float kk = a.kz ? kzv * sc(a.kx, gx) * kyv : 1.f;
         float v = kk * (2.f * q[5][j] - ppv[j] + cv[j] * lap);
         if (a.mode == 0) {
           if (gz == a.sz && gx == a.sx && y == a.sy) v += a.amp;

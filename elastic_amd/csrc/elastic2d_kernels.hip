@@ -113,6 +113,14 @@ template <class T>
 __device__ __forceinline__ void gst_nt(T* p, int i, T v) {
   __builtin_nontemporal_store(v, (__attribute__((address_space(1))) T*)p + i);
 }
+// Load of an element whose index is the same in every lane, of an array no
+// kernel writes while it runs: through the constant address space it is
+// one scalar load into a scalar register (a divergent index would still be
+// served correctly, by vector loads).
+template <class T>
+__device__ __forceinline__ T gldu(const T* p, int i) {
+  return ((const __attribute__((address_space(4))) T*)p)[i];
+}
 // Masked load without a branch: lanes outside read element 0 (a valid
 // address) and return zero.
 template <class T>
@@ -238,7 +246,7 @@ __global__ __launch_bounds__(256) void k_fwd(const ShotDev* __restrict__ shots,
     v[4] = fmaf(m3, dvx_zm, v[4]);
     v[4] = fmaf(m3, dvz_xm, v[4]);
     if (flags & F_K) {
-      const float kx = gld(S.kx, gx);
+      const float kx = gldu(S.kx, gx);
 #pragma unroll
       for (int f = 0; f < 5; ++f) v[f] = (v[f] * kz) * kx;
     }
@@ -893,7 +901,7 @@ __global__ __launch_bounds__(256) void k_adj(const ShotDev* __restrict__ shots,
     const int gi = gz + ldz * gx;
     if (flags & A_KSRC) {
       // R/Src sponge: the injection is added to the damped level L+2
-      const float kxx = gld(S.kx, gx);
+      const float kxx = gldu(S.kx, gx);
       const bool inj = row && gx >= S.rx0 && gx < S.rx0 + S.nrx;
       const float* r = S.rec + (long long)(L - 1) * S.nrx + (inj ? gx - S.rx0 : 0);
       const long long rs = (long long)S.nt * S.nrx;
@@ -917,7 +925,7 @@ __global__ __launch_bounds__(256) void k_adj(const ShotDev* __restrict__ shots,
     const bool ksrc = flags & A_KSRC;
     float kx = 1.f;
     if ((flags & A_K) && (!ksrc || L >= 2)) {
-      kx = gld(S.kx, gx);
+      kx = gldu(S.kx, gx);
 #pragma unroll
       for (int f = 0; f < 5; ++f) v[f] = (v[f] * kz) * kx;
     }
@@ -1628,7 +1636,7 @@ __device__ __forceinline__ void fwdm_strip(const ShotDev& S, const TileRef& tr, 
       v[4] = fmaf(m3, dvx_zm, v[4]);
       v[4] = fmaf(m3, dvz_xm, v[4]);
       if (flags & F_K) {
-        const float kx = gld(S.kx, gx);
+        const float kx = gldu(S.kx, gx);
 #pragma unroll
         for (int f = 0; f < 5; ++f) v[f] = (v[f] * kz) * kx;
       }
@@ -1941,7 +1949,7 @@ __device__ __forceinline__ void adjm_strip(const ShotDev& S, const TileRef& tr, 
       const int p = m.lane + 5;
       const int gx = x;
       const int gi = gz + ldz * gx;
-      const float kx = (flags & A_K) ? gld(S.kx, gx) : 1.f;
+      const float kx = (flags & A_K) ? gldu(S.kx, gx) : 1.f;
       float bb[5];
 #pragma unroll
       for (int f = 0; f < 5; ++f) {

@@ -411,10 +411,12 @@ __global__ void k_pack_infer(const float* __restrict__ w, unsigned char* __restr
 __global__ __launch_bounds__(256) void k_infer_mfma(const unsigned char* __restrict__ packed,
                                                     const ShotDev* __restrict__ shots,
                                                     const LCell* __restrict__ cells, int nband, int nrim, int L,
-                                                    int hist) {
+                                                    int hist, int nbw) {
+  // workgroups 0 .. nbw-1 loop over the batches of 64 band cells (the
+  // weights are copied to LDS once per workgroup); the rest clear the rim
   const int nbb = (nband + 4 * MC - 1) / (4 * MC);
-  if ((int)blockIdx.x >= nbb) {  // rim blocks
-    const int i = ((int)blockIdx.x - nbb) * 256 + (int)threadIdx.x;
+  if ((int)blockIdx.x >= nbw) {  // rim blocks
+    const int i = ((int)blockIdx.x - nbw) * 256 + (int)threadIdx.x;
     if (i < nrim) {
       const LCell c = cells[nband + i];
       const ShotDev& S = shots[c.shot];
@@ -433,10 +435,11 @@ __global__ __launch_bounds__(256) void k_infer_mfma(const unsigned char* __restr
   const float* sW2 = reinterpret_cast<const float*>(sWp + PW1);
   const float* sW3 = sW2 + H * H;
   const float* sB = sW3 + H * NO;
-
+  __syncthreads();
+  for (int bb = (int)blockIdx.x; bb < nbb; bb += nbw) {
   // features: 4 lanes per cell
   const int cl = lane >> 2, part = lane & 3;
-  const int c = (int)blockIdx.x * 4 * MC + wv * MC + cl;
+  const int c = bb * 4 * MC + wv * MC + cl;
   const bool live = c < nband;  // uniform over the cell's 4 lanes
   LCell cc = live ? cells[c] : LCell{0, 0, 0, 0};
   const ShotDev& S = shots[cc.shot];
@@ -519,6 +522,8 @@ __global__ __launch_bounds__(256) void k_infer_mfma(const unsigned char* __restr
       if (h && o < 2) h[o * S.n2 + gi] += d;
     }
   }
+  __syncthreads();  // sX and sH are rewritten by the next batch
+  }
 }
 
 // Multiplicative noise on the band cells of one level (training
@@ -545,13 +550,20 @@ void labc_cell_table(const std::vector<std::pair<int, int>>& grids, std::vector<
                      int* nrim) {
   using namespace labc;
   cells.clear();
+  // The inference kernel gives 16 consecutive cells to a wave, which loads
+  // their patches feature by feature: top and bottom band cells are listed
+  // column by column, the 8 depths of a column in a row (two columns per
+  // wave, one cache line each per feature), left and right band cells row
+  // by row at each depth (16 consecutive rows per wave).
   for (int s = 0; s < (int)grids.size(); ++s) {
-    const Band b = band_of(grids[s].first, grids[s].second);
-    for (int c = 0; c < b.total; ++c) {
-      int iz, ix, edge, depth;
-      cell_coords(b, c, iz, ix, edge, depth);
-      cells.push_back(LCell{s, iz, ix, edge | (depth << 2)});
-    }
+    const int nz = grids[s].first, nx = grids[s].second;
+    for (int edge = 0; edge < 2; ++edge)
+      for (int ix = R; ix < nx - R; ++ix)
+        for (int d = 0; d < W; ++d) cells.push_back(LCell{s, edge == 0 ? R + d : nz - R - 1 - d, ix, edge | (d << 2)});
+    for (int edge = 2; edge < 4; ++edge)
+      for (int d = 0; d < W; ++d)
+        for (int iz = R + W; iz < nz - R - W; ++iz)
+          cells.push_back(LCell{s, iz, edge == 2 ? R + d : nx - R - 1 - d, edge | (d << 2)});
   }
   *nband = (int)cells.size();
   for (int s = 0; s < (int)grids.size(); ++s) {
@@ -566,9 +578,21 @@ void labc_cell_table(const std::vector<std::pair<int, int>>& grids, std::vector<
 int launch_labc_infer(const unsigned char* w, const ShotDev* shots, const LCell* cells, int nband, int nrim, int L,
                       bool hist, hipStream_t st) {
   using namespace labc;
-  const int blocks = (nband + 4 * MC - 1) / (4 * MC) + (nrim + 255) / 256;
+  // band workgroups: one round of resident workgroups, each looping over
+  // batches of 64 cells
+  static int resident = 0;
+  if (resident == 0) {
+    int dev = 0, cus = 0, per_cu = 0;
+    HIPCHK(hipGetDevice(&dev));
+    HIPCHK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_infer_mfma, 256, 0));
+    resident = std::max(1, cus * per_cu);
+  }
+  const int nbb = (nband + 4 * MC - 1) / (4 * MC);
+  const int nbw = std::min(nbb, resident);
+  const int blocks = nbw + (nrim + 255) / 256;
   if (blocks == 0) return OK;
-  hipLaunchKernelGGL(k_infer_mfma, dim3(blocks), dim3(256), 0, st, w, shots, cells, nband, nrim, L, (int)hist);
+  hipLaunchKernelGGL(k_infer_mfma, dim3(blocks), dim3(256), 0, st, w, shots, cells, nband, nrim, L, (int)hist, nbw);
   HIPCHK(hipGetLastError());
   return OK;
 }

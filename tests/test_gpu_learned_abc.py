@@ -296,3 +296,24 @@ def test_learned_experiment_small(L, hip):
     for grp in ("reflection_rel_l2", "receiver_rel_l2", "image_rel_l2_vs_wide"):
         for k, v in out[grp].items():
             assert v is not None and np.isfinite(v), (grp, k)
+
+
+def test_learned_layer_400x800_against_both_sponges(L, hip):
+    """The learned layer at the 400 x 800 size: 2000 teacher-forced snapshot
+    pairs from runs with a 120-point sponge, held-out models, every boundary
+    against the 120-point run. The layer must cut the forward reflection of
+    its own 8-point band sponge by at least 1.3x (1.57x measured) and leave
+    every image finite; the 40-point sponge (45 cells against 13) stays
+    ahead of it on reflection, receiver data and images, and the check
+    states so, so that a layer that closed the gap would show up here."""
+    out, net = L.experiment(nz=400, nx=800, nt=2500)
+    net.close()
+    r, d, im = out["reflection_rel_l2"], out["receiver_rel_l2"], out["image_rel_l2_vs_wide"]
+    print("reflection", r, "receiver", d, "image", im, "ms/level", out["ms_per_forward_level"], flush=True)
+    assert out["training"]["snapshot_pairs"] == 2000
+    assert r["band_sponge_net"] * 1.3 <= r["band_sponge"]
+    assert d["band_sponge_net"] < d["band_sponge"]
+    for grp in (r, d, im):
+        for v in grp.values():
+            assert v is not None and np.isfinite(v) and v > 0
+        assert grp["sponge40"] < grp["band_sponge_net"]
