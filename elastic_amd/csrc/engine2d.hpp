@@ -71,11 +71,12 @@ struct Timing {
 constexpr int kEngineRing = 4;
 
 // Learned absorbing layer (learned_abc.hip): cell table entry of the
-// inference launch -- a band cell (ed = edge | depth << 2) or a rim cell
-// (ed = -1) of grid `shot`.
+// inference launch -- a band cell (ed = edge | depth << 2), a padding entry
+// (ed = kLDead) or a rim cell (ed = -1) of grid `shot`.
 struct LCell {
   int shot, iz, ix, ed;
 };
+constexpr int kLDead = 32;
 // Band cells, then rim cells, of the grids (nz, nx) listed.
 void labc_cell_table(const std::vector<std::pair<int, int>>& grids, std::vector<LCell>& cells, int* nband,
                      int* nrim);

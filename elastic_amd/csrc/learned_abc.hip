@@ -371,6 +371,14 @@ __device__ __forceinline__ __bf16 to_bf16(float x) {  // x already bf16-represen
   return __builtin_bit_cast(__bf16, (unsigned short)(__float_as_uint(x) >> 16));
 }
 
+// Global (not flat) load / store at an element offset.
+__device__ __forceinline__ float gldv(const float* p, int i) {
+  return ((const __attribute__((address_space(1))) float*)p)[i];
+}
+__device__ __forceinline__ void gstv(float* p, int i, float v) {
+  ((__attribute__((address_space(1))) float*)p)[i] = v;
+}
+
 struct GridView {
   const float* l1;  // level L-1, field f at + f*fs
   const float* l2;  // level L-2
@@ -440,14 +448,37 @@ __global__ __launch_bounds__(256) void k_infer_mfma(const unsigned char* __restr
   // features: 4 lanes per cell
   const int cl = lane >> 2, part = lane & 3;
   const int c = bb * 4 * MC + wv * MC + cl;
-  const bool live = c < nband;  // uniform over the cell's 4 lanes
-  LCell cc = live ? cells[c] : LCell{0, 0, 0, 0};
-  const ShotDev& S = shots[cc.shot];
-  const int edge = cc.ed & 3, depth = cc.ed >> 2;
-  const GridView g{lev_ptr(S.lev, S.lev_stride, S.lev_mod, S.lev_off, L - 1),
-                   lev_ptr(S.lev, S.lev_stride, S.lev_mod, S.lev_off, L - 2), S.fstride, S.ldz, S.nz, S.nx};
+  const bool in = c < nband;  // uniform over the cell's 4 lanes
+  LCell cc = in ? cells[c] : LCell{0, R, R, kLDead};
+  const bool live = in && cc.ed != kLDead;
+  // a wave's 16 cells belong to one grid (the table pads every grid to
+  // whole waves), so its descriptor comes from scalar loads
+  const ShotDev& S = shots[__builtin_amdgcn_readfirstlane(cc.shot)];
+  const int edge = cc.ed & 3, depth = (cc.ed >> 2) & 7;
+  const float* l1 = lev_ptr(S.lev, S.lev_stride, S.lev_mod, S.lev_off, L - 1);
+  const float* l2 = lev_ptr(S.lev, S.lev_stride, S.lev_mod, S.lev_off, L - 2);
+  // patch offsets: the cell, one step inward (normal) and one step along
+  // the edge (tangent); every patch lies inside the grid (the cell table
+  // admits only grids of at least 2 (R + W) + 1 cells a side), and the
+  // engine's field offsets fit in 32 bits
+  const int fs = (int)S.fstride, ldz = S.ldz;
   const int dz = edge == 0 ? 1 : edge == 1 ? -1 : 0, dx = edge == 2 ? 1 : edge == 3 ? -1 : 0;
-  const int tz = edge >= 2 ? 1 : 0, tx = edge < 2 ? 1 : 0;
+  const int nstep = dz + ldz * dx, tstep = edge >= 2 ? 1 : ldz;
+  const int base = live ? cc.iz + ldz * cc.ix : 2 * tstep;  // dead lanes read near element 0
+  // the cell's coefficients and its outputs (level L, the Vx, Vz history),
+  // loaded with the features
+  float cp = 0.f, cs = 0.f, zp = 0.f;
+  Mat{S.coef, (int)S.n2, S.ldz}.at(cc.iz, cc.ix, cp, cs, zp);
+  float* const out = lev_ptr(S.lev, S.lev_stride, S.lev_mod, S.lev_off, L);
+  float* const hh = hist ? static_cast<float*>(S.snap) + (long long)(L - 1) * 2 * S.n2 : nullptr;
+  const int gi0 = cc.iz + ldz * cc.ix;
+  float ov[2], hv[2];
+#pragma unroll
+  for (int q = 0; q < 2; ++q) {
+    const int o = part + 4 * q;
+    ov[q] = o < NO ? gldv(out, o * fs + gi0) : 0.f;
+    hv[q] = (hh && o < 2) ? gldv(hh, (int)S.n2 * o + gi0) : 0.f;
+  }
   constexpr int NV = (NRAW + 3) / 4;  // values per lane
   float rv[NV];
   float m = 0.f;
@@ -455,12 +486,16 @@ __global__ __launch_bounds__(256) void k_infer_mfma(const unsigned char* __restr
   for (int u = 0; u < NV; ++u) {
     const int k = part + 4 * u;
     float v = 0.f;
-    if (live && k < 225) {
-      const int kn = k / 25, rem = k - 25 * kn, mm = rem / 5 - 2, f = rem - 5 * (rem / 5);
-      v = g.get(g.l1, f, cc.iz + kn * dz + mm * tz, cc.ix + kn * dx + mm * tx);
-    } else if (live && k < NRAW) {
-      v = g.get(g.l2, k - 225, cc.iz, cc.ix);
+    if (k < 225) {  // field f of level L-1 at (kn inward, mm - 2 along the edge)
+      const int kn = k / 25, rem = k - 25 * kn, mi = rem / 5, f = rem - 5 * mi;
+      v = gldv(l1, f * fs + base + kn * nstep + (mi - 2) * tstep);
+      if (f >= 2) v = v * (1.f / Z0);
+    } else if (k < NRAW) {  // the cell's own fields of level L-2
+      const int f = k - 225;
+      v = gldv(l2, f * fs + base);
+      if (f >= 2) v = v * (1.f / Z0);
     }
+    v = live ? v : 0.f;
     rv[u] = v;
     m = fmaxf(m, fabsf(v));
   }
@@ -472,8 +507,6 @@ __global__ __launch_bounds__(256) void k_infer_mfma(const unsigned char* __restr
     const int k = part + 4 * u;
     if (k < NRAW) sX[wv][cl][k] = to_bf16(bf16r(rv[u] * inv));
   }
-  float cp = 0.f, cs = 0.f, zp = 0.f;
-  if (live) Mat{S.coef, (int)S.n2, S.ldz}.at(cc.iz, cc.ix, cp, cs, zp);
   for (int k = NRAW + part; k < KX; k += 4)
     sX[wv][cl][k] = to_bf16(bf16r(live ? static_feature(edge, depth, cp, cs, zp, k) : 0.f));
   __syncthreads();
@@ -513,13 +546,14 @@ __global__ __launch_bounds__(256) void k_infer_mfma(const unsigned char* __restr
     y[o] += __shfl_xor(y[o], 2, 64);
   }
   if (live && m > 1e-30f) {
-    float* out = lev_ptr(S.lev, S.lev_stride, S.lev_mod, S.lev_off, L);
-    const long long gi = cc.iz + (long long)S.ldz * cc.ix;
-    float* const h = hist ? static_cast<float*>(S.snap) + (long long)(L - 1) * 2 * S.n2 : nullptr;
-    for (int o = part; o < NO; o += 4) {
-      const float d = (y[o] + sB[2 * H + o]) * m * (o >= 2 ? Z0 : 1.f);
-      out[o * S.fstride + gi] += d;
-      if (h && o < 2) h[o * S.n2 + gi] += d;
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+      const int o = part + 4 * q;
+      if (o < NO) {
+        const float d = (y[o] + sB[2 * H + o]) * m * (o >= 2 ? Z0 : 1.f);
+        gstv(out, o * fs + gi0, ov[q] + d);
+        if (hh && o < 2) gstv(hh, (int)S.n2 * o + gi0, hv[q] + d);
+      }
     }
   }
   __syncthreads();  // sX and sH are rewritten by the next batch
@@ -557,6 +591,13 @@ void labc_cell_table(const std::vector<std::pair<int, int>>& grids, std::vector<
   // by row at each depth (16 consecutive rows per wave).
   for (int s = 0; s < (int)grids.size(); ++s) {
     const int nz = grids[s].first, nx = grids[s].second;
+    // every feature patch of a band cell lies inside the grid from this size
+    // on (the inference kernel loads the patches without bounds checks)
+    if (nz < 2 * (R + W) + 1 || nx < 2 * (R + W) + 1) {
+      cells.clear();
+      *nband = *nrim = 0;
+      return;
+    }
     for (int edge = 0; edge < 2; ++edge)
       for (int ix = R; ix < nx - R; ++ix)
         for (int d = 0; d < W; ++d) cells.push_back(LCell{s, edge == 0 ? R + d : nz - R - 1 - d, ix, edge | (d << 2)});
@@ -564,6 +605,9 @@ void labc_cell_table(const std::vector<std::pair<int, int>>& grids, std::vector<
       for (int d = 0; d < W; ++d)
         for (int iz = R + W; iz < nz - R - W; ++iz)
           cells.push_back(LCell{s, iz, edge == 2 ? R + d : nx - R - 1 - d, edge | (d << 2)});
+    // whole waves per grid, so that a wave's grid is the same in every lane:
+    // padding entries (ed = kLDead) are evaluated at a valid cell and discarded
+    while (cells.size() % 16) cells.push_back(LCell{s, R, R, kLDead});
   }
   *nband = (int)cells.size();
   for (int s = 0; s < (int)grids.size(); ++s) {
