@@ -394,7 +394,7 @@ int Engine::build_lcells() {
     std::vector<LCell> cells;
     labc_cell_table(grids, cells, &g.nlband, &g.nlrim);
     if (g.nlband == 0) {
-      set_error("learned layer: a grid of this group is too small for the 8-cell band");
+      set_error("learned layer: a grid of this group is smaller than 27 cells on a side (rim + band + patch)");
       return E_ARG;
     }
     g.lcell0 = (int)all.size();

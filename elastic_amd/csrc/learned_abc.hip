@@ -527,16 +527,34 @@ __global__ __launch_bounds__(256) void k_infer_mfma(const unsigned char* __restr
     for (int i = 0; i < 4; ++i) sH[wv][(lane >> 4) * 4 + i][nt * 16 + r16] = gelu(acc[i] + sB[nt * 16 + r16]);
   }
   __syncthreads();
-  // layer 2 (FP32): hidden units part*8 .. part*8+7 of cell cl; layer 3
-  // partial sums over them, reduced over the cell's 4 lanes
+  // layer 2 on FP32 MFMA: Z2[16 x 32] = H1[16 x 32] W2[32 x 32] in 8 steps
+  // of 4 (lane: A row r16, B column r16, k = lane / 16); the wave's own
+  // rows of sH are then overwritten with the hidden layer 2 (its LDS
+  // operations run in program order, so every read above precedes them)
+  {
+    const int kr = lane >> 4;
+    f32x4 z[2];
+#pragma unroll
+    for (int nt = 0; nt < 2; ++nt) {
+      f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+      for (int kk = 0; kk < H / 4; ++kk)
+        acc = __builtin_amdgcn_mfma_f32_16x16x4f32(sH[wv][r16][kk * 4 + kr], sW2[(kk * 4 + kr) * H + nt * 16 + r16],
+                                                   acc, 0, 0, 0);
+      z[nt] = acc;
+    }
+#pragma unroll
+    for (int nt = 0; nt < 2; ++nt)
+#pragma unroll
+      for (int i = 0; i < 4; ++i) sH[wv][kr * 4 + i][nt * 16 + r16] = gelu(z[nt][i] + sB[H + nt * 16 + r16]);
+  }
+  // layer 3: partial sums over hidden units part*8 .. part*8+7 of cell cl,
+  // reduced over the cell's 4 lanes
   float y[NO] = {0.f, 0.f, 0.f, 0.f, 0.f};
 #pragma unroll
   for (int u = 0; u < 8; ++u) {
     const int j = part * 8 + u;
-    float a = sB[H + j];
-#pragma unroll 8
-    for (int k = 0; k < H; ++k) a = fmaf(sH[wv][cl][k], sW2[k * H + j], a);
-    const float h2 = gelu(a);
+    const float h2 = sH[wv][cl][j];
 #pragma unroll
     for (int o = 0; o < NO; ++o) y[o] = fmaf(h2, sW3[j * NO + o], y[o]);
   }

@@ -106,7 +106,8 @@ int labc_eval(labc_net *net, labc_data *d, float *loss);
 /* One forward propagation of a band grid with the learned layer (net) and
  * the sponge s->abc (0 = none; both may act together), or, with
  * net = NULL, with the sponge alone: records receivers like rtm_forward_hip
- * and returns the last two levels. */
+ * and returns the last two levels. With a network the grid must be at least
+ * 2 * (LABC_RIM + LABC_WIDTH) + 1 cells on each side (RTM_E_ARG otherwise). */
 int labc_forward(labc_net *net, const rtm_shot_t *s, const float *coef5, const float *wavelet,
                  float *rec, float *last);
 /* Device time of the last labc_forward time loop (ms). */
@@ -120,7 +121,9 @@ double labc_last_forward_ms(void);
  * cores. The shots' grids should be padded by LABC_RIM + LABC_WIDTH cells
  * and the batch's sponge width (cfg abc) be the band sponge the network was
  * trained with; the adjoint keeps the batch's adjoint sponge option. NULL
- * drops the layer. The network must outlive the batch's runs. */
+ * drops the layer. The network must outlive the batch's runs. Every grid
+ * must be at least 2 * (LABC_RIM + LABC_WIDTH) + 1 cells on each side
+ * (RTM_E_ARG otherwise). */
 int rtm_batch_set_learned(rtm_batch *b, labc_net *net);
 
 #ifdef __cplusplus
