@@ -432,7 +432,8 @@ def survey_with_layer(args, net):
     nz, nx, nshots, nt, dx, dt, fp = args.nz, args.nx, args.shots, args.nt, 10.0, 8e-4, 15.0
     res = {}
     for key, abc, use_net in (("sponge40", 40, False), ("band_sponge", 8, False), ("band_sponge_net", 8, True)):
-        b, specs, _, _, _ = build_batch(lib, synth, nz, nx, nshots, nt, dx, dt, fp, abc, "bf16", args.seed)
+        b, specs, _, _, _ = build_batch(lib, synth, nz, nx, nshots, nt, dx, dt, fp, abc, "bf16", args.seed,
+                                        graph=not args.no_graph)
         try:
             if use_net:
                 b.set_learned(net)
@@ -452,6 +453,45 @@ def survey_with_layer(args, net):
             b.close()
     res["net_over_sponge40"] = round(res["band_sponge_net"]["shots_per_hour"] / res["sponge40"]["shots_per_hour"], 3)
     return res
+
+
+def learned_cost(args):
+    """Inference cost of the learned layer: the first --shots shots of the
+    survey (one BF16 group, 13-cell band grids, 8-point band sponge, --nt
+    levels; a network with random weights, which costs what a trained one
+    does), one migration after the data synthesis with and without the
+    layer, host-launched levels. Prints its own JSON line."""
+    from elastic_amd import learned as LRN
+    from elastic_amd import lib, synth
+
+    net = LRN.Net(seed=123)
+    res = {}
+    try:
+        for key, use_net in (("band_sponge", False), ("band_sponge_net", True)):
+            b, specs, _, _, _ = build_batch(lib, synth, args.nz, args.nx, args.shots, args.nt, 10.0, 8e-4, 15.0, 8,
+                                            "bf16", args.seed, graph=False)
+            try:
+                if use_net:
+                    b.set_learned(net)
+                b.synthesize()
+                lib.synchronize()
+                b.migrate()
+                lib.synchronize()
+                info = b.info()
+                res[key] = {"fwd_ms": round(info["fwd_ms"], 2), "adj_ms": round(info["adj_ms"], 2),
+                            "fwd_launches": int(info["fwd_launches"]), "group_size": int(info["group_size"])}
+            finally:
+                b.close()
+    finally:
+        net.close()
+    per = (res["band_sponge_net"]["fwd_ms"] - res["band_sponge"]["fwd_ms"]) / res["band_sponge_net"]["fwd_launches"]
+    line = {"metric": "learned-layer inference, ms per launch (forward-phase time with the layer minus without, "
+                      "per level launch)", "value": round(per, 4), "unit": "ms", "n_gpus": 1, "steps": 1,
+            "higher_is_better": False, "dtype": "fp32", "data": "synthetic (the survey's first shots)",
+            "config": {"workload": f"learned-layer cost, {args.shots} shots of the {args.nz}x{args.nx} survey, "
+                                   f"{args.nt} levels, BF16 snapshots every 4th level, 13-cell band grids"},
+            "results": res}
+    print(json.dumps(line), flush=True)
 
 
 def learned(args):
@@ -502,10 +542,11 @@ def main():
                     help="launch every level from the host instead of replaying HIP graphs (same kernels; "
                          "for tracing tools that do not follow graph launches)")
     ap.add_argument("--workload", default="elastic2d",
-                    choices=("elastic2d", "acoustic3d", "elastic3d", "probe", "learned"),
+                    choices=("elastic2d", "acoustic3d", "elastic3d", "probe", "learned", "learned-cost"),
                     help="acoustic3d: the 3D acoustic subdomain; elastic3d: the 9-field 3D elastic adjoint "
                          "step; probe: the streaming-kernel bandwidth probe; learned: training and evaluation "
-                         "of the learned absorbing layer; all instead of the headline 2D workload")
+                         "of the learned absorbing layer; learned-cost: the layer's inference cost on one BF16 "
+                         "group of the survey; all instead of the headline 2D workload")
     ap.add_argument("--n3l", type=int, nargs=2, default=(400, 800))
     ap.add_argument("--nt3l", type=int, default=2500)
     ap.add_argument("--labc-steps", type=int, nargs=3, default=(4000, 2000, 2000))
@@ -539,6 +580,9 @@ def main():
         return
     if args.workload == "learned":
         learned(args)
+        return
+    if args.workload == "learned-cost":
+        learned_cost(args)
         return
     dx, dt, fp, abc = 10.0, 8e-4, 15.0, 40
     log(f"bench: {args.shots}-shot survey {args.nz}x{args.nx}, nt={args.nt}, variant={args.variant}")

@@ -642,14 +642,17 @@ int launch_labc_infer(const unsigned char* w, const ShotDev* shots, const LCell*
   using namespace labc;
   // band workgroups: one round of resident workgroups, each looping over
   // batches of 64 cells
-  static int resident = 0;
-  if (resident == 0) {
+  // (queried once per process, thread-safe; a failed query falls back to
+  // 2 workgroups on each of 256 CUs, which only changes how the batches are
+  // spread)
+  static const int resident = [] {
     int dev = 0, cus = 0, per_cu = 0;
-    HIPCHK(hipGetDevice(&dev));
-    HIPCHK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
-    HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_infer_mfma, 256, 0));
-    resident = std::max(1, cus * per_cu);
-  }
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_infer_mfma, 256, 0) != hipSuccess || cus * per_cu <= 0)
+      return 512;
+    return cus * per_cu;
+  }();
   const int nbb = (nband + 4 * MC - 1) / (4 * MC);
   const int nbw = std::min(nbb, resident);
   const int blocks = nbw + (nrim + 255) / 256;
