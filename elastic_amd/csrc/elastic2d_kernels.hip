@@ -1744,7 +1744,7 @@ __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(4))) vo
 // XIN: the drop-in form (the level's own content added first, no imaging
 // or injection), a separate instance: the engine's instance carries neither
 // its loads nor their registers.
-template <int SNAP, int DIR, int NW, bool XIN, bool COMP>
+template <int SNAP, int DIR, int NW, bool XIN, bool COMP, bool PROP>
 __device__ __forceinline__ void adjm_strip(const ShotDev& S, const TileRef& tr, int L, int flags, int xw,
                                            float (*col)[6][MarchCol<NW>::MZ], float (*hring)[6][10]) {
   const int nz = S.nz, nx = S.nx, ldz = S.ldz;
@@ -1758,8 +1758,9 @@ __device__ __forceinline__ void adjm_strip(const ShotDev& S, const TileRef& tr, 
   const int ncol = xe - xb, xs = DIR > 0 ? xb : xe - 1;  // columns, first column of the march
   const MarchLane<NW> m(z0, nz);
   const int gz = m.gz;
-  // the drop-in form has no imaging, injection or sponge
-  const bool imaging = !XIN && (flags & A_IMG);
+  // the drop-in form has no imaging, injection or sponge; PROP: the
+  // instance for levels that are not imaged (no imaging code or registers)
+  const bool imaging = !XIN && !PROP && (flags & A_IMG);
   const bool next = flags & A_NEXT;
   constexpr bool comp = !XIN && COMP;  // compensated image sums (A_COMP), an instance of its own
   constexpr bool xin = XIN;
@@ -2044,7 +2045,7 @@ __device__ __forceinline__ void adjm_strip(const ShotDev& S, const TileRef& tr, 
   }
 }
 
-template <int SNAP, int NW, bool XIN, bool COMP>
+template <int SNAP, int NW, bool XIN, bool COMP, bool PROP = false>
 __device__ __forceinline__ void adjm_kernel(const ShotDev* __restrict__ shots, const TileRef* __restrict__ strips,
                                             int nstrips, int L, int flags, int xw, int xcd) {
   __shared__ float col[2][6][MarchCol<NW>::MZ];  // P1, P2, P3, P4, source Vx, source Vz of the centre column
@@ -2052,9 +2053,9 @@ __device__ __forceinline__ void adjm_kernel(const ShotDev* __restrict__ shots, c
   const TileRef tr = strips[xcd ? xcd_tile(blockIdx.x, nstrips) : (int)blockIdx.x];
   // neighbouring strips march in opposite directions (see k_fwdm)
   if (((tr.x0 / xw) & 1) == 0)
-    adjm_strip<SNAP, 1, NW, XIN, COMP>(shots[tr.shot], tr, L, flags, xw, col, hring);
+    adjm_strip<SNAP, 1, NW, XIN, COMP, PROP>(shots[tr.shot], tr, L, flags, xw, col, hring);
   else
-    adjm_strip<SNAP, -1, NW, XIN, COMP>(shots[tr.shot], tr, L, flags, xw, col, hring);
+    adjm_strip<SNAP, -1, NW, XIN, COMP, PROP>(shots[tr.shot], tr, L, flags, xw, col, hring);
 }
 // fp32 snapshot formats: held to 168 VGPRs (3 waves per SIMD; a few spills,
 // measured 9% faster than 2 waves without them)
@@ -2070,6 +2071,14 @@ __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(3))) vo
     const ShotDev* __restrict__ shots, const TileRef* __restrict__ strips, int nstrips, int L, int flags, int xw,
     int xcd) {
   adjm_kernel<SNAP_HIST, NW, true, false>(shots, strips, nstrips, L, flags, xw, xcd);
+}
+// levels that are not imaged (decimated snapshots): propagation, injection
+// and sponge only, held to 128 VGPRs (4 waves per SIMD)
+template <int NW>
+__global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(4))) void k_adjm_prop(
+    const ShotDev* __restrict__ shots, const TileRef* __restrict__ strips, int nstrips, int L, int flags, int xw,
+    int xcd) {
+  adjm_kernel<SNAP_HIST, NW, false, false, true>(shots, strips, nstrips, L, flags, xw, xcd);
 }
 // 16-bit snapshot formats: their decode needs more registers; held to 168
 // (3 waves per SIMD, at most 3 spills), measured 7% faster on the BF16
@@ -2283,6 +2292,10 @@ static void adjm_dispatch(int snap, dim3 g, hipStream_t st, const ShotDev* s, co
   const dim3 b(TILE_Z * NW);
   if (flags & A_XIN) {  // drop-in form (fp32 histories, checked by the launcher)
     hipLaunchKernelGGL((k_adjm_xin<NW>), g, b, 0, st, s, t, n, L, flags, xw, xcd);
+    return;
+  }
+  if (!(flags & A_IMG)) {  // a level that is not imaged: any snapshot format
+    hipLaunchKernelGGL((k_adjm_prop<NW>), g, b, 0, st, s, t, n, L, flags, xw, xcd);
     return;
   }
   if (flags & A_COMP) {
