@@ -422,21 +422,26 @@ def run_variant(lib, synth, args, variant, steps, warmup, data=None, graph=True)
 
 def survey_with_layer(args, net):
     """The batched BF16 survey (64 shots, 800 x 2400, 6000 levels, BF16
-    snapshots every 4th level, compensated images) timed with three
+    snapshots every 4th level, compensated images) timed with four
     boundaries: the 40-point sponge (45 padding cells), the 8-point band
-    sponge alone and with the learned layer (13 cells; the network is the
-    one trained at 400 x 800, so this measures cost, not image quality).
-    One timed migration each, after the data synthesis."""
+    sponge alone, with the learned layer in the forward runs, and with the
+    layer in the forward and the adjoint runs (13 cells; the network is the
+    one trained at 400 x 800, used for both directions here: this measures
+    cost, not image quality). One timed migration each, after the data
+    synthesis."""
     from elastic_amd import lib, synth
 
     nz, nx, nshots, nt, dx, dt, fp = args.nz, args.nx, args.shots, args.nt, 10.0, 8e-4, 15.0
     res = {}
-    for key, abc, use_net in (("sponge40", 40, False), ("band_sponge", 8, False), ("band_sponge_net", 8, True)):
+    for key, abc, use_net in (("sponge40", 40, 0), ("band_sponge", 8, 0), ("band_sponge_net", 8, 1),
+                              ("band_sponge_net_both", 8, 2)):
         b, specs, _, _, _ = build_batch(lib, synth, nz, nx, nshots, nt, dx, dt, fp, abc, "bf16", args.seed,
                                         graph=not args.no_graph)
         try:
             if use_net:
                 b.set_learned(net)
+            if use_net == 2:
+                b.set_learned_adjoint(net)
             b.synthesize()
             lib.synchronize()
             t0 = time.perf_counter()
@@ -452,6 +457,8 @@ def survey_with_layer(args, net):
         finally:
             b.close()
     res["net_over_sponge40"] = round(res["band_sponge_net"]["shots_per_hour"] / res["sponge40"]["shots_per_hour"], 3)
+    res["net_both_over_sponge40"] = round(res["band_sponge_net_both"]["shots_per_hour"] /
+                                          res["sponge40"]["shots_per_hour"], 3)
     return res
 
 
@@ -459,32 +466,38 @@ def learned_cost(args):
     """Inference cost of the learned layer: the first --shots shots of the
     survey (one BF16 group, 13-cell band grids, 8-point band sponge, --nt
     levels; a network with random weights, which costs what a trained one
-    does), one migration after the data synthesis with and without the
-    layer, host-launched levels. Prints its own JSON line."""
+    does), one migration after the data synthesis without the layer, with
+    it in the forward runs and with it in both directions, host-launched
+    levels. Prints its own JSON line."""
     from elastic_amd import learned as LRN
     from elastic_amd import lib, synth
 
     net = LRN.Net(seed=123)
     res = {}
     try:
-        for key, use_net in (("band_sponge", False), ("band_sponge_net", True)):
+        for key, use_net in (("band_sponge", 0), ("band_sponge_net", 1), ("band_sponge_net_both", 2)):
             b, specs, _, _, _ = build_batch(lib, synth, args.nz, args.nx, args.shots, args.nt, 10.0, 8e-4, 15.0, 8,
                                             "bf16", args.seed, graph=False)
             try:
                 if use_net:
                     b.set_learned(net)
+                if use_net == 2:
+                    b.set_learned_adjoint(net)
                 b.synthesize()
                 lib.synchronize()
                 b.migrate()
                 lib.synchronize()
                 info = b.info()
                 res[key] = {"fwd_ms": round(info["fwd_ms"], 2), "adj_ms": round(info["adj_ms"], 2),
-                            "fwd_launches": int(info["fwd_launches"]), "group_size": int(info["group_size"])}
+                            "fwd_launches": int(info["fwd_launches"]), "adj_launches": int(info["adj_launches"]),
+                            "group_size": int(info["group_size"])}
             finally:
                 b.close()
     finally:
         net.close()
     per = (res["band_sponge_net"]["fwd_ms"] - res["band_sponge"]["fwd_ms"]) / res["band_sponge_net"]["fwd_launches"]
+    res["adjoint_ms_per_launch"] = round((res["band_sponge_net_both"]["adj_ms"] - res["band_sponge"]["adj_ms"]) /
+                                         res["band_sponge_net_both"]["adj_launches"], 4)
     line = {"metric": "learned-layer inference, ms per launch (forward-phase time with the layer minus without, "
                       "per level launch)", "value": round(per, 4), "unit": "ms", "n_gpus": 1, "steps": 1,
             "higher_is_better": False, "dtype": "fp32", "data": "synthetic (the survey's first shots)",

@@ -386,7 +386,7 @@ int Engine::build_groups() {
 // Band and rim cells of every group's shots (learned layer), one table.
 int Engine::build_lcells() {
   d_lcells_.release();
-  if (!cfg_.labc_w) return OK;
+  if (!cfg_.labc_w && !cfg_.labc_wa) return OK;
   std::vector<LCell> all;
   for (Group& g : groups_) {
     std::vector<std::pair<int, int>> grids;
@@ -408,6 +408,16 @@ int Engine::build_lcells() {
 int Engine::set_learned(const unsigned char* w) {
   destroy_graphs();  // their forward loops change
   cfg_.labc_w = w;
+  return build_lcells();
+}
+
+int Engine::set_learned_adjoint(const unsigned char* w) {
+  if (w && cfg_.adj_abc == 2) {
+    set_error("learned adjoint layer: not with the R/Src adjoint sponge");
+    return E_ARG;
+  }
+  destroy_graphs();  // their adjoint loops change
+  cfg_.labc_wa = w;
   return build_lcells();
 }
 
@@ -548,8 +558,10 @@ int Engine::enqueue_adjoint(int g, hipStream_t st) {
     STCHK(launch_fill(r.img.as<float>(), 3 * r.n2, 0.f, st));
     if (cfg_.comp) STCHK(launch_fill(r.imgc.as<float>(), 3 * r.n2, 0.f, st));
   }
-  // the two-level adjoint has no R/Src sponge form: one level per launch
-  const bool pairs = cfg_.tblock == 1 && cfg_.txr <= 4 && cfg_.adj_abc != 2;
+  // the two-level adjoint has no R/Src sponge form, and the learned layer
+  // corrects every level: one level per launch
+  const bool pairs = cfg_.tblock == 1 && cfg_.txr <= 4 && cfg_.adj_abc != 2 && !cfg_.labc_wa;
+  const LCell* lcells = cfg_.labc_wa ? d_lcells_.as<LCell>() + gr.lcell0 : nullptr;
   for (int L = cfg_.nt; L >= 1;) {
     const bool two = pairs && L - 1 >= 1;
     int fl = A_INJ;
@@ -566,6 +578,7 @@ int Engine::enqueue_adjoint(int g, hipStream_t st) {
       STCHK(launch_adjm(desc, strips, gr.nstrips, L, fl, gr.march_w, cfg_.march_nw, cfg_.snap_fmt, cfg_.xcd, st));
     else
       STCHK(launch_adj(desc, tiles, gr.ntiles, L, fl, cfg_.txr, cfg_.snap_fmt, cfg_.xcd, st));
+    if (lcells) STCHK(launch_labc_infer(cfg_.labc_wa, desc, lcells, gr.nlband, gr.nlrim, L, false, st, true));
     L -= two ? 2 : 1;
   }
   if (cfg_.comp)
@@ -586,7 +599,7 @@ long long Engine::fwd_launch_count(int mode) const {
   return n;
 }
 long long Engine::adj_launch_count() const {
-  const bool pairs = cfg_.tblock == 1 && cfg_.txr <= 4 && cfg_.adj_abc != 2;
+  const bool pairs = cfg_.tblock == 1 && cfg_.txr <= 4 && cfg_.adj_abc != 2 && !cfg_.labc_wa;
   long long n = 0;
   for (int L = cfg_.nt; L >= 1; ++n) L -= (pairs && L - 1 >= 1) ? 2 : 1;
   return n;

@@ -56,8 +56,14 @@ struct EngineCfg {
   // Learned absorbing layer: the network's packed inference weights on the
   // device (null = none). The forward runs then apply the network's correction to
   // the band cells of every level L >= 3 after its plain update (and clear
-  // the rim), one level per launch; the adjoint is left as configured.
+  // the rim), one level per launch; the adjoint has its own network
+  // (labc_wa).
   const unsigned char* labc_w = nullptr;
+  // The adjoint network (null = none): the adjoint runs then correct the
+  // band cells of every level after its step, one level per launch. The
+  // imaging of a level happens in its step, before the correction: only
+  // images of band cells (outside the physical region) miss it.
+  const unsigned char* labc_wa = nullptr;
 };
 
 struct Timing {
@@ -85,8 +91,10 @@ void labc_cell_table(const std::vector<std::pair<int, int>>& grids, std::vector<
 // weights).
 // With hist, the correction of Vx, Vz also goes into the fp32 source
 // history of level L (SNAP_HIST), which the step wrote before it.
+// With adj, after an adjoint step: the correction comes from levels L+1 and
+// L+2 of an adjoint state (hist ignored).
 int launch_labc_infer(const unsigned char* w, const ShotDev* shots, const LCell* cells, int nband, int nrim, int L,
-                      bool hist, hipStream_t st);
+                      bool hist, hipStream_t st, bool adj = false);
 
 // Strip-width fitting of the marching forms (see Engine::build_groups).
 bool march_fit();
@@ -116,6 +124,9 @@ class Engine {
   // Use (w = the network's packed inference weights on the device) or drop
   // (null) the learned layer in the forward runs.
   int set_learned(const unsigned char* w);
+  // The same for the adjoint runs (the network trained on adjoint states);
+  // not with the R/Src adjoint sponge.
+  int set_learned_adjoint(const unsigned char* w);
   const Timing& timing() const { return tm_; }
   // strip width of the first group's marching launches: the adjoint's for
   // form 2, the forward's for form 3 (cfg march_w when the form does not

@@ -10,6 +10,7 @@
 #include "engine2d.hpp"
 
 #include <algorithm>
+#include <functional>
 #include <cmath>
 #include <cstring>
 #include <memory>
@@ -91,13 +92,19 @@ __device__ __forceinline__ float gelu_d(float x) {
 }
 
 // Wavefield value of field f at (iz, ix) scaled to velocity units, 0 outside.
+// Fields divided by Z0 in the features, targets and corrections: the
+// stresses of a forward state, the velocities of an adjoint state (in each,
+// the larger pair, by about the impedance).
+__host__ __device__ inline bool z_scaled(int f, int adj) { return adj ? f < 2 : f >= 2; }
+
 struct Lev {
   const float* p;  // field f at p + f*fs
   int fs, ldz, nz, nx;
+  int adj;  // 1: an adjoint state
   __device__ __forceinline__ float get(int f, int iz, int ix) const {
     if (iz < 0 || iz >= nz || ix < 0 || ix >= nx) return 0.f;
     const float v = p[f * fs + iz + ldz * ix];
-    return f >= 2 ? v * (1.f / Z0) : v;
+    return z_scaled(f, adj) ? v * (1.f / Z0) : v;
   }
 };
 
@@ -162,7 +169,7 @@ __global__ void k_features(Lev m1, Lev m2, Mat mat, Band b, float* feat, float* 
 
 // Targets: (wide(L) - plain(L)) / (s * unit) on the band cells.
 __global__ void k_targets(const float* wide, int wfs, int wldz, int off, const float* plain, int pfs,
-                          int pldz, Band b, const float* scale, float* tgt) {
+                          int pldz, Band b, const float* scale, float* tgt, int adj) {
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= b.total) return;
   int iz, ix, edge, depth;
@@ -171,7 +178,7 @@ __global__ void k_targets(const float* wide, int wfs, int wldz, int off, const f
   for (int f = 0; f < NO; ++f) {
     const float w = wide[f * wfs + (iz + off) + wldz * (ix + off)];
     const float p = plain[f * pfs + iz + pldz * ix];
-    const float u = f >= 2 ? Z0 : 1.f;
+    const float u = z_scaled(f, adj) ? Z0 : 1.f;
     tgt[(size_t)c * NO + f] = s > 0.f ? (w - p) / (s * u) : 0.f;
   }
 }
@@ -416,6 +423,9 @@ __global__ void k_pack_infer(const float* __restrict__ w, unsigned char* __restr
   if (i < NO) f[H * H + H * NO + 2 * H + i] = w[OB3 + i];
 }
 
+// ADJ: an adjoint step's correction (features from levels L+1, L+2, the
+// adjoint's field scaling, no history).
+template <bool ADJ>
 __global__ __launch_bounds__(256) void k_infer_mfma(const unsigned char* __restrict__ packed,
                                                     const ShotDev* __restrict__ shots,
                                                     const LCell* __restrict__ cells, int nband, int nrim, int L,
@@ -455,8 +465,8 @@ __global__ __launch_bounds__(256) void k_infer_mfma(const unsigned char* __restr
   // whole waves), so its descriptor comes from scalar loads
   const ShotDev& S = shots[__builtin_amdgcn_readfirstlane(cc.shot)];
   const int edge = cc.ed & 3, depth = (cc.ed >> 2) & 7;
-  const float* l1 = lev_ptr(S.lev, S.lev_stride, S.lev_mod, S.lev_off, L - 1);
-  const float* l2 = lev_ptr(S.lev, S.lev_stride, S.lev_mod, S.lev_off, L - 2);
+  const float* l1 = lev_ptr(S.lev, S.lev_stride, S.lev_mod, S.lev_off, ADJ ? L + 1 : L - 1);
+  const float* l2 = lev_ptr(S.lev, S.lev_stride, S.lev_mod, S.lev_off, ADJ ? L + 2 : L - 2);
   // patch offsets: the cell, one step inward (normal) and one step along
   // the edge (tangent); every patch lies inside the grid (the cell table
   // admits only grids of at least 2 (R + W) + 1 cells a side), and the
@@ -470,7 +480,7 @@ __global__ __launch_bounds__(256) void k_infer_mfma(const unsigned char* __restr
   float cp = 0.f, cs = 0.f, zp = 0.f;
   Mat{S.coef, (int)S.n2, S.ldz}.at(cc.iz, cc.ix, cp, cs, zp);
   float* const out = lev_ptr(S.lev, S.lev_stride, S.lev_mod, S.lev_off, L);
-  float* const hh = hist ? static_cast<float*>(S.snap) + (long long)(L - 1) * 2 * S.n2 : nullptr;
+  float* const hh = !ADJ && hist ? static_cast<float*>(S.snap) + (long long)(L - 1) * 2 * S.n2 : nullptr;
   const int gi0 = cc.iz + ldz * cc.ix;
   float ov[2], hv[2];
 #pragma unroll
@@ -486,14 +496,14 @@ __global__ __launch_bounds__(256) void k_infer_mfma(const unsigned char* __restr
   for (int u = 0; u < NV; ++u) {
     const int k = part + 4 * u;
     float v = 0.f;
-    if (k < 225) {  // field f of level L-1 at (kn inward, mm - 2 along the edge)
+    if (k < 225) {  // field f of level L-1 (L+1) at (kn inward, mm - 2 along the edge)
       const int kn = k / 25, rem = k - 25 * kn, mi = rem / 5, f = rem - 5 * mi;
       v = gldv(l1, f * fs + base + kn * nstep + (mi - 2) * tstep);
-      if (f >= 2) v = v * (1.f / Z0);
-    } else if (k < NRAW) {  // the cell's own fields of level L-2
+      if (z_scaled(f, ADJ)) v = v * (1.f / Z0);
+    } else if (k < NRAW) {  // the cell's own fields of level L-2 (L+2)
       const int f = k - 225;
       v = gldv(l2, f * fs + base);
-      if (f >= 2) v = v * (1.f / Z0);
+      if (z_scaled(f, ADJ)) v = v * (1.f / Z0);
     }
     v = live ? v : 0.f;
     rv[u] = v;
@@ -568,7 +578,7 @@ __global__ __launch_bounds__(256) void k_infer_mfma(const unsigned char* __restr
     for (int q = 0; q < 2; ++q) {
       const int o = part + 4 * q;
       if (o < NO) {
-        const float d = (y[o] + sB[2 * H + o]) * m * (o >= 2 ? Z0 : 1.f);
+        const float d = (y[o] + sB[2 * H + o]) * m * (z_scaled(o, ADJ) ? Z0 : 1.f);
         gstv(out, o * fs + gi0, ov[q] + d);
         if (hh && o < 2) gstv(hh, (int)S.n2 * o + gi0, hv[q] + d);
       }
@@ -638,7 +648,7 @@ void labc_cell_table(const std::vector<std::pair<int, int>>& grids, std::vector<
 }
 
 int launch_labc_infer(const unsigned char* w, const ShotDev* shots, const LCell* cells, int nband, int nrim, int L,
-                      bool hist, hipStream_t st) {
+                      bool hist, hipStream_t st, bool adj) {
   using namespace labc;
   // band workgroups: one round of resident workgroups, each looping over
   // batches of 64 cells
@@ -649,7 +659,8 @@ int launch_labc_infer(const unsigned char* w, const ShotDev* shots, const LCell*
     int dev = 0, cus = 0, per_cu = 0;
     if (hipGetDevice(&dev) != hipSuccess ||
         hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
-        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_infer_mfma, 256, 0) != hipSuccess || cus * per_cu <= 0)
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_infer_mfma<false>, 256, 0) != hipSuccess ||
+        cus * per_cu <= 0)
       return 512;
     return cus * per_cu;
   }();
@@ -657,7 +668,11 @@ int launch_labc_infer(const unsigned char* w, const ShotDev* shots, const LCell*
   const int nbw = std::min(nbb, resident);
   const int blocks = nbw + (nrim + 255) / 256;
   if (blocks == 0) return OK;
-  hipLaunchKernelGGL(k_infer_mfma, dim3(blocks), dim3(256), 0, st, w, shots, cells, nband, nrim, L, (int)hist, nbw);
+  if (adj)
+    hipLaunchKernelGGL(k_infer_mfma<true>, dim3(blocks), dim3(256), 0, st, w, shots, cells, nband, nrim, L, 0, nbw);
+  else
+    hipLaunchKernelGGL(k_infer_mfma<false>, dim3(blocks), dim3(256), 0, st, w, shots, cells, nband, nrim, L,
+                       (int)hist, nbw);
   HIPCHK(hipGetLastError());
   return OK;
 }
@@ -793,8 +808,8 @@ RTM_EXPORT int labc_net_grad(labc_net* n, const float* feat, const float* target
   return OK;
 }
 
-RTM_EXPORT int labc_features(int nz, int nx, const float* lev_m1, const float* lev_m2, const float* coef5,
-                             float* feat, float* scale) {
+static int features_impl(int nz, int nx, const float* lev_m1, const float* lev_m2, const float* coef5,
+                         float* feat, float* scale, int adj) {
   const int nc = labc_cell_count(nz, nx);
   if (nc == 0 || !lev_m1 || !lev_m2 || !coef5 || !feat || !scale) { set_error("bad argument"); return E_ARG; }
   STCHK(ensure_device());
@@ -808,7 +823,7 @@ RTM_EXPORT int labc_features(int nz, int nx, const float* lev_m1, const float* l
   STCHK(s.alloc((size_t)nc * 4, "scales"));
   HIPCHK(hipMemcpy(a.p, lev_m1, 5 * n2 * 4, hipMemcpyHostToDevice));
   HIPCHK(hipMemcpy(b.p, lev_m2, 5 * n2 * 4, hipMemcpyHostToDevice));
-  Lev l1{a.as<float>(), (int)n2, nz, nz, nx}, l2{b.as<float>(), (int)n2, nz, nz, nx};
+  Lev l1{a.as<float>(), (int)n2, nz, nz, nx, adj}, l2{b.as<float>(), (int)n2, nz, nz, nx, adj};
   const Band bd = band_of(nz, nx);
   const Mat mat{cf.as<float>(), (int)n2, nz};
   hipLaunchKernelGGL(k_features, dim3((nc + 255) / 256), dim3(256), 0, nullptr, l1, l2, mat, bd, f.as<float>(),
@@ -818,6 +833,16 @@ RTM_EXPORT int labc_features(int nz, int nx, const float* lev_m1, const float* l
   HIPCHK(hipMemcpy(feat, f.p, (size_t)nc * KP * 4, hipMemcpyDeviceToHost));
   HIPCHK(hipMemcpy(scale, s.p, (size_t)nc * 4, hipMemcpyDeviceToHost));
   return OK;
+}
+
+RTM_EXPORT int labc_features(int nz, int nx, const float* lev_m1, const float* lev_m2, const float* coef5,
+                             float* feat, float* scale) {
+  return features_impl(nz, nx, lev_m1, lev_m2, coef5, feat, scale, 0);
+}
+
+RTM_EXPORT int labc_features_adj(int nz, int nx, const float* lev_p1, const float* lev_p2, const float* coef5,
+                                 float* feat, float* scale) {
+  return features_impl(nz, nx, lev_p1, lev_p2, coef5, feat, scale, 1);
 }
 
 RTM_EXPORT int labc_data_create(labc_data** out, int capacity) {
@@ -911,7 +936,11 @@ struct LGrid {
   int step(int L, int flags, hipStream_t st) {
     return launch_fwd(desc.as<ShotDev>(), tiles.as<TileRef>(), ntiles, L, flags, 4, SNAP_HIST, 1, st);
   }
-  Lev lev_view(int L) const { return Lev{level(L), (int)n2, ldz, h.nz, h.nx}; }
+  // adjoint step: level L from L+1, L+2 (no imaging)
+  int astep(int L, int flags, hipStream_t st) {
+    return launch_adj(desc.as<ShotDev>(), tiles.as<TileRef>(), ntiles, L, flags, 4, SNAP_HIST, 1, st);
+  }
+  Lev lev_view(int L, int adj = 0) const { return Lev{level(L), (int)n2, ldz, h.nz, h.nx, adj}; }
   Mat mat() const { return Mat{coef.as<float>(), (int)n2, ldz}; }
 };
 
@@ -969,10 +998,91 @@ RTM_EXPORT int labc_collect(labc_data* d, int nz_phys, int nx_phys, int pad_wide
         STCHK(gs.step(L, F_SRC | (abc_band > 0 ? F_K : 0), nullptr));
         hipLaunchKernelGGL(k_targets, dim3((bd.total + 255) / 256), dim3(256), 0, nullptr, gw.level(L),
                            (int)gw.n2, gw.ldz, off, gs.level(L), (int)gs.n2, gs.ldz, bd,
-                           scales.as<float>(), d->tgt.as<float>() + (size_t)d->size * NO);
+                           scales.as<float>(), d->tgt.as<float>() + (size_t)d->size * NO, 0);
         HIPCHK(hipGetLastError());
         d->size += bd.total;
       }
+      ++next;
+    }
+  }
+  HIPCHK(hipDeviceSynchronize());
+  return OK;
+}
+
+// Teacher-forced pairs of the adjoint (receiver-side) propagation, the
+// counterpart of labc_collect: the wide grid models receiver data at depth
+// zr_phys (every physical column), then back-propagates them with the exact
+// adjoint of its sponge (continued by the band sponge, as in labc_collect);
+// at each listed level L the band grid's levels L+1, L+2 are windows of the
+// wide run, the features are those of an adjoint state, and the target is
+// the wide run's level L minus the band grid's adjoint update (its band
+// sponge, the same injected data).
+RTM_EXPORT int labc_collect_adj(labc_data* d, int nz_phys, int nx_phys, int pad_wide, int abc_wide, int nt,
+                                float dt, int zs_phys, int xs_phys, int zr_phys, const float* coef5_wide,
+                                const float* coef5_small, const float* wavelet, const int* levels, int nlev,
+                                float noise, unsigned seed, int abc_band) {
+  if (!d || !coef5_wide || !coef5_small || !wavelet || !levels || nlev < 1 || zr_phys < 0 || zr_phys >= nz_phys) {
+    set_error("bad argument");
+    return E_ARG;
+  }
+  const int pad_s = R + W;
+  if (pad_wide < pad_s + 5) { set_error("wide padding must exceed the band padding"); return E_ARG; }
+  if (abc_band < 0 || abc_band > W) { set_error("band sponge width %d not in [0, %d]", abc_band, W); return E_ARG; }
+  const int nzw = nz_phys + 2 * pad_wide, nxw = nx_phys + 2 * pad_wide;
+  const int nzs = nz_phys + 2 * pad_s, nxs = nx_phys + 2 * pad_s;
+  const int off = pad_wide - pad_s;
+  const Band bd = band_of(nzs, nxs);
+  if ((long long)d->size + (long long)nlev * bd.total > d->capacity) {
+    set_error("training set capacity %d exceeded", d->capacity);
+    return E_ARG;
+  }
+  STCHK(ensure_device());
+  DevBuf rec;
+  STCHK(rec.alloc((size_t)5 * nt * nx_phys * 4, "receivers"));
+  STCHK(launch_fill(rec.as<float>(), (size_t)5 * nt * nx_phys, 0.f, nullptr));
+  LGrid gw, gs;
+  STCHK(gw.setup(nzw, nxw, nt, dt, pad_wide + zs_phys, pad_wide + xs_phys, coef5_wide, wavelet, abc_wide,
+                 pad_wide + zr_phys, pad_wide, nx_phys, rec.as<float>(), abc_band, off, nzs, nxs));
+  STCHK(gs.setup(nzs, nxs, nt, dt, pad_s + zs_phys, pad_s + xs_phys, coef5_small, wavelet, abc_band,
+                 pad_s + zr_phys, pad_s, nx_phys, rec.as<float>()));
+  const bool kw = abc_wide > 0 || abc_band > 0;
+  // the data: one forward run of the wide grid
+  for (int L = 2; L <= nt; ++L) STCHK(gw.step(L, (L >= 3 && kw ? F_K : 0) | F_SRC | F_REC, nullptr));
+  STCHK(launch_fill(gw.ring.as<float>(), 20 * gw.n2, 0.f, nullptr));
+  DevBuf scales;
+  STCHK(scales.alloc((size_t)bd.total * 4, "scales"));
+  std::vector<int> lv(levels, levels + nlev);
+  std::sort(lv.begin(), lv.end(), std::greater<int>());
+  size_t next = 0;
+  const dim3 wgrid((nzs + 255) / 256, nxs);
+  for (int L = nt; L >= 1 && next < lv.size(); --L) {
+    const bool take = lv[next] == L;
+    if (take) {
+      // band grid levels L+1, L+2 = windowed wide levels (rim zero)
+      hipLaunchKernelGGL(k_window, wgrid, dim3(256), 0, nullptr, gw.level(L + 1), (int)gw.n2, gw.ldz, off,
+                         gs.level(L + 1), (int)gs.n2, gs.ldz, nzs, nxs);
+      hipLaunchKernelGGL(k_window, wgrid, dim3(256), 0, nullptr, gw.level(L + 2), (int)gw.n2, gw.ldz, off,
+                         gs.level(L + 2), (int)gs.n2, gs.ldz, nzs, nxs);
+      if (noise > 0.f) {
+        hipLaunchKernelGGL(k_noise, dim3((bd.total + 255) / 256), dim3(256), 0, nullptr, gs.level(L + 1), (int)gs.n2,
+                           gs.ldz, bd, noise, seed * 7919u + 2u * (unsigned)L);
+        hipLaunchKernelGGL(k_noise, dim3((bd.total + 255) / 256), dim3(256), 0, nullptr, gs.level(L + 2), (int)gs.n2,
+                           gs.ldz, bd, noise, seed * 7919u + 2u * (unsigned)L + 1u);
+      }
+      HIPCHK(hipGetLastError());
+      hipLaunchKernelGGL(k_features, dim3((bd.total + 255) / 256), dim3(256), 0, nullptr, gs.lev_view(L + 1, 1),
+                         gs.lev_view(L + 2, 1), gs.mat(), bd, d->feat.as<float>() + (size_t)d->size * KP,
+                         scales.as<float>());
+      HIPCHK(hipGetLastError());
+    }
+    STCHK(gw.astep(L, (kw ? A_K : 0) | A_INJ, nullptr));
+    if (take) {
+      STCHK(gs.astep(L, (abc_band > 0 ? A_K : 0) | A_INJ, nullptr));
+      hipLaunchKernelGGL(k_targets, dim3((bd.total + 255) / 256), dim3(256), 0, nullptr, gw.level(L),
+                         (int)gw.n2, gw.ldz, off, gs.level(L), (int)gs.n2, gs.ldz, bd,
+                         scales.as<float>(), d->tgt.as<float>() + (size_t)d->size * NO, 1);
+      HIPCHK(hipGetLastError());
+      d->size += bd.total;
       ++next;
     }
   }
@@ -1026,6 +1136,12 @@ RTM_EXPORT int rtm_batch_set_learned(rtm_batch* b, labc_net* n) {
   Engine* e = batch_engine(b);
   if (!e) { set_error("null batch"); return E_ARG; }
   return e->set_learned(n ? n->packed.as<unsigned char>() : nullptr);
+}
+
+RTM_EXPORT int rtm_batch_set_learned_adjoint(rtm_batch* b, labc_net* n) {
+  Engine* e = batch_engine(b);
+  if (!e) { set_error("null batch"); return E_ARG; }
+  return e->set_learned_adjoint(n ? n->packed.as<unsigned char>() : nullptr);
 }
 
 RTM_EXPORT int labc_forward(labc_net* n, const rtm_shot_t* s, const float* coef5, const float* wavelet,

@@ -44,16 +44,18 @@ def cell_coords(nz, nx, c):
     return iz.value, ix.value, e.value, d.value
 
 
-def features(lev_m1, lev_m2, coef5):
+def features(lev_m1, lev_m2, coef5, adjoint=False):
     """Normalised bf16 features (ncell, KPAD) and scales of the band cells
     of one grid from levels L-1 and L-2 and the grid's coefficients (five
-    (nz, nx) arrays each)."""
+    (nz, nx) arrays each); with adjoint=True from levels L+1 and L+2 of an
+    adjoint state."""
     nz, nx = np.asarray(lev_m1[0]).shape
     nc = cell_count(nz, nx)
     a, b, c = pack_fields(lev_m1), pack_fields(lev_m2), pack_fields(coef5)
     f = np.zeros((nc, KPAD), np.float32)
     s = np.zeros(nc, np.float32)
-    _chk(_L().labc_features(C.c_int(nz), C.c_int(nx), _p(a), _p(b), _p(c), _p(f), _p(s)))
+    fn = _L().labc_features_adj if adjoint else _L().labc_features
+    _chk(fn(C.c_int(nz), C.c_int(nx), _p(a), _p(b), _p(c), _p(f), _p(s)))
     return f, s
 
 
@@ -152,6 +154,18 @@ class Data:
                                _p(w), _p(lv), C.c_int(lv.size), C.c_float(noise), C.c_uint(seed),
                                C.c_int(abc_band)))
 
+    def collect_adj(self, nz_phys, nx_phys, pad_wide, abc_wide, nt, dt, zs_phys, xs_phys, zr_phys, coefs_wide,
+                    coefs_small, wavelet, levels, noise=0.0, seed=0, abc_band=0):
+        """Teacher-forced pairs of the adjoint propagation of one wide-sponge
+        run's receiver data (see labc_collect_adj)."""
+        cw, cs = pack_fields(coefs_wide), pack_fields(coefs_small)
+        w = np.ascontiguousarray(wavelet, np.float32)
+        lv = np.ascontiguousarray(levels, np.int32)
+        _chk(_L().labc_collect_adj(self.h, C.c_int(nz_phys), C.c_int(nx_phys), C.c_int(pad_wide),
+                                   C.c_int(abc_wide), C.c_int(nt), C.c_float(dt), C.c_int(zs_phys),
+                                   C.c_int(xs_phys), C.c_int(zr_phys), _p(cw), _p(cs), _p(w), _p(lv),
+                                   C.c_int(lv.size), C.c_float(noise), C.c_uint(seed), C.c_int(abc_band)))
+
 
 def propagate(net, s: _lib.RtmShot, coef5, wavelet, want_last=False):
     """Forward propagation of a band grid with the learned layer (net, and
@@ -202,7 +216,7 @@ def _rel(a, b):
 
 def experiment(nz=400, nx=800, dx=10.0, dt=1e-3, nt=2500, abc_wide=120, band_abc=8, train_models=5,
                shots_per_model=2, levels_per_shot=200, test_models=2, steps=(4000, 2000, 2000), batch=8192,
-               noise=0.03, seed=20260301, log=None):
+               noise=0.03, seed=20260301, adjoint_layer=True, log=None):
     """The learned layer end to end on nz x nx grids (all on the GPU):
 
     * training: `train_models` seeded random layered models (Vp 1500-4500
@@ -210,7 +224,9 @@ def experiment(nz=400, nx=800, dx=10.0, dt=1e-3, nt=2500, abc_wide=120, band_abc
       random positions), teacher-forced pairs from runs with an `abc_wide`
       sponge placed outside the band grid (`levels_per_shot` levels each:
       train_models * shots_per_model * levels_per_shot snapshot pairs),
-      noise augmentation, Adam in three learning-rate stages;
+      noise augmentation, Adam in three learning-rate stages; with
+      adjoint_layer, a second network trained the same way on the adjoint
+      runs of the same shots' receiver data (labc_collect_adj);
     * reflection energy: on held-out models, the physical-region wavefield
       of the last two levels and the receiver data of an 8-cell band grid
       with the band sponge alone, with the sponge and the network, and of a
@@ -218,7 +234,8 @@ def experiment(nz=400, nx=800, dx=10.0, dt=1e-3, nt=2500, abc_wide=120, band_abc
     * images: RTM (exact-adjoint sponge) of the held-out shots through the
       batch engine with each boundary treatment, the observed data the wide
       grid's (true minus smoothed model), relative L2 of the physical-region
-      images against the wide-sponge RTM;
+      images against the wide-sponge RTM (with adjoint_layer also with the
+      adjoint network alone and with both networks);
     * cost: device time per forward level with and without the layer.
     Returns (metrics dict, trained Net)."""
     from . import synth
@@ -231,6 +248,8 @@ def experiment(nz=400, nx=800, dx=10.0, dt=1e-3, nt=2500, abc_wide=120, band_abc
     nshot_train = train_models * shots_per_model
     d = Data(ncell * levels_per_shot * nshot_train)
     t0 = _time()
+    zr = 2  # receiver depth below the band, in the training and held-out shots
+    train_shots = []
     for mi in range(train_models):
         vp = random_layered_vp(nz, nx, rng)
         cw, cs = _coefs(vp, pad_w, dx, dt), _coefs(vp, PAD, dx, dt)
@@ -241,22 +260,48 @@ def experiment(nz=400, nx=800, dx=10.0, dt=1e-3, nt=2500, abc_wide=120, band_abc
             lv = np.linspace(150, nt, levels_per_shot).round().astype(int)
             d.collect(nz, nx, pad_w, abc_wide, nt, dt, zs, xs, cw, cs, wav, lv, noise=noise,
                       seed=int(seed + 31 * mi + si), abc_band=band_abc)
+            if adjoint_layer:
+                train_shots.append((cw, cs, zs, xs, wav, int(seed + 31 * mi + si)))
     say(f"learned: {len(d)} training rows from {nshot_train * levels_per_shot} snapshot pairs, "
         f"{_time() - t0:.1f}s")
-    net = Net(seed=int(seed % 100000))
-    l0 = net.eval(d)
-    t0 = _time()
-    for lr, n in zip((2e-3, 5e-4, 1e-4), steps):
-        net.train(d, n, batch=batch, lr=lr, seed=int(lr * 1e6) + seed)
-    l1 = net.eval(d)
-    train_s = _time() - t0
+
+    def fit(data, net_seed):
+        n = Net(seed=net_seed)
+        a = n.eval(data)
+        t = _time()
+        for lr, k in zip((2e-3, 5e-4, 1e-4), steps):
+            n.train(data, k, batch=batch, lr=lr, seed=int(lr * 1e6) + seed)
+        return n, a, n.eval(data), _time() - t
+
+    net, l0, l1, train_s = fit(d, int(seed % 100000))
     say(f"learned: loss {l0:.3e} -> {l1:.3e} in {sum(steps)} Adam steps, {train_s:.1f}s")
     d.close()
+    net_adj, adj_train = None, None
+    if adjoint_layer:
+        # the adjoint network: pairs of the adjoint runs of the same shots
+        # (the data recorded on the wide grid), levels spread over the
+        # adjoint's range, the set built after the forward one is freed
+        da = Data(ncell * levels_per_shot * nshot_train)
+        t0 = _time()
+        for cw, cs, zs, xs, wav, sd in train_shots:
+            lv = np.linspace(1, nt - 150, levels_per_shot).round().astype(int)
+            da.collect_adj(nz, nx, pad_w, abc_wide, nt, dt, zs, xs, zr, cw, cs, wav, lv, noise=noise, seed=sd,
+                           abc_band=band_abc)
+        collect_s = _time() - t0
+        net_adj, a0, a1, a_s = fit(da, int(seed % 100000) + 1)
+        da.close()
+        adj_train = {"rows": ncell * levels_per_shot * nshot_train, "loss_initial": a0, "loss_final": a1,
+                     "collect_seconds": collect_s, "seconds": a_s}
+        say(f"learned: adjoint network loss {a0:.3e} -> {a1:.3e}, {collect_s:.1f}s collection, {a_s:.1f}s training")
+    train_shots = None
 
     # held-out models: reflection energy and images
     refl = {"band_sponge": [], "band_sponge_net": [], "sponge40": []}
     rec_err = {k: [] for k in refl}
     img_err = {k: [] for k in refl}
+    if net_adj is not None:
+        img_err["band_sponge_net_adj"] = []
+        img_err["band_sponge_net_both"] = []
     ms_level = {}
     for mi in range(test_models):
         vp = random_layered_vp(nz, nx, rng)
@@ -265,7 +310,6 @@ def experiment(nz=400, nx=800, dx=10.0, dt=1e-3, nt=2500, abc_wide=120, band_abc
                  for _ in range(shots_per_model)]
         for zs, xs, fp in shots:
             wav = _lib.ricker(nt, dt, fp)
-            zr = 2
             # forward runs of the true model: the wide reference and the three boundaries
             sw = _lib.shot(nz + 2 * pad_w, nx + 2 * pad_w, nt, dx, dt, pad_w + xs, pad_w + zs, pad_w + zr, pad_w, nx,
                            abc_wide)
@@ -287,14 +331,19 @@ def experiment(nz=400, nx=800, dx=10.0, dt=1e-3, nt=2500, abc_wide=120, band_abc
             rws = _lib.forward(sw, _coefs(vps, pad_w, dx, dt), wav)
             dobs = rw - rws
             imgs = {}
-            for key, pad, abc, use_net in (("wide", pad_w, abc_wide, False), ("sponge40", p40, 40, False),
-                                           ("band_sponge", PAD, band_abc, False),
-                                           ("band_sponge_net", PAD, band_abc, True)):
+            cases = [("wide", pad_w, abc_wide, None, None), ("sponge40", p40, 40, None, None),
+                     ("band_sponge", PAD, band_abc, None, None), ("band_sponge_net", PAD, band_abc, net, None)]
+            if net_adj is not None:
+                cases += [("band_sponge_net_adj", PAD, band_abc, None, net_adj),
+                          ("band_sponge_net_both", PAD, band_abc, net, net_adj)]
+            for key, pad, abc, fwd_net, adj_net in cases:
                 spec = dict(nz=nz + 2 * pad, nx=nx + 2 * pad, xs=pad + xs, zs=pad + zs, zr=pad + zr, rx0=pad, nrx=nx,
                             img_z0=pad, img_x0=pad, img_nz=nz, img_nx=nx, stack_z0=0, stack_x0=0)
                 b = _lib.Batch(nt, dx, dt, [spec], [_coefs(vps, pad, dx, dt)], wav, abc=abc, adj_abc=1, txr=2)
-                if use_net:
-                    b.set_learned(net)
+                if fwd_net is not None:
+                    b.set_learned(fwd_net)
+                if adj_net is not None:
+                    b.set_learned_adjoint(adj_net)
                 b.set_data(0, dobs)
                 b.migrate()
                 img = b.image(0)
@@ -308,8 +357,11 @@ def experiment(nz=400, nx=800, dx=10.0, dt=1e-3, nt=2500, abc_wide=120, band_abc
     def mean(v):
         return float(np.mean(v)) if v else None
 
+    if net_adj is not None:
+        net_adj.close()
     out = {"training": {"snapshot_pairs": nshot_train * levels_per_shot, "rows": ncell * levels_per_shot * nshot_train,
                         "loss_initial": l0, "loss_final": l1, "adam_steps": int(sum(steps)), "seconds": train_s},
+           "adjoint_training": adj_train,
            "reflection_rel_l2": {k: mean(v) for k, v in refl.items()},
            "receiver_rel_l2": {k: mean(v) for k, v in rec_err.items()},
            "image_rel_l2_vs_wide": {k: mean(v) for k, v in img_err.items()},

@@ -357,6 +357,13 @@ class Batch:
         self._net = net
         _chk(load().rtm_batch_set_learned(self._h, net.h if net is not None else None))
 
+    def set_learned_adjoint(self, net):
+        """Use the learned layer `net` (trained on adjoint states) in this
+        batch's adjoint runs, or drop it with None; the network must stay
+        alive while the batch runs."""
+        self._net_adj = net
+        _chk(load().rtm_batch_set_learned_adjoint(self._h, net.h if net is not None else None))
+
     def close(self):
         if self._h:
             load().rtm_batch_destroy(self._h)

@@ -70,6 +70,10 @@ int labc_net_grad(labc_net *net, const float *feat, const float *target, int n, 
  * feat [ncell][LABC_KPAD], scale [ncell]. */
 int labc_features(int nz, int nx, const float *lev_m1, const float *lev_m2, const float *coef5,
                   float *feat, float *scale);
+/* The same for an adjoint state: levels L+1 and L+2 in place of L-1 and
+ * L-2, the velocities (not the stresses) divided by the impedance scale. */
+int labc_features_adj(int nz, int nx, const float *lev_p1, const float *lev_p2, const float *coef5,
+                      float *feat, float *scale);
 
 /* Training set on the device. */
 typedef struct labc_data labc_data;
@@ -97,6 +101,20 @@ int labc_collect(labc_data *d, int nz_phys, int nx_phys, int pad_wide, int abc_w
                  float dt, int zs_phys, int xs_phys, const float *coef5_wide,
                  const float *coef5_small, const float *wavelet, const int *levels, int nlev,
                  float noise, unsigned seed, int abc_band);
+/* Teacher-forced pairs of the adjoint (receiver-side) propagation, for a
+ * layer in the adjoint runs. The wide grid records receivers at depth
+ * zr_phys over every physical column in a forward run (sponges as in
+ * labc_collect), then propagates them back with the exact adjoint of its
+ * sponge. At each listed level L the band grid's levels L+1, L+2 are the
+ * windowed wide levels (rim zeroed, noise as in labc_collect); the features
+ * are those of labc_features_adj; the target is wide(L) - the band grid's
+ * adjoint update(L) (the same injected data, the exact adjoint of the band
+ * sponge when abc_band > 0) on the band cells, divided by the cell scale
+ * (velocities also by the impedance scale). */
+int labc_collect_adj(labc_data *d, int nz_phys, int nx_phys, int pad_wide, int abc_wide, int nt,
+                     float dt, int zs_phys, int xs_phys, int zr_phys, const float *coef5_wide,
+                     const float *coef5_small, const float *wavelet, const int *levels, int nlev,
+                     float noise, unsigned seed, int abc_band);
 /* Adam on random minibatches: loss per step in losses[steps]. */
 int labc_train(labc_net *net, labc_data *d, int steps, int batch, float lr, unsigned seed,
                float *losses);
@@ -125,6 +143,16 @@ double labc_last_forward_ms(void);
  * must be at least 2 * (LABC_RIM + LABC_WIDTH) + 1 cells on each side
  * (RTM_E_ARG otherwise). */
 int rtm_batch_set_learned(rtm_batch *b, labc_net *net);
+/* The learned layer in the batch's adjoint runs (no counterpart in the
+ * reference): a network trained on adjoint states (labc_collect_adj)
+ * corrects the band cells of every adjoint level after its step and clears
+ * the rim, one level per launch. The imaging of a level happens in its step:
+ * only the images of band cells (outside the physical region) miss the
+ * correction. The batch's adjoint sponge should be the exact adjoint
+ * (adj_abc 1) of the band sponge the network was trained with; the R/Src
+ * adjoint sponge (adj_abc 2) is refused with RTM_E_ARG. NULL drops the
+ * layer; grid sizes as for rtm_batch_set_learned. */
+int rtm_batch_set_learned_adjoint(rtm_batch *b, labc_net *net);
 
 #ifdef __cplusplus
 }

@@ -17,7 +17,11 @@ GPU results can be checked:
   (hand-written backward pass);
 * teacher-forced training pairs: the band grid's levels L-1, L-2 are windows
   of a wide-sponge run, the target is (wide(L) - plain band-grid update(L)) / s
-  per field unit, using the 2D oracle's forward and single-level update.
+  per field unit, using the 2D oracle's forward and single-level update;
+* the same for the adjoint propagation of the wide run's receiver data
+  (levels L+1, L+2, the adjoint's field units), each adjoint level the
+  oracle's gather-form rtm_op3_c_ on a three-level history followed by the
+  exact adjoint of the sponge.
 
 The reference has no learned boundary: it only has the sponge of
 W/abc-mod.f90:59-78. The network is this project's addition, so there is no
@@ -31,7 +35,7 @@ from __future__ import annotations
 import numpy as np
 from scipy.special import erf
 
-from . import abc_K, forward, fwd_level, make_params
+from . import abc_K, forward, fwd_level, image_S, image_adj_level, make_params, rtm_op3_c
 
 R, W, NF, KP, H, NO = 5, 8, 245, 248, 32, 5
 NRAW = 230
@@ -82,9 +86,17 @@ def _unit(f, dtype):
     return dtype(1.0) / dtype(Z0) if dtype == np.float32 else 1.0 / Z0
 
 
-def raw_patch(lev_m1, lev_m2, cells, dtype=np.float64):
+def _field_unit(f, dtype, adjoint):
+    """Scale of field f in the features: the stresses of a forward state,
+    the velocities of an adjoint state (the larger field pair of each, by
+    about the impedance Z0) are divided by Z0."""
+    return _unit(2, dtype) if (f < 2 if adjoint else f >= 2) else None
+
+
+def raw_patch(lev_m1, lev_m2, cells, dtype=np.float64, adjoint=False):
     """Raw features 0..229 of the band cells: (ncell, 230) in dtype.
-    lev_m1 / lev_m2: five (nz, nx) arrays each."""
+    lev_m1 / lev_m2: five (nz, nx) arrays each (levels L-1, L-2 of a forward
+    state; L+1, L+2 of an adjoint state with adjoint=True)."""
     iz, ix, edge, _ = cells
     nz, nx = lev_m1[0].shape
     dz = np.select([edge == 0, edge == 1], [1, -1], 0)
@@ -100,13 +112,13 @@ def raw_patch(lev_m1, lev_m2, cells, dtype=np.float64):
             zc, xc = np.clip(z, 0, nz - 1), np.clip(x, 0, nx - 1)
             for f in range(5):
                 v = np.where(ok, np.asarray(lev_m1[f], dtype)[zc, xc], dtype(0))
-                u = _unit(f, dtype)
+                u = _field_unit(f, dtype, adjoint)
                 if u is not None:
                     v = (v * u).astype(dtype)
                 out[:, kn * 25 + (m + 2) * 5 + f] = v
     for f in range(5):
         v = np.asarray(lev_m2[f], dtype)[iz, ix]
-        u = _unit(f, dtype)
+        u = _field_unit(f, dtype, adjoint)
         out[:, 225 + f] = v if u is None else (v * u).astype(dtype)
     return out
 
@@ -125,11 +137,11 @@ def material(coefs, cells, dtype=np.float64):
     return cp, cs, zp.astype(dtype)
 
 
-def features(lev_m1, lev_m2, coefs, dtype=np.float64, cells=None):
+def features(lev_m1, lev_m2, coefs, dtype=np.float64, cells=None, adjoint=False):
     """Normalised, bf16-rounded features (ncell, KP) and scales (ncell,)."""
     if cells is None:
         cells = band_cells(*lev_m1[0].shape)
-    raw = raw_patch(lev_m1, lev_m2, cells, dtype)
+    raw = raw_patch(lev_m1, lev_m2, cells, dtype, adjoint)
     s = np.abs(raw).max(axis=1)
     live = s > 1e-30
     inv = np.zeros_like(s)
@@ -275,6 +287,126 @@ def teacher_pairs(nz_phys, nx_phys, pad_wide, abc_wide, nt, dt, zs_phys, xs_phys
             u = Z0 if fi >= 2 else 1.0
             with np.errstate(divide="ignore", invalid="ignore"):
                 t[:, fi] = np.where(s > 0, (wide - plain[fi][iz, ix]) / (s * u), 0.0)
+        feats.append(f)
+        tgts.append(t)
+    return np.concatenate(feats), np.concatenate(tgts)
+
+
+def adjoint_level(inj, p1, p2, coefs, kz, kx):
+    """One adjoint level K (inj + X(L+2) + Op^T X(L+1)) (the step of
+    W/rtmlib-modC.c:120-253 in gather form, followed by the exact adjoint of
+    a separable sponge kz[iz]*kx[ix]): the oracle's rtm_op3_c_ on the
+    three-level history (inj, X(L+1), X(L+2)), which computes level 0 only."""
+    nz, nx = p1[0].shape
+    h = []
+    for f in range(5):
+        a = np.zeros((nz, nx, 3), order="F")
+        a[:, :, 0], a[:, :, 1], a[:, :, 2] = inj[f], p1[f], p2[f]
+        h.append(a)
+    out = rtm_op3_c(h, coefs, form="gather", dtype=np.float64)
+    return [(o[:, :, 0] * kz[:, None]) * kx[None, :] for o in out]
+
+
+def adjoint_run(coefs, kz, kx, rec, zr, rx0, nt, on_level=None):
+    """Adjoint levels nt..1 of receiver data rec (5, nrx, nt) injected on row
+    zr, columns rx0.. at the level of the same index (level L gets
+    rec[:, :, L-1]), levels nt+1, nt+2 zero, each level
+    K (inj + X(L+2) + Op^T X(L+1)) (the recursion of the oracle's RTM,
+    oracle/rtm_oracle.cpp rtm). on_level(L, level, levels), when given, sees
+    each new level before it is stored and may change it in place (imaging,
+    a learned correction). Returns {L: five (nz, nx)}."""
+    nz, nx = np.asarray(coefs[0]).shape
+    nrx = rec.shape[1]
+    lev = {nt + 1: [np.zeros((nz, nx)) for _ in range(5)], nt + 2: [np.zeros((nz, nx)) for _ in range(5)]}
+    for L in range(nt, 0, -1):
+        inj = [np.zeros((nz, nx)) for _ in range(5)]
+        for f in range(5):
+            inj[f][zr, rx0:rx0 + nrx] = rec[f, :, L - 1]
+        cur = adjoint_level(inj, lev[L + 1], lev[L + 2], coefs, kz, kx)
+        if on_level is not None:
+            on_level(L, cur, lev)
+        lev[L] = cur
+    return lev
+
+
+def rtm_adjoint_layer(p, wavelet, coefs, rec, params=None):
+    """Single-shot RTM images (drho, dlambda, dmu) as the oracle's rtm
+    (forward with the sponge p.abc and a full Vx/Vz history, the adjoint
+    recursion with the exact-adjoint sponge when p.adj_abc, the Lame imaging
+    condition at every level) with, when params is given, the learned layer
+    in the adjoint: after a level is formed and imaged, its band cells get
+    s * NN(features of levels L+1, L+2 of the adjoint state) (field units
+    applied) and its rim is cleared."""
+    nz, nx, nt = p.nz, p.nx, p.nt
+    fw = forward(p, wavelet, coefs, want_hist=True)
+    hvx, hvz = fw["hist_vx"], fw["hist_vz"]
+    kz, kx = sponge_axes(nz, nx, p.abc) if p.adj_abc else (np.ones(nz), np.ones(nx))
+    cells = band_cells(nz, nx)
+    iz, ix = cells[0], cells[1]
+    dm = [None]
+
+    def on_level(L, cur, lev):
+        S = image_S(hvx[:, :, L - 1], hvz[:, :, L - 1], hvx[:, :, L] if L < nt else None,
+                    hvz[:, :, L] if L < nt else None, p.dx, p.dt)
+        dm[0] = image_adj_level(cur, S, 1.0, dm[0])
+        if params is None:
+            return
+        f, sc = features(lev[L + 1], lev[L + 2], coefs, np.float64, cells, adjoint=True)
+        y = mlp(params, f)
+        for fi in range(5):
+            cur[fi][iz, ix] += y[:, fi] * sc * (Z0 if fi < 2 else 1.0)
+            cur[fi][:R, :] = 0
+            cur[fi][nz - R:, :] = 0
+            cur[fi][:, :R] = 0
+            cur[fi][:, nx - R:] = 0
+        lev.pop(L + 3, None)
+
+    adjoint_run(coefs, kz, kx, np.asarray(rec, np.float64), p.zr, p.rx0, nt, on_level)
+    return tuple(dm[0])
+
+
+def teacher_pairs_adj(nz_phys, nx_phys, pad_wide, abc_wide, nt, dt, zs_phys, xs_phys, zr_phys, coefs_wide,
+                      coefs_small, wavelet, levels, abc_band=0):
+    """FP64 (features, targets) of the adjoint propagation, stacked in
+    decreasing level order like labc_collect_adj: the wide grid's forward run
+    (sponges as in teacher_pairs) records receivers on row zr_phys of every
+    physical column; its adjoint run (the same separable sponge, exact
+    adjoint) back-propagates them; at each listed level L the band grid's
+    levels L+1, L+2 are windows of it, the features those of an adjoint
+    state, the target (wide(L) - band-grid adjoint update(L)) / (s * unit)
+    with the velocities' unit Z0."""
+    pad_s = R + W
+    nzw, nxw = nz_phys + 2 * pad_wide, nx_phys + 2 * pad_wide
+    nzs, nxs = nz_phys + 2 * pad_s, nx_phys + 2 * pad_s
+    off = pad_wide - pad_s
+    pw = make_params(nzw, nxw, nt, 1.0, dt, pad_wide + xs_phys, pad_wide + zs_phys, 0, 0, 0, abc_wide)
+    kzw, kxw = sponge_axes(nzw, nxw, abc_wide)
+    kzb, kxb = sponge_axes(nzs, nxs, abc_band)
+    if abc_band > 0:
+        kzw = kzw * kzb[np.clip(np.arange(nzw) - off, 0, nzs - 1)]
+        kxw = kxw * kxb[np.clip(np.arange(nxw) - off, 0, nxs - 1)]
+    hist = _forward_custom_k(pw, np.asarray(wavelet, np.float64), coefs_wide, kzw, kxw, nt)
+    zrw = pad_wide + zr_phys
+    rec = np.stack([h[zrw, pad_wide:pad_wide + nx_phys, :] for h in hist])
+    wide = adjoint_run(coefs_wide, kzw, kxw, rec, zrw, pad_wide, nt)
+    cells = band_cells(nzs, nxs)
+    iz, ix = cells[0], cells[1]
+    zrs = pad_s + zr_phys
+    feats, tgts = [], []
+    for L in sorted(levels, reverse=True):
+        p1 = window(wide[L + 1], off, nzs, nxs)
+        p2 = window(wide[L + 2], off, nzs, nxs)
+        f, s = features(p1, p2, coefs_small, np.float64, cells, adjoint=True)
+        inj = [np.zeros((nzs, nxs)) for _ in range(5)]
+        for fi in range(5):
+            inj[fi][zrs, pad_s:pad_s + nx_phys] = rec[fi, :, L - 1]
+        band = adjoint_level(inj, p1, p2, coefs_small, kzb, kxb)
+        t = np.zeros((iz.size, NO))
+        for fi in range(5):
+            w = wide[L][fi][iz + off, ix + off]
+            u = Z0 if fi < 2 else 1.0
+            with np.errstate(divide="ignore", invalid="ignore"):
+                t[:, fi] = np.where(s > 0, (w - band[fi][iz, ix]) / (s * u), 0.0)
         feats.append(f)
         tgts.append(t)
     return np.concatenate(feats), np.concatenate(tgts)

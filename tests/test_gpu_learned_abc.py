@@ -43,6 +43,25 @@ def test_features_bit_exact(L):
     assert np.array_equal(f, fo)
 
 
+def test_adjoint_features_bit_exact(L):
+    """Features of an adjoint state (levels L+1, L+2; the velocities divided
+    by Z0) equal the oracle's float32 emulation bit for bit."""
+    rng = np.random.default_rng(15)
+    nz, nx = 48, 70
+    p1 = [(rng.standard_normal((nz, nx)) * (1e4 if f < 2 else 1e-3)).astype(np.float32) for f in range(5)]
+    p2 = [(rng.standard_normal((nz, nx)) * (1e4 if f < 2 else 1e-3)).astype(np.float32) for f in range(5)]
+    p1[3][:, 40:] = 0
+    vp = synth.layered_vp(nz, nx, 2, vp_top=1800.0, vp_bot=3500.0, nlayers=2, anom_sigma=5.0)
+    m = synth.elastic_from_vp(vp)
+    coefs = O.model_op(m.rho, m.lam, m.mu, 10.0, 1e-3, np.float32)
+    f, s = L.features(p1, p2, coefs, adjoint=True)
+    fo, so = LA.features(p1, p2, coefs, np.float32, adjoint=True)
+    assert np.array_equal(s, so.astype(np.float32))
+    assert np.array_equal(f, fo)
+    fw, _ = L.features(p1, p2, coefs)
+    assert not np.array_equal(f, fw)
+
+
 def _params(seed, scale=0.2):
     return (np.random.default_rng(seed).standard_normal(LA.NPARAM) * scale).astype(np.float32)
 
@@ -106,6 +125,25 @@ def test_collect_vs_oracle(L, band):
     # targets: FP32 differences of nearly equal levels; normalised units
     assert np.abs(t - to).max() <= 1e-4 * max(1.0, np.abs(to).max())
     assert np.abs(to).max() > 1e-3  # the wave reached the rim
+    d.close()
+
+
+@pytest.mark.parametrize("band", [0, 8])
+def test_collect_adj_vs_oracle(L, band):
+    """Adjoint teacher pairs (wide forward recording receivers, its adjoint
+    run with the exact-adjoint sponge, windows of levels L+1, L+2, the band
+    grid's adjoint update) against the FP64 oracle."""
+    case = _case()
+    levels = [20, 60, 95, 130, 159]
+    d = L.Data(100000)
+    d.collect_adj(levels=levels, abc_band=band, zr_phys=2, **case)
+    fo, to = LA.teacher_pairs_adj(levels=levels, abc_band=band, zr_phys=2, **case)
+    f, t = d.get()
+    assert f.shape == fo.shape
+    assert np.all(np.abs(f - fo) <= 2.0 ** -7 * np.abs(fo) + 1e-4)
+    assert np.abs(f - fo).mean() < 1e-4
+    assert np.abs(t - to).max() <= 1e-4 * max(1.0, np.abs(to).max())
+    assert np.abs(to).max() > 1e-3  # the adjoint wavefield reached the rim
     d.close()
 
 
@@ -282,6 +320,48 @@ def test_batch_engine_learned_layer_equals_single_grid_path(L, hip):
     for a, c in zip(with_net, without):
         assert np.isfinite(a).all() and np.abs(a).max() > 0
     assert max(O.rel_l2(a, c) for a, c in zip(with_net, without)) > 1e-6
+
+
+def test_batch_engine_adjoint_layer_vs_oracle(L, hip):
+    """The learned layer in the batch engine's adjoint runs: after every
+    adjoint level (formed and imaged in its step) the band cells get the
+    network's correction from levels L+1, L+2 and the rim is cleared. The
+    images (FP32 state, the parallel inference kernel) match the oracle's
+    RTM with the same layer (FP64 state, FP64 network on the same bf16
+    weights); the layer acted; dropping it restores the plain images."""
+    nzp, nxp, nt, dx, dt = 30, 44, 120, 10.0, 1e-3
+    pad = L.PAD
+    nz, nx = nzp + 2 * pad, nxp + 2 * pad
+    vp = synth.layered_vp(nzp, nxp, 8, vp_top=2000.0, vp_bot=2800.0, nlayers=2, jump=100.0, anom_sigma=4.0)
+    m = synth.elastic_from_vp(synth.pad_edge(vp, pad))
+    coefs = O.model_op(m.rho, m.lam, m.mu, dx, dt, np.float32)
+    wav = O.ricker(nt, dt, 30.0, nsource=50, dtype=np.float32)
+    spec = dict(nz=nz, nx=nx, xs=pad + 20, zs=pad + 4, zr=pad + 2, rx0=pad, nrx=nxp)
+    p = O.make_params(nz, nx, nt, dx, dt, spec["xs"], spec["zs"], spec["zr"], pad, nxp, 8, adj_abc=1)
+    rec = O.forward(p, wav, coefs, np.float32)["rec"]
+    params = (np.random.default_rng(19).standard_normal(LA.NPARAM) * 0.05).astype(np.float32)
+    net = L.Net(seed=1)
+    net.params = params
+    b = hip.Batch(nt, dx, dt, [spec], [coefs], wav, abc=8, adj_abc=1, txr=2)
+    b.set_data(0, rec)
+    b.migrate()
+    plain = b.image(0)
+    b.set_learned_adjoint(net)
+    b.migrate()
+    got = b.image(0)
+    b.set_learned_adjoint(None)
+    b.migrate()
+    again = b.image(0)
+    b.close()
+    net.close()
+    ref = LA.rtm_adjoint_layer(p, wav, coefs, rec, params)
+    ref_plain = LA.rtm_adjoint_layer(p, wav, coefs, rec)
+    for a, r, q, r0 in zip(got, ref, plain, ref_plain):
+        assert O.rel_l2(a, r) <= 1e-3
+        assert O.rel_l2(q, r0) <= 1e-4
+        assert O.rel_l2(r, r0) > 1e-3  # the layer acted
+    for a, c in zip(again, plain):
+        assert np.array_equal(a, c)
 
 
 def test_learned_experiment_small(L, hip):

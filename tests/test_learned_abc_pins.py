@@ -272,3 +272,89 @@ def test_band_sponge_targets_local_to_the_rim():
     d = np.tile(dist, len(levels))
     assert np.abs(t[d >= 5]).max() <= 1e-12 * np.abs(t).max()
     assert np.abs(t[d < 5]).max() > 1e-3
+
+
+def test_adjoint_features_are_forward_features_in_swapped_units():
+    """An adjoint state's features divide the velocities (not the stresses)
+    by Z0: an adjoint state (v Z0, s / Z0) has the features of the forward
+    state (v, s), and the scales differ by nothing."""
+    nz, nx = 36, 44
+    m1, m2 = _rand_levels(nz, nx, 11)
+    coefs, _ = _coefs(nz, nx, 12)
+    f, s = LA.features(m1, m2, coefs)
+
+    def swap(lev):
+        return [a * LA.Z0 if fi < 2 else a / LA.Z0 for fi, a in enumerate(lev)]
+
+    fa, sa = LA.features(swap(m1), swap(m2), coefs, adjoint=True)
+    assert np.allclose(sa, s, rtol=1e-14, atol=0)
+    assert np.all(np.abs(fa - f) <= 2.0 ** -8 * np.abs(f))
+    # and the unit is not applied to a forward state's velocities
+    fw, _ = LA.features(swap(m1), swap(m2), coefs)
+    assert not np.allclose(fw, f)
+
+
+def test_adjoint_rtm_loop_matches_oracle_rtm():
+    """The per-level adjoint recursion of the learned-layer oracle (the
+    gather-form rtm_op3_c_ on three-level histories, then the separable
+    exact-adjoint sponge), with its imaging, reproduces the oracle's own RTM
+    images with and without the adjoint sponge."""
+    nz, nx, nt, dx, dt = 40, 52, 90, 10.0, 1e-3
+    vp = synth.layered_vp(nz - 16, nx - 16, 3, vp_top=2000.0, vp_bot=3000.0, nlayers=2, anom_sigma=4.0)
+    m = synth.elastic_from_vp(synth.pad_edge(vp, 8))
+    coefs = O.model_op(m.rho, m.lam, m.mu, dx, dt)
+    wav = O.ricker(nt, dt, 30.0, nsource=50)
+    rec = np.random.default_rng(4).standard_normal((5, nx - 16, nt))
+    for adj_abc in (0, 1):
+        p = O.make_params(nz, nx, nt, dx, dt, 20, 12, 10, 8, nx - 16, 8, adj_abc=adj_abc)
+        got = LA.rtm_adjoint_layer(p, wav, coefs, rec)
+        ref = O.rtm(p, wav, coefs, rec)
+        for a, b in zip(got, ref):
+            assert O.rel_l2(a, b) <= 1e-12, adj_abc
+
+
+def test_adjoint_teacher_targets_are_the_rim_contribution():
+    """Adjoint teacher targets: the wide and band-grid updates share the
+    injection and level L+2, so with no band sponge the target is the wide
+    grid's adjoint stencil applied to the rim values of level L+1 alone
+    (zero beyond the stencil radius of the rim), in the adjoint's units.
+    (The band grid's own adjoint stencil reads nothing from its rim: the
+    gather form masks the coefficients of cells that the forward operators
+    never update.)"""
+    case = _small_case()
+    case["nt"] = 100
+    case["wavelet"] = case["wavelet"][:100]
+    levels = [30, 55]
+    f, t = LA.teacher_pairs_adj(levels=levels, zr_phys=2, **case)
+    pad_s = LA.R + LA.W
+    nzs, nxs = case["nz_phys"] + 2 * pad_s, case["nx_phys"] + 2 * pad_s
+    nzw, nxw = case["nz_phys"] + 2 * case["pad_wide"], case["nx_phys"] + 2 * case["pad_wide"]
+    off = case["pad_wide"] - pad_s
+    pw = O.make_params(nzw, nxw, case["nt"], 1.0, case["dt"], case["pad_wide"] + case["xs_phys"],
+                       case["pad_wide"] + case["zs_phys"], case["pad_wide"] + 2, case["pad_wide"],
+                       case["nx_phys"], case["abc_wide"])
+    rec = O.forward(pw, case["wavelet"], case["coefs_wide"])["rec"]
+    kz, kx = LA.sponge_axes(nzw, nxw, case["abc_wide"])
+    wide = LA.adjoint_run(case["coefs_wide"], kz, kx, rec, pw.zr, pw.rx0, case["nt"])
+    iz, ix, _, _ = LA.band_cells(nzs, nxs)
+    ones_z, ones_x = np.ones(nzw), np.ones(nxw)
+    zero = [np.zeros((nzw, nxw)) for _ in range(5)]
+    rows = []
+    for L in sorted(levels, reverse=True):  # the stacking order
+        rim = []
+        for a in wide[L + 1]:
+            w = np.zeros((nzw, nxw))
+            w[off:off + nzs, off:off + nxs] = a[off:off + nzs, off:off + nxs]
+            w[off + LA.R:off + nzs - LA.R, off + LA.R:off + nxs - LA.R] = 0
+            rim.append(w)
+        cw = LA.adjoint_level(zero, rim, zero, case["coefs_wide"], ones_z, ones_x)
+        contrib = [a[off:off + nzs, off:off + nxs] for a in cw]
+        _, s = LA.features(LA.window(wide[L + 1], off, nzs, nxs), LA.window(wide[L + 2], off, nzs, nxs),
+                           case["coefs_small"], adjoint=True)
+        rows.append(np.stack([np.where(s > 0, contrib[fi][iz, ix] / np.where(s > 0, s * (LA.Z0 if fi < 2 else 1.0),
+                                                                             1.0), 0.0) for fi in range(5)], 1))
+    want = np.concatenate(rows)
+    assert np.abs(want).max() > 1e-3  # the adjoint wavefield reached the rim
+    assert np.allclose(t, want, rtol=1e-9, atol=1e-12 * np.abs(want).max())
+    dist = np.minimum(np.minimum(iz, nzs - 1 - iz), np.minimum(ix, nxs - 1 - ix)) - LA.R
+    assert np.all(t[np.tile(dist, len(levels)) >= 5] == 0.0)
