@@ -423,6 +423,15 @@ __global__ void k_pack_infer(const float* __restrict__ w, unsigned char* __restr
   if (i < NO) f[H * H + H * NO + 2 * H + i] = w[OB3 + i];
 }
 
+// LDS written by lanes of a wave and read by other lanes of the same wave:
+// a wave's LDS operations complete in order, so only the compiler has to be
+// kept from moving them across this point.
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 // ADJ: an adjoint step's correction (features from levels L+1, L+2, the
 // adjoint's field scaling, no history).
 template <bool ADJ>
@@ -444,6 +453,9 @@ __global__ __launch_bounds__(256) void k_infer_mfma(const unsigned char* __restr
     return;
   }
   __shared__ __align__(16) unsigned char sWp[PBYTES];  // the packed weights
+  // features and hidden layers: each wave uses only its own [wv] rows, so
+  // after the weights are in place the waves of a workgroup run their
+  // batches without workgroup barriers
   __shared__ __bf16 sX[4][MC][KXP];
   __shared__ float sH[4][MC][H];
   const int t = threadIdx.x, wv = t >> 6, lane = t & 63;
@@ -519,7 +531,7 @@ __global__ __launch_bounds__(256) void k_infer_mfma(const unsigned char* __restr
   }
   for (int k = NRAW + part; k < KX; k += 4)
     sX[wv][cl][k] = to_bf16(bf16r(live ? static_feature(edge, depth, cp, cs, zp, k) : 0.f));
-  __syncthreads();
+  wave_sync();
 
   // layer 1 on MFMA: Z1[16 x 32] = X[16 x 256] W1[256 x 32], two 16-column tiles
   const int r16 = lane & 15, kq = (lane >> 4) * 8;
@@ -536,7 +548,7 @@ __global__ __launch_bounds__(256) void k_infer_mfma(const unsigned char* __restr
 #pragma unroll
     for (int i = 0; i < 4; ++i) sH[wv][(lane >> 4) * 4 + i][nt * 16 + r16] = gelu(acc[i] + sB[nt * 16 + r16]);
   }
-  __syncthreads();
+  wave_sync();
   // layer 2 on FP32 MFMA: Z2[16 x 32] = H1[16 x 32] W2[32 x 32] in 8 steps
   // of 4 (lane: A row r16, B column r16, k = lane / 16); the wave's own
   // rows of sH are then overwritten with the hidden layer 2 (its LDS
@@ -584,7 +596,7 @@ __global__ __launch_bounds__(256) void k_infer_mfma(const unsigned char* __restr
       }
     }
   }
-  __syncthreads();  // sX and sH are rewritten by the next batch
+  wave_sync();  // sX and sH are rewritten by the next batch
   }
 }
 
