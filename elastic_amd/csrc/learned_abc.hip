@@ -366,11 +366,15 @@ __global__ void k_sample(int* idx, int batch, int n, unsigned seed) {
 // layer 1 (245 -> 32, 90% of the arithmetic) runs as 16 x 16 x 32 BF16
 // matrix products with FP32 accumulation (the features and weights are
 // BF16-rounded already, so the products are exact and only the summation
-// order differs from a scalar loop), layers 2 and 3 run in FP32 on the
-// vector ALUs from LDS. Weights are read once per block (64 cells).
+// order differs from a scalar loop), layer 2 on FP32 matrix cores and
+// layer 3 in FP32 from LDS. A workgroup of 12 waves (the weights copied to
+// LDS once, 148 KB of LDS: one workgroup per CU, 3 waves per SIMD) loops
+// over batches of 192 cells.
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 constexpr int MC = 16;        // cells per wave (rows of the matrix product)
+constexpr int IW = 12;        // waves per inference workgroup (one per CU: 3 per SIMD)
+constexpr int IT = 64 * IW;   // its threads
 constexpr int KX = 256;       // features padded to 8 K-steps of 32
 constexpr int KXP = KX + 8;   // LDS row pitch (bf16): rows 16 B apart in the banks
 
@@ -435,15 +439,15 @@ __device__ __forceinline__ void wave_sync() {
 // ADJ: an adjoint step's correction (features from levels L+1, L+2, the
 // adjoint's field scaling, no history).
 template <bool ADJ>
-__global__ __launch_bounds__(256) void k_infer_mfma(const unsigned char* __restrict__ packed,
+__global__ __launch_bounds__(IT) void k_infer_mfma(const unsigned char* __restrict__ packed,
                                                     const ShotDev* __restrict__ shots,
                                                     const LCell* __restrict__ cells, int nband, int nrim, int L,
                                                     int hist, int nbw) {
-  // workgroups 0 .. nbw-1 loop over the batches of 64 band cells (the
+  // workgroups 0 .. nbw-1 loop over the batches of IW * MC band cells (the
   // weights are copied to LDS once per workgroup); the rest clear the rim
-  const int nbb = (nband + 4 * MC - 1) / (4 * MC);
+  const int nbb = (nband + IW * MC - 1) / (IW * MC);
   if ((int)blockIdx.x >= nbw) {  // rim blocks
-    const int i = ((int)blockIdx.x - nbw) * 256 + (int)threadIdx.x;
+    const int i = ((int)blockIdx.x - nbw) * IT + (int)threadIdx.x;
     if (i < nrim) {
       const LCell c = cells[nband + i];
       const ShotDev& S = shots[c.shot];
@@ -456,10 +460,10 @@ __global__ __launch_bounds__(256) void k_infer_mfma(const unsigned char* __restr
   // features and hidden layers: each wave uses only its own [wv] rows, so
   // after the weights are in place the waves of a workgroup run their
   // batches without workgroup barriers
-  __shared__ __bf16 sX[4][MC][KXP];
-  __shared__ float sH[4][MC][H];
+  __shared__ __bf16 sX[IW][MC][KXP];
+  __shared__ float sH[IW][MC][H];
   const int t = threadIdx.x, wv = t >> 6, lane = t & 63;
-  for (int i = t; i < PBYTES / 16; i += 256)
+  for (int i = t; i < PBYTES / 16; i += IT)
     reinterpret_cast<uint4*>(sWp)[i] = reinterpret_cast<const uint4*>(packed)[i];
   const __bf16(*sW1)[KXP] = reinterpret_cast<const __bf16(*)[KXP]>(sWp);
   const float* sW2 = reinterpret_cast<const float*>(sWp + PW1);
@@ -469,7 +473,7 @@ __global__ __launch_bounds__(256) void k_infer_mfma(const unsigned char* __restr
   for (int bb = (int)blockIdx.x; bb < nbb; bb += nbw) {
   // features: 4 lanes per cell
   const int cl = lane >> 2, part = lane & 3;
-  const int c = bb * 4 * MC + wv * MC + cl;
+  const int c = bb * IW * MC + wv * MC + cl;
   const bool in = c < nband;  // uniform over the cell's 4 lanes
   LCell cc = in ? cells[c] : LCell{0, R, R, kLDead};
   const bool live = in && cc.ed != kLDead;
@@ -663,27 +667,27 @@ int launch_labc_infer(const unsigned char* w, const ShotDev* shots, const LCell*
                       bool hist, hipStream_t st, bool adj) {
   using namespace labc;
   // band workgroups: one round of resident workgroups, each looping over
-  // batches of 64 cells
+  // batches of IW * MC cells
   // (queried once per process, thread-safe; a failed query falls back to
-  // 2 workgroups on each of 256 CUs, which only changes how the batches are
-  // spread)
+  // one workgroup on each of 256 CUs, which only changes how the batches
+  // are spread)
   static const int resident = [] {
     int dev = 0, cus = 0, per_cu = 0;
     if (hipGetDevice(&dev) != hipSuccess ||
         hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
-        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_infer_mfma<false>, 256, 0) != hipSuccess ||
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_infer_mfma<false>, IT, 0) != hipSuccess ||
         cus * per_cu <= 0)
-      return 512;
+      return 256;
     return cus * per_cu;
   }();
-  const int nbb = (nband + 4 * MC - 1) / (4 * MC);
+  const int nbb = (nband + IW * MC - 1) / (IW * MC);
   const int nbw = std::min(nbb, resident);
-  const int blocks = nbw + (nrim + 255) / 256;
+  const int blocks = nbw + (nrim + IT - 1) / IT;
   if (blocks == 0) return OK;
   if (adj)
-    hipLaunchKernelGGL(k_infer_mfma<true>, dim3(blocks), dim3(256), 0, st, w, shots, cells, nband, nrim, L, 0, nbw);
+    hipLaunchKernelGGL(k_infer_mfma<true>, dim3(blocks), dim3(IT), 0, st, w, shots, cells, nband, nrim, L, 0, nbw);
   else
-    hipLaunchKernelGGL(k_infer_mfma<false>, dim3(blocks), dim3(256), 0, st, w, shots, cells, nband, nrim, L,
+    hipLaunchKernelGGL(k_infer_mfma<false>, dim3(blocks), dim3(IT), 0, st, w, shots, cells, nband, nrim, L,
                        (int)hist, nbw);
   HIPCHK(hipGetLastError());
   return OK;
