@@ -960,11 +960,30 @@ struct LGrid {
   Mat mat() const { return Mat{coef.as<float>(), (int)n2, ldz}; }
 };
 
+// The levels of a collection: each in [lo, hi], none twice (the collectors
+// walk the levels in order and take each listed one once).
+static int check_levels(const int* levels, int nlev, int lo, int hi) {
+  std::vector<int> v(levels, levels + nlev);
+  std::sort(v.begin(), v.end());
+  for (size_t i = 0; i < v.size(); ++i) {
+    if (v[i] < lo || v[i] > hi) {
+      set_error("level %d not in [%d, %d]", v[i], lo, hi);
+      return E_ARG;
+    }
+    if (i > 0 && v[i] == v[i - 1]) {
+      set_error("level %d listed twice", v[i]);
+      return E_ARG;
+    }
+  }
+  return OK;
+}
+
 RTM_EXPORT int labc_collect(labc_data* d, int nz_phys, int nx_phys, int pad_wide, int abc_wide, int nt,
                             float dt, int zs_phys, int xs_phys, const float* coef5_wide,
                             const float* coef5_small, const float* wavelet, const int* levels, int nlev,
                             float noise, unsigned seed, int abc_band) {
   if (!d || !coef5_wide || !coef5_small || !wavelet || !levels || nlev < 1) { set_error("bad argument"); return E_ARG; }
+  STCHK(check_levels(levels, nlev, 2, nt));
   const int pad_s = R + W;
   if (pad_wide < pad_s + 5) { set_error("wide padding must exceed the band padding"); return E_ARG; }
   const int nzw = nz_phys + 2 * pad_wide, nxw = nx_phys + 2 * pad_wide;
@@ -1041,6 +1060,7 @@ RTM_EXPORT int labc_collect_adj(labc_data* d, int nz_phys, int nx_phys, int pad_
     set_error("bad argument");
     return E_ARG;
   }
+  STCHK(check_levels(levels, nlev, 1, nt));
   const int pad_s = R + W;
   if (pad_wide < pad_s + 5) { set_error("wide padding must exceed the band padding"); return E_ARG; }
   if (abc_band < 0 || abc_band > W) { set_error("band sponge width %d not in [0, %d]", abc_band, W); return E_ARG; }

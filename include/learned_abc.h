@@ -96,7 +96,9 @@ int labc_data_get(labc_data *d, int row0, int n, float *feat, float *tgt);
  * abc_band > 0 (<= LABC_WIDTH) trains the layer to work with a sponge of that
  * width in the band: the band grid's update applies it, and the wide grid
  * applies the same factors at the band's position (continued outwards), so
- * the target is the exterior contribution of a damped wavefield. */
+ * the target is the exterior contribution of a damped wavefield. The levels
+ * must lie in [2, nt] (level 2 gives no pair), each listed once
+ * (RTM_E_ARG otherwise). */
 int labc_collect(labc_data *d, int nz_phys, int nx_phys, int pad_wide, int abc_wide, int nt,
                  float dt, int zs_phys, int xs_phys, const float *coef5_wide,
                  const float *coef5_small, const float *wavelet, const int *levels, int nlev,
@@ -110,7 +112,8 @@ int labc_collect(labc_data *d, int nz_phys, int nx_phys, int pad_wide, int abc_w
  * are those of labc_features_adj; the target is wide(L) - the band grid's
  * adjoint update(L) (the same injected data, the exact adjoint of the band
  * sponge when abc_band > 0) on the band cells, divided by the cell scale
- * (velocities also by the impedance scale). */
+ * (velocities also by the impedance scale). The levels must lie in [1, nt],
+ * each listed once (RTM_E_ARG otherwise). */
 int labc_collect_adj(labc_data *d, int nz_phys, int nx_phys, int pad_wide, int abc_wide, int nt,
                      float dt, int zs_phys, int xs_phys, int zr_phys, const float *coef5_wide,
                      const float *coef5_small, const float *wavelet, const int *levels, int nlev,

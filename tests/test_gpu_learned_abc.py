@@ -147,6 +147,27 @@ def test_collect_adj_vs_oracle(L, band):
     d.close()
 
 
+def test_collect_rejects_bad_level_lists(L):
+    """A level outside the run or listed twice is an error, not a silently
+    shorter training set."""
+    from elastic_amd.lib import RtmError
+
+    case = _case(nt=60)
+    d = L.Data(100000)
+    for levels in ([30, 30], [1, 30], [30, 61]):
+        with pytest.raises(RtmError):
+            d.collect(levels=levels, **case)
+    for levels in ([0, 30], [30, 61], [20, 20]):
+        with pytest.raises(RtmError):
+            d.collect_adj(levels=levels, zr_phys=2, **case)
+    assert len(d) == 0
+    d.collect(levels=[2, 30, 60], **case)
+    d.collect_adj(levels=[1, 30, 60], zr_phys=2, **case)
+    n = L.cell_count(case["nz_phys"] + 2 * L.PAD, case["nx_phys"] + 2 * L.PAD)
+    assert len(d) == 2 * n + 3 * n  # level 2 gives no forward pair
+    d.close()
+
+
 def test_training_reduces_loss(L):
     case = _case(nt=200)
     d = L.Data(200000)
