@@ -470,12 +470,19 @@ __global__ __launch_bounds__(IT) void k_infer_mfma(const unsigned char* __restri
   const float* sW3 = sW2 + H * H;
   const float* sB = sW3 + H * NO;
   __syncthreads();
-  for (int bb = (int)blockIdx.x; bb < nbb; bb += nbw) {
-  // features: 4 lanes per cell
+  // features: 4 lanes per cell; each batch's cell entry is loaded during
+  // the batch before it
   const int cl = lane >> 2, part = lane & 3;
+  auto cell_of = [&](int bb) {
+    const int c = bb * IW * MC + wv * MC + cl;
+    return c < nband ? cells[c] : LCell{0, R, R, kLDead};
+  };
+  LCell cnext = cell_of((int)blockIdx.x);
+  for (int bb = (int)blockIdx.x; bb < nbb; bb += nbw) {
   const int c = bb * IW * MC + wv * MC + cl;
   const bool in = c < nband;  // uniform over the cell's 4 lanes
-  LCell cc = in ? cells[c] : LCell{0, R, R, kLDead};
+  const LCell cc = cnext;
+  cnext = cell_of(bb + nbw < nbb ? bb + nbw : bb);
   const bool live = in && cc.ed != kLDead;
   // a wave's 16 cells belong to one grid (the table pads every grid to
   // whole waves), so its descriptor comes from scalar loads
