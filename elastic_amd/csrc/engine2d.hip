@@ -405,10 +405,14 @@ int Engine::build_lcells() {
   return OK;
 }
 
+// A failed cell table (grids too small, no memory) drops both networks, so
+// that no later run launches the inference without its table.
 int Engine::set_learned(const unsigned char* w) {
   destroy_graphs();  // their forward loops change
   cfg_.labc_w = w;
-  return build_lcells();
+  const int rc = build_lcells();
+  if (rc != OK) cfg_.labc_w = cfg_.labc_wa = nullptr;
+  return rc;
 }
 
 int Engine::set_learned_adjoint(const unsigned char* w) {
@@ -418,7 +422,9 @@ int Engine::set_learned_adjoint(const unsigned char* w) {
   }
   destroy_graphs();  // their adjoint loops change
   cfg_.labc_wa = w;
-  return build_lcells();
+  const int rc = build_lcells();
+  if (rc != OK) cfg_.labc_w = cfg_.labc_wa = nullptr;
+  return rc;
 }
 
 // Descriptors of group g (variant 0: migration coefficients; 1: true

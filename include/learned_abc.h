@@ -144,7 +144,8 @@ double labc_last_forward_ms(void);
  * trained with; the adjoint keeps the batch's adjoint sponge option. NULL
  * drops the layer. The network must outlive the batch's runs. Every grid
  * must be at least 2 * (LABC_RIM + LABC_WIDTH) + 1 cells on each side
- * (RTM_E_ARG otherwise). */
+ * (RTM_E_ARG otherwise; after an error the batch has no learned layer in
+ * either direction). */
 int rtm_batch_set_learned(rtm_batch *b, labc_net *net);
 /* The learned layer in the batch's adjoint runs (no counterpart in the
  * reference): a network trained on adjoint states (labc_collect_adj)

@@ -385,6 +385,30 @@ def test_batch_engine_adjoint_layer_vs_oracle(L, hip):
         assert np.array_equal(a, c)
 
 
+def test_batch_refuses_layer_on_small_grids(L, hip):
+    """Grids below 27 cells a side have no room for the band's patches: the
+    batch refuses either network and keeps running without a layer."""
+    nt, dx, dt = 80, 10.0, 1e-3
+    vp = synth.layered_vp(20, 24, 3, vp_top=2000.0, vp_bot=2600.0, nlayers=2, anom_sigma=4.0)
+    m = synth.elastic_from_vp(vp)
+    coefs = hip.model_op(m.rho, m.lam, m.mu, dx, dt)
+    spec = dict(nz=20, nx=24, xs=12, zs=6, zr=3, rx0=0, nrx=24)
+    w = hip.ricker(nt, dt, 30.0, nsource=40)
+    b = hip.Batch(nt, dx, dt, [spec], [coefs], w, abc=0, adj_abc=0, txr=2)
+    b.set_data(0, np.ones((5, 24, nt), np.float32))
+    b.migrate()
+    plain = b.image(0)
+    net = L.Net(seed=1)
+    for setter in (b.set_learned, b.set_learned_adjoint):
+        with pytest.raises(hip.RtmError):
+            setter(net)
+    b.migrate()
+    for a, c in zip(b.image(0), plain):
+        assert np.array_equal(a, c)
+    b.close()
+    net.close()
+
+
 def test_learned_experiment_small(L, hip):
     """The end-to-end experiment (training on wide-sponge teachers, held-out
     reflection and RTM image errors for the three boundaries) at a small
