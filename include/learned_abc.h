@@ -141,7 +141,8 @@ double labc_last_forward_ms(void);
  * level over the group's shots, the network's first layer on BF16 matrix
  * cores. The shots' grids should be padded by LABC_RIM + LABC_WIDTH cells
  * and the batch's sponge width (cfg abc) be the band sponge the network was
- * trained with; the adjoint keeps the batch's adjoint sponge option. NULL
+ * trained with; the adjoint keeps the batch's adjoint sponge option (and
+ * takes a network of its own through rtm_batch_set_learned_adjoint). NULL
  * drops the layer. The network must outlive the batch's runs. Every grid
  * must be at least 2 * (LABC_RIM + LABC_WIDTH) + 1 cells on each side
  * (RTM_E_ARG otherwise; after an error the batch has no learned layer in
