@@ -112,7 +112,7 @@ struct Lev {
 __device__ inline float raw_feature(const Lev& m1, const Lev& m2, int iz, int ix, int edge, int depth,
                                     int k) {
   if (k < 225) {
-    const int kn = k / 25, rem = k % 25, m = rem / 5 - 2, f = rem % 5;
+    const int fm = k / 9, kn = k % 9, f = fm / 5, m = fm % 5 - 2;
     const int dz = edge == 0 ? 1 : edge == 1 ? -1 : 0;
     const int dx = edge == 2 ? 1 : edge == 3 ? -1 : 0;
     const int tz = (edge >= 2) ? 1 : 0, tx = (edge < 2) ? 1 : 0;
@@ -520,7 +520,7 @@ __global__ __launch_bounds__(IT) void k_infer_mfma(const unsigned char* __restri
     const int k = part + 4 * u;
     float v = 0.f;
     if (k < 225) {  // field f of level L-1 (L+1) at (kn inward, mm - 2 along the edge)
-      const int kn = k / 25, rem = k - 25 * kn, mi = rem / 5, f = rem - 5 * mi;
+      const int fm = k / 9, kn = k - 9 * fm, f = fm / 5, mi = fm - 5 * f;
       v = gldv(l1, f * fs + base + kn * nstep + (mi - 2) * tstep);
       if (z_scaled(f, ADJ)) v = v * (1.f / Z0);
     } else if (k < NRAW) {  // the cell's own fields of level L-2 (L+2)
@@ -636,10 +636,12 @@ void labc_cell_table(const std::vector<std::pair<int, int>>& grids, std::vector<
   using namespace labc;
   cells.clear();
   // The inference kernel gives 16 consecutive cells to a wave, which loads
-  // their patches feature by feature: top and bottom band cells are listed
-  // column by column, the 8 depths of a column in a row (two columns per
-  // wave, one cache line each per feature), left and right band cells row
-  // by row at each depth (16 consecutive rows per wave).
+  // their patches 4 features at a time (one per lane of a cell; consecutive
+  // features are consecutive inward steps): top and bottom band cells are
+  // listed column by column, the 8 depths of a column in a row (two columns
+  // per wave; the 8 cells' 4 inward steps span 11 consecutive rows), left
+  // and right band cells row by row at each depth (16 consecutive rows per
+  // wave).
   for (int s = 0; s < (int)grids.size(); ++s) {
     const int nz = grids[s].first, nx = grids[s].second;
     // every feature patch of a band cell lies inside the grid from this size

@@ -11,8 +11,10 @@
  * Features of a band cell (LABC_NFEAT = 245, padded to LABC_KPAD = 248):
  *   0..224   the 5 fields of level L-1 on a 9 x 5 patch: 9 cells from the
  *            cell inwards along the edge normal (slot kn) times 5 tangential
- *            offsets m = -2..2, feature kn*25 + (m+2)*5 + field; stresses
- *            are divided by LABC_Z0 (an impedance scale, Pa / (m/s));
+ *            offsets m = -2..2, feature (field*5 + m+2)*9 + kn (the inward
+ *            steps of one field and offset are consecutive features);
+ *            stresses are divided by LABC_Z0 (an impedance scale,
+ *            Pa / (m/s));
  *   225..229 the cell's own 5 fields of level L-2 (same scaling);
  *   230..233 one-hot edge (top, bottom, left, right);
  *   234..241 one-hot depth in the band;

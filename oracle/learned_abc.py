@@ -115,7 +115,7 @@ def raw_patch(lev_m1, lev_m2, cells, dtype=np.float64, adjoint=False):
                 u = _field_unit(f, dtype, adjoint)
                 if u is not None:
                     v = (v * u).astype(dtype)
-                out[:, kn * 25 + (m + 2) * 5 + f] = v
+                out[:, (f * 5 + m + 2) * 9 + kn] = v
     for f in range(5):
         v = np.asarray(lev_m2[f], dtype)[iz, ix]
         u = _field_unit(f, dtype, adjoint)

@@ -160,7 +160,7 @@ def test_features_scaling_and_layout():
     c = np.nonzero(edge == 2)[0][5]
     for kn, m, fi in ((0, 0, 0), (8, -2, 3), (3, 2, 4)):
         v = m1[fi][iz[c] + m, ix[c] + kn] / (LA.Z0 if fi >= 2 else 1.0)
-        assert f[c, kn * 25 + (m + 2) * 5 + fi] == LA.bf16(np.float32([v / s[c]]))[0]
+        assert f[c, (fi * 5 + m + 2) * 9 + kn] == LA.bf16(np.float32([v / s[c]]))[0]
     # own cell of level L-2
     assert f[c, 227] == LA.bf16(np.float32([m2[2][iz[c], ix[c]] / LA.Z0 / s[c]]))[0]
     # float32 emulation agrees with the exact definition within one bf16 step
