@@ -1,7 +1,9 @@
 """GPU tests of the learned absorbing layer (elastic_amd/csrc/learned_abc.hip)
-against the numpy oracle (oracle/learned_abc.py): band geometry, features,
-network forward and gradient, teacher-forced pairs from a wide-sponge run,
-training, and propagation with the layer."""
+against the numpy oracle (oracle/learned_abc.py): band geometry, features
+(forward and adjoint states), network forward and gradient, teacher-forced
+pairs from wide-sponge runs (forward and adjoint), training, propagation
+with the layer, and the layer in the batch engine's forward and adjoint
+runs."""
 import numpy as np
 import pytest
 
