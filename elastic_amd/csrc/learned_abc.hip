@@ -1055,8 +1055,10 @@ RTM_EXPORT int labc_collect(labc_data* d, int nz_phys, int nx_phys, int pad_wide
 
 // Teacher-forced pairs of the adjoint (receiver-side) propagation, the
 // counterpart of labc_collect: the wide grid models receiver data at depth
-// zr_phys (every physical column), then back-propagates them with the exact
-// adjoint of its sponge (continued by the band sponge, as in labc_collect);
+// zr_phys (every physical column; with coef5_wide_obs, the data of that
+// model minus those of coef5_wide, as a migration injects them), then
+// back-propagates them with the exact adjoint of its sponge (continued by
+// the band sponge, as in labc_collect);
 // at each listed level L the band grid's levels L+1, L+2 are windows of the
 // wide run, the features are those of an adjoint state, and the target is
 // the wide run's level L minus the band grid's adjoint update (its band
@@ -1064,7 +1066,7 @@ RTM_EXPORT int labc_collect(labc_data* d, int nz_phys, int nx_phys, int pad_wide
 RTM_EXPORT int labc_collect_adj(labc_data* d, int nz_phys, int nx_phys, int pad_wide, int abc_wide, int nt,
                                 float dt, int zs_phys, int xs_phys, int zr_phys, const float* coef5_wide,
                                 const float* coef5_small, const float* wavelet, const int* levels, int nlev,
-                                float noise, unsigned seed, int abc_band) {
+                                float noise, unsigned seed, int abc_band, const float* coef5_wide_obs) {
   if (!d || !coef5_wide || !coef5_small || !wavelet || !levels || nlev < 1 || zr_phys < 0 || zr_phys >= nz_phys) {
     set_error("bad argument");
     return E_ARG;
@@ -1091,8 +1093,16 @@ RTM_EXPORT int labc_collect_adj(labc_data* d, int nz_phys, int nx_phys, int pad_
   STCHK(gs.setup(nzs, nxs, nt, dt, pad_s + zs_phys, pad_s + xs_phys, coef5_small, wavelet, abc_band,
                  pad_s + zr_phys, pad_s, nx_phys, rec.as<float>()));
   const bool kw = abc_wide > 0 || abc_band > 0;
-  // the data: one forward run of the wide grid
-  for (int L = 2; L <= nt; ++L) STCHK(gw.step(L, (L >= 3 && kw ? F_K : 0) | F_SRC | F_REC, nullptr));
+  // the data: one forward run of the wide grid (of the observed model, then
+  // subtracting the run of coef5_wide)
+  if (coef5_wide_obs) {
+    LGrid go;
+    STCHK(go.setup(nzw, nxw, nt, dt, pad_wide + zs_phys, pad_wide + xs_phys, coef5_wide_obs, wavelet, abc_wide,
+                   pad_wide + zr_phys, pad_wide, nx_phys, rec.as<float>(), abc_band, off, nzs, nxs));
+    for (int L = 2; L <= nt; ++L) STCHK(go.step(L, (L >= 3 && kw ? F_K : 0) | F_SRC | F_REC, nullptr));
+  }
+  const int recf = coef5_wide_obs ? F_RECSUB : F_REC;
+  for (int L = 2; L <= nt; ++L) STCHK(gw.step(L, (L >= 3 && kw ? F_K : 0) | F_SRC | recf, nullptr));
   STCHK(launch_fill(gw.ring.as<float>(), 20 * gw.n2, 0.f, nullptr));
   DevBuf scales;
   STCHK(scales.alloc((size_t)bd.total * 4, "scales"));

@@ -155,16 +155,19 @@ class Data:
                                C.c_int(abc_band)))
 
     def collect_adj(self, nz_phys, nx_phys, pad_wide, abc_wide, nt, dt, zs_phys, xs_phys, zr_phys, coefs_wide,
-                    coefs_small, wavelet, levels, noise=0.0, seed=0, abc_band=0):
+                    coefs_small, wavelet, levels, noise=0.0, seed=0, abc_band=0, coefs_wide_obs=None):
         """Teacher-forced pairs of the adjoint propagation of one wide-sponge
-        run's receiver data (see labc_collect_adj)."""
+        run's receiver data, or with coefs_wide_obs of the data of that model
+        minus those of coefs_wide (see labc_collect_adj)."""
         cw, cs = pack_fields(coefs_wide), pack_fields(coefs_small)
+        co = None if coefs_wide_obs is None else pack_fields(coefs_wide_obs)
         w = np.ascontiguousarray(wavelet, np.float32)
         lv = np.ascontiguousarray(levels, np.int32)
         _chk(_L().labc_collect_adj(self.h, C.c_int(nz_phys), C.c_int(nx_phys), C.c_int(pad_wide),
                                    C.c_int(abc_wide), C.c_int(nt), C.c_float(dt), C.c_int(zs_phys),
                                    C.c_int(xs_phys), C.c_int(zr_phys), _p(cw), _p(cs), _p(w), _p(lv),
-                                   C.c_int(lv.size), C.c_float(noise), C.c_uint(seed), C.c_int(abc_band)))
+                                   C.c_int(lv.size), C.c_float(noise), C.c_uint(seed), C.c_int(abc_band),
+                                   _p(co)))
 
 
 def propagate(net, s: _lib.RtmShot, coef5, wavelet, want_last=False):
@@ -216,7 +219,7 @@ def _rel(a, b):
 
 def experiment(nz=400, nx=800, dx=10.0, dt=1e-3, nt=2500, abc_wide=120, band_abc=8, train_models=5,
                shots_per_model=2, levels_per_shot=200, test_models=2, steps=(4000, 2000, 2000), batch=8192,
-               noise=0.03, seed=20260301, adjoint_layer=True, log=None):
+               noise=0.03, seed=20260301, adjoint_layer=True, adjoint_residual=False, log=None):
     """The learned layer end to end on nz x nx grids (all on the GPU):
 
     * training: `train_models` seeded random layered models (Vp 1500-4500
@@ -226,7 +229,9 @@ def experiment(nz=400, nx=800, dx=10.0, dt=1e-3, nt=2500, abc_wide=120, band_abc
       train_models * shots_per_model * levels_per_shot snapshot pairs),
       noise augmentation, Adam in three learning-rate stages; with
       adjoint_layer, a second network trained the same way on the adjoint
-      runs of the same shots' receiver data (labc_collect_adj);
+      runs of the same shots' receiver data (labc_collect_adj; with
+      adjoint_residual, through the smoothed model injecting the residual
+      data of a migration instead);
     * reflection energy: on held-out models, the physical-region wavefield
       of the last two levels and the receiver data of an 8-cell band grid
       with the band sponge alone, with the sponge and the network, and of a
@@ -253,6 +258,11 @@ def experiment(nz=400, nx=800, dx=10.0, dt=1e-3, nt=2500, abc_wide=120, band_abc
     for mi in range(train_models):
         vp = random_layered_vp(nz, nx, rng)
         cw, cs = _coefs(vp, pad_w, dx, dt), _coefs(vp, PAD, dx, dt)
+        # the migration model of the adjoint teachers (smoothed, as in the
+        # held-out RTM below)
+        vps = synth.smooth(vp, 10.0) if adjoint_residual else None
+        cws = _coefs(vps, pad_w, dx, dt) if adjoint_residual else cw
+        css = _coefs(vps, PAD, dx, dt) if adjoint_residual else cs
         for si in range(shots_per_model):
             fp = float(rng.uniform(8.0, 25.0))
             zs, xs = int(rng.integers(2, nz // 3)), int(rng.integers(0, nx))
@@ -261,7 +271,7 @@ def experiment(nz=400, nx=800, dx=10.0, dt=1e-3, nt=2500, abc_wide=120, band_abc
             d.collect(nz, nx, pad_w, abc_wide, nt, dt, zs, xs, cw, cs, wav, lv, noise=noise,
                       seed=int(seed + 31 * mi + si), abc_band=band_abc)
             if adjoint_layer:
-                train_shots.append((cw, cs, zs, xs, wav, int(seed + 31 * mi + si)))
+                train_shots.append((cw, cws, css, zs, xs, wav, int(seed + 31 * mi + si)))
     say(f"learned: {len(d)} training rows from {nshot_train * levels_per_shot} snapshot pairs, "
         f"{_time() - t0:.1f}s")
 
@@ -279,14 +289,15 @@ def experiment(nz=400, nx=800, dx=10.0, dt=1e-3, nt=2500, abc_wide=120, band_abc
     net_adj, adj_train = None, None
     if adjoint_layer:
         # the adjoint network: pairs of the adjoint runs of the same shots
-        # (the data recorded on the wide grid), levels spread over the
-        # adjoint's range, the set built after the forward one is freed
+        # (the recorded data, or with adjoint_residual the residual data a
+        # migration injects through the smoothed model), levels spread over
+        # the adjoint's range, the set built after the forward one is freed
         da = Data(ncell * levels_per_shot * nshot_train)
         t0 = _time()
-        for cw, cs, zs, xs, wav, sd in train_shots:
+        for cw, cws, css, zs, xs, wav, sd in train_shots:
             lv = np.linspace(1, nt - 150, levels_per_shot).round().astype(int)
-            da.collect_adj(nz, nx, pad_w, abc_wide, nt, dt, zs, xs, zr, cw, cs, wav, lv, noise=noise, seed=sd,
-                           abc_band=band_abc)
+            da.collect_adj(nz, nx, pad_w, abc_wide, nt, dt, zs, xs, zr, cws, css, wav, lv, noise=noise, seed=sd,
+                           abc_band=band_abc, coefs_wide_obs=cw if adjoint_residual else None)
         collect_s = _time() - t0
         net_adj, a0, a1, a_s = fit(da, int(seed % 100000) + 1)
         da.close()

@@ -115,11 +115,14 @@ int labc_collect(labc_data *d, int nz_phys, int nx_phys, int pad_wide, int abc_w
  * adjoint update(L) (the same injected data, the exact adjoint of the band
  * sponge when abc_band > 0) on the band cells, divided by the cell scale
  * (velocities also by the impedance scale). The levels must lie in [1, nt],
- * each listed once (RTM_E_ARG otherwise). */
+ * each listed once (RTM_E_ARG otherwise). coef5_wide_obs (NULL: none)
+ * makes the injected data those of a migration: the wide run of that
+ * (observed) model minus the wide run of coef5_wide, the model of the
+ * adjoint runs. */
 int labc_collect_adj(labc_data *d, int nz_phys, int nx_phys, int pad_wide, int abc_wide, int nt,
                      float dt, int zs_phys, int xs_phys, int zr_phys, const float *coef5_wide,
                      const float *coef5_small, const float *wavelet, const int *levels, int nlev,
-                     float noise, unsigned seed, int abc_band);
+                     float noise, unsigned seed, int abc_band, const float *coef5_wide_obs);
 /* Adam on random minibatches: loss per step in losses[steps]. */
 int labc_train(labc_net *net, labc_data *d, int steps, int batch, float lr, unsigned seed,
                float *losses);

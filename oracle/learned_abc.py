@@ -366,7 +366,7 @@ def rtm_adjoint_layer(p, wavelet, coefs, rec, params=None):
 
 
 def teacher_pairs_adj(nz_phys, nx_phys, pad_wide, abc_wide, nt, dt, zs_phys, xs_phys, zr_phys, coefs_wide,
-                      coefs_small, wavelet, levels, abc_band=0):
+                      coefs_small, wavelet, levels, abc_band=0, coefs_wide_obs=None):
     """FP64 (features, targets) of the adjoint propagation, stacked in
     decreasing level order like labc_collect_adj: the wide grid's forward run
     (sponges as in teacher_pairs) records receivers on row zr_phys of every
@@ -374,7 +374,9 @@ def teacher_pairs_adj(nz_phys, nx_phys, pad_wide, abc_wide, nt, dt, zs_phys, xs_
     adjoint) back-propagates them; at each listed level L the band grid's
     levels L+1, L+2 are windows of it, the features those of an adjoint
     state, the target (wide(L) - band-grid adjoint update(L)) / (s * unit)
-    with the velocities' unit Z0."""
+    with the velocities' unit Z0. With coefs_wide_obs the injected data
+    are that model's wide run minus the run of coefs_wide (the model of the
+    adjoint runs), as in a migration."""
     pad_s = R + W
     nzw, nxw = nz_phys + 2 * pad_wide, nx_phys + 2 * pad_wide
     nzs, nxs = nz_phys + 2 * pad_s, nx_phys + 2 * pad_s
@@ -388,6 +390,9 @@ def teacher_pairs_adj(nz_phys, nx_phys, pad_wide, abc_wide, nt, dt, zs_phys, xs_
     hist = _forward_custom_k(pw, np.asarray(wavelet, np.float64), coefs_wide, kzw, kxw, nt)
     zrw = pad_wide + zr_phys
     rec = np.stack([h[zrw, pad_wide:pad_wide + nx_phys, :] for h in hist])
+    if coefs_wide_obs is not None:
+        ho = _forward_custom_k(pw, np.asarray(wavelet, np.float64), coefs_wide_obs, kzw, kxw, nt)
+        rec = np.stack([h[zrw, pad_wide:pad_wide + nx_phys, :] for h in ho]) - rec
     wide = adjoint_run(coefs_wide, kzw, kxw, rec, zrw, pad_wide, nt)
     cells = band_cells(nzs, nxs)
     iz, ix = cells[0], cells[1]
