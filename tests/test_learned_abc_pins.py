@@ -358,3 +358,37 @@ def test_adjoint_teacher_targets_are_the_rim_contribution():
     assert np.allclose(t, want, rtol=1e-9, atol=1e-12 * np.abs(want).max())
     dist = np.minimum(np.minimum(iz, nzs - 1 - iz), np.minimum(ix, nxs - 1 - ix)) - LA.R
     assert np.all(t[np.tile(dist, len(levels)) >= 5] == 0.0)
+
+
+def test_adjoint_teacher_residual_data():
+    """With an observed model equal to the adjoint's model the residual data
+    vanish, and with it every feature and target; the residual of a model
+    with an anomaly equals the difference of the two models' recorded data
+    (the injected data enter linearly: pairs of obs-data minus pairs of the
+    model's own data, before normalisation, cancel to the residual's)."""
+    case = _small_case()
+    case["nt"] = 80
+    case["wavelet"] = case["wavelet"][:80]
+    levels = [20, 50]
+    f0, t0 = LA.teacher_pairs_adj(levels=levels, zr_phys=2, coefs_wide_obs=case["coefs_wide"], **case)
+    assert np.all(f0[:, :LA.NRAW] == 0) and np.all(t0 == 0)
+    # the injected data: obs minus the model's own data, as recorded
+    nzw, nxw = case["nz_phys"] + 2 * case["pad_wide"], case["nx_phys"] + 2 * case["pad_wide"]
+    vp = np.full((case["nz_phys"], case["nx_phys"]), 2500.0)
+    vp[8:14, 10:18] = 3000.0
+    mo = synth.elastic_from_vp(synth.pad_edge(vp, case["pad_wide"]))
+    co = O.model_op(mo.rho, mo.lam, mo.mu, 10.0, case["dt"])
+    f1, t1 = LA.teacher_pairs_adj(levels=levels, zr_phys=2, coefs_wide_obs=co, **case)
+    assert np.abs(f1[:, :LA.NRAW]).max() > 0
+    pw = O.make_params(nzw, nxw, case["nt"], 1.0, case["dt"], case["pad_wide"] + case["xs_phys"],
+                       case["pad_wide"] + case["zs_phys"], case["pad_wide"] + 2, case["pad_wide"], case["nx_phys"],
+                       case["abc_wide"])
+    r_own = O.forward(pw, case["wavelet"], case["coefs_wide"])["rec"]
+    r_obs = O.forward(pw, case["wavelet"], co)["rec"]
+    kz, kx = LA.sponge_axes(nzw, nxw, case["abc_wide"])
+    lev = LA.adjoint_run(case["coefs_wide"], kz, kx, r_obs - r_own, pw.zr, pw.rx0, case["nt"])
+    off = case["pad_wide"] - (LA.R + LA.W)
+    nzs, nxs = case["nz_phys"] + 2 * (LA.R + LA.W), case["nx_phys"] + 2 * (LA.R + LA.W)
+    fa, _ = LA.features(LA.window(lev[51], off, nzs, nxs), LA.window(lev[52], off, nzs, nxs), case["coefs_small"],
+                        adjoint=True)
+    assert np.array_equal(f1[:fa.shape[0]], fa)  # level 50 comes first (decreasing order)
