@@ -538,10 +538,12 @@ __global__ __launch_bounds__(IT) void k_infer_mfma(const unsigned char* __restri
 #pragma unroll
   for (int u = 0; u < NV; ++u) {
     const int k = part + 4 * u;
-    if (k < NRAW) sX[wv][cl][k] = to_bf16(bf16r(rv[u] * inv));
+    // the hardware conversion (round to nearest even, as bf16r) in one
+    // instruction
+    if (k < NRAW) sX[wv][cl][k] = (__bf16)(rv[u] * inv);
   }
   for (int k = NRAW + part; k < KX; k += 4)
-    sX[wv][cl][k] = to_bf16(bf16r(live ? static_feature(edge, depth, cp, cs, zp, k) : 0.f));
+    sX[wv][cl][k] = (__bf16)(live ? static_feature(edge, depth, cp, cs, zp, k) : 0.f);
   wave_sync();
 
   // layer 1 on MFMA: Z1[16 x 32] = X[16 x 256] W1[256 x 32], two 16-column tiles
